@@ -1,0 +1,249 @@
+/*
+ * pixeltrack.h - C ABI of the MI355X (gfx950) pixel-track stage.
+ *
+ * The library replaces the pixel-track stage of the serial Julia Patatrack
+ * port: CA doublets -> cell connection -> fishbone -> ntuplets -> multiplicity
+ * -> broken-line fit -> track classification.  Every entry point takes plain
+ * pointers and sizes; index conventions are the reference's:
+ *   - stored hit ids are 1-based (gpu_pixel_doublets.jl:214, hist_to_container.jl:242)
+ *   - module (detector) indices are 0-based (gpuPixelRecHits.jl:287)
+ *   - hits_layer_start holds 0-based hit offsets (PixelRecHits.jl:20-27)
+ *   - the tuple container off[] holds 1-based start positions (hist_to_container.jl:141,159)
+ *   - count-filled containers hold exclusive offsets (hist_to_container.jl:201-212)
+ *
+ * Status: every function returns PT_OK (0) or a negative PT_ERR_* code;
+ * pt_last_error() gives a message.  Capacity overflows are NOT errors: they
+ * truncate exactly like the reference and set bits in pt_counters.overflow.
+ * Inputs on which the reference would stop with an AssertionError /
+ * InexactError / DomainError are reported in pt_counters.ref_assert.
+ */
+#ifndef PIXELTRACK_H
+#define PIXELTRACK_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- capacities (ca_constants.jl:10-29, PixelTrackHeterogeneous.jl:55-60,
+ *      gpu_clustering_constants.jl:14, ca_hit_ntuplet_generator.jl:12) ---- */
+#define PT_MAX_HITS            49152u  /* MAX_NUMBER_OF_HITS                     */
+#define PT_MAX_DOUBLETS        458752u /* maxNumberOfDoublets                    */
+#define PT_MAX_CELLS_PER_HIT   128u    /* OuterHitOfCell capacity                */
+#define PT_MAX_NEIGHBORS       64u     /* CellNeighbors capacity                 */
+#define PT_MAX_CELL_TRACKS     64u     /* CellTracks capacity                    */
+#define PT_MAX_ACTIVE_LISTS    65536u  /* MAX_NUM_OF_ACTIVE_DOUBLETS (columns)   */
+#define PT_MAX_TUPLES          32768u  /* TrackSOA stride / HitContainer bins    */
+#define PT_MAX_TUPLE_HITS      163840u /* HitContainer capacity                  */
+#define PT_MAX_FITS            24576u  /* MAX_NUM_OF_CONCURRENT_FITS per length  */
+#define PT_N_LAYERS            10u
+#define PT_N_PHI_BINS          128u
+#define PT_HIST_NOFF           1281u   /* 10 layers x 128 bins + 1               */
+#define PT_N_MODULES           1856u   /* phase1PixelTopology.jl:23              */
+#define PT_N_LAYER_PAIRS       19u
+
+/* ---- status codes ---- */
+#define PT_OK                  0
+#define PT_ERR_ARG            (-1)
+#define PT_ERR_HIP            (-2)
+#define PT_ERR_UNSUPPORTED    (-3)
+#define PT_ERR_CAPACITY       (-4)
+
+/* ---- pt_counters.overflow bits (silent truncations in the reference) ---- */
+#define PT_OVF_DOUBLETS        0x001u /* > max_num_of_doublets cells             */
+#define PT_OVF_OUTER_HIT_CELLS 0x002u /* a hit is the outer hit of > 128 cells   */
+#define PT_OVF_NEIGHBORS       0x004u /* a cell has > 64 outer neighbours        */
+#define PT_OVF_NEIGHBOR_LISTS  0x008u /* > 65535 neighbour lists (extend!)       */
+#define PT_OVF_CELL_TRACKS     0x010u /* a cell is on > 64 tuples                */
+#define PT_OVF_TRACK_LISTS     0x020u /* > 65535 cell-track lists (extend!)      */
+#define PT_OVF_TUPLES          0x040u /* > 32768 tuples                          */
+#define PT_OVF_FITS            0x080u /* a length bucket holds > 24576 tuples    */
+#define PT_OVF_HIT_TO_TUPLE    0x100u /* hit->tuple association full             */
+
+/* ---- pt_counters.ref_assert bits (the reference would throw here) ---- */
+#define PT_REF_TOO_MANY_HITS   0x001u /* n_hits > 49152 (ca_hit_ntuplet_generator_kernels.jl:173) */
+#define PT_REF_HIST_LAYER      0x002u /* histogram bin holds a hit of another layer (gpu_pixel_doublets.jl:318-319) */
+#define PT_REF_SIZE_CUT_INF    0x004u /* Int(Inf/NaN) in z_size_cut (gpu_pixel_doublets.jl:87) */
+#define PT_REF_DEPTH           0x008u /* > 4 doublets in a chain (gpu_ca_cell.jl:234) */
+#define PT_REF_NO_CELLS        0x010u /* zero doublets (ca_hit_ntuplet_generator_kernels_impl.jl:145) */
+#define PT_REF_FIT_NAN         0x020u /* NaN fast fit (broken_line_fit_on_gpu.jl:89-92) */
+#define PT_REF_FIT_CHARGE      0x040u /* q*d > 0 (broken_line.jl:444) */
+#define PT_REF_SQRT_DOMAIN     0x080u /* sqrt of a negative number in the fit (Julia DomainError) */
+#define PT_REF_BAD_MODULE      0x100u /* B1 hit with module >= 96 (gpu_pixel_doublets.jl:231) */
+#define PT_REF_MULT            0x200u /* tuple with >= 8 hits (ca_hit_ntuplet_generator_kernels_impl.jl:189) */
+
+/* Quality enum (track_quality.jl:129-136) */
+#define PT_QUALITY_BAD         0
+#define PT_QUALITY_DUP         1
+#define PT_QUALITY_LOOSE       2
+
+/* Algorithm parameters: field-for-field mirror of Params / QualityCuts
+ * (ca_hit_ntuplet_generator_kernels.jl:31-87), defaults from
+ * ca_hit_ntuplet_generator.jl:36-58 via pt_params_default(). */
+typedef struct pt_params {
+    uint32_t min_hits_per_ntuplet;             /* 3      */
+    uint32_t max_num_of_doublets;              /* 458752 */
+    uint8_t  use_riemann_fit;                  /* 0 (1 is unsupported: the reference only prints) */
+    uint8_t  fit_5_as_4;                       /* 1 (0 is unsupported: the reference throws)       */
+    uint8_t  include_jumping_forward_doublets; /* 1      */
+    uint8_t  early_fish_bone;                  /* 1      */
+    uint8_t  late_fish_bone;                   /* 0      */
+    uint8_t  ideal_conditions;                 /* 1      */
+    uint8_t  do_stats;                         /* 0      */
+    uint8_t  do_cluster_cut;                   /* 1      */
+    uint8_t  do_z0_cut;                        /* 1      */
+    uint8_t  do_pt_cut;                        /* 1      */
+    uint8_t  pad_[2];
+    float    pt_min;                           /* 0.9f   */
+    float    ca_theta_cut_barrel;              /* 0.002f */
+    float    ca_theta_cut_forward;             /* 0.003f */
+    float    hard_curv_cut;                    /* 0.0328407224959f */
+    float    dca_cut_inner_triplet;            /* 0.15f  */
+    float    dca_cut_outer_triplet;            /* 0.25f  */
+    /* QualityCuts */
+    float    chi2_coeff[4];                    /* 0.68177776, 0.74609577, -0.08035491, 0.00315399 */
+    float    chi2_max_pt;                      /* 10     */
+    float    chi2_scale;                       /* 30     */
+    float    triplet_max_tip, triplet_min_pt, triplet_max_zip;          /* 0.3, 0.5, 12 */
+    float    quadruplet_max_tip, quadruplet_min_pt, quadruplet_max_zip; /* 0.5, 0.3, 12 */
+} pt_params;
+
+/* Hits of one event: TrackingRecHit2DSOAView (TrackingRecHit2DSOAView.jl:35-54)
+ * plus the phi histogram `Hist` (TrackingRecHit2DHeterogeneous.jl:28) and the
+ * layer starts.  Arrays have n_hits entries (hist_off has 1281). */
+typedef struct pt_hits_view {
+    const float    *xg, *yg, *zg, *rg;   /* global position (cm), rg = sqrt(xg^2+yg^2) in f32 */
+    const float    *xerr, *yerr;         /* LOCAL variances (gpuPixelRecHits.jl:303-306)      */
+    const int16_t  *iphi;                /* unsafe_atan2s(yg, xg, 7)                          */
+    const int16_t  *ysize;               /* cluster size y                                    */
+    const int16_t  *det_ind;             /* 0-based module index                              */
+    const uint32_t *hist_off;            /* [1281] exclusive bin offsets                      */
+    const uint16_t *hist_bins;           /* [n_hits] 1-based hit ids, descending in a bin     */
+    uint32_t        n_hits;
+    uint32_t        layer_start[11];     /* host copy of hits_layer_start (0-based)           */
+} pt_hits_view;
+
+/* Output: TrackSOA = TrackSOAT{32768,163840} (PixelTrackHeterogeneous.jl:10-33,59).
+ * state and covariance are column-major (one column of 32768 per component,
+ * TrajectoryStateSOA.jl:8-19).  Fields the reference leaves uninitialised
+ * (quality of absent tuples, pt/eta/chi2 of unfitted tuples) are not written. */
+typedef struct pt_track_soa {
+    int32_t  *quality;     /* [32768]                                   */
+    float    *chi2;        /* [32768]                                   */
+    float    *eta;         /* [32768]                                   */
+    float    *pt;          /* [32768]                                   */
+    float    *state;       /* [5][32768]  (phi, tip, 1/pt signed, cot(theta), zip) */
+    float    *covariance;  /* [15][32768]                               */
+    uint32_t *hit_off;     /* [32769] 1-based starts                    */
+    uint16_t *hit_bins;    /* [163840] 1-based hit ids                  */
+    uint32_t *det_off;     /* [32769]                                   */
+    uint16_t *det_bins;    /* [163840] module ids                       */
+} pt_track_soa;
+
+/* Per-event counters.  Sizes feed the algorithmic byte model. */
+typedef struct pt_counters {
+    uint32_t n_hits;
+    uint32_t n_inner_slots;     /* sum over layer pairs of inner-layer sizes       */
+    uint32_t n_cells;           /* doublets kept                                   */
+    uint32_t n_cells_found;     /* doublets passing the cuts before the cap        */
+    uint32_t n_connections;     /* neighbour-list entries stored                   */
+    uint32_t n_connect_pairs;   /* (cell, neighbour) pairs evaluated in connect    */
+    uint32_t n_killed;          /* cells killed by the fishbone                    */
+    uint32_t n_tuples;          /* tuples found (may exceed 32768)                 */
+    uint32_t n_tuple_hits;      /* hits over all found tuples                      */
+    uint32_t n_by_len[8];       /* multiplicity bucket sizes, index = hit count    */
+    uint32_t n_fitted;          /* tracks fitted                                   */
+    uint32_t n_loose;           /* tracks with quality loose after classification  */
+    uint32_t overflow;          /* PT_OVF_* bits                                   */
+    uint32_t ref_assert;        /* PT_REF_* bits                                   */
+} pt_counters;
+
+/* Internal CA state of one event, for the stage-level parity checks.  Arrays
+ * are caller-allocated with the given capacities; cell ids are 1-based.
+ * nbr_off/trk_off have n_cells+1 entries, ohc_off has n_hits+1 entries. */
+typedef struct pt_ca_debug {
+    uint32_t  cap_cells, cap_nbr, cap_ohc, cap_trk;
+    uint32_t  n_cells;          /* out */
+    uint16_t *cell_inner;       /* [cap_cells] 1-based inner hit id  */
+    uint16_t *cell_outer;       /* [cap_cells] 1-based outer hit id  */
+    uint8_t  *cell_pair;        /* [cap_cells] 0-based layer pair    */
+    int32_t  *cell_doublet_id;  /* [cap_cells] 1-based, -1 = killed  */
+    uint8_t  *cell_used;        /* [cap_cells] the_used after connect */
+    uint32_t *nbr_off;          /* [cap_cells+1] outer neighbours, CSR */
+    uint32_t *nbr;              /* [cap_nbr]                          */
+    uint32_t *ohc_off;          /* [n_hits+1] is_outer_hit_of_cell    */
+    uint32_t *ohc;              /* [cap_ohc]                          */
+    uint32_t *trk_off;          /* [cap_cells+1] cell -> tuple ids    */
+    uint16_t *trk;              /* [cap_trk]                          */
+} pt_ca_debug;
+
+typedef struct pt_ctx pt_ctx;
+
+/* ---------------------------------------------------------------------------
+ * Context (one per Julia task / stream, mirrors the TaskLocalValue scratch of
+ * ca_hit_ntuplet_generator.jl:9).  max_events = events in flight per batch.
+ * ------------------------------------------------------------------------- */
+void        pt_params_default(pt_params *p);
+int         pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_events);
+int         pt_ctx_destroy(pt_ctx *ctx);
+const char *pt_last_error(void);
+int         pt_device_count(int *n);
+
+/* Module frames (CondFormats pixelCPEforGPU.jl:54-73 DetParams.frame.rot):
+ * rot[9*m + k] = (R11,R12,R13,R21,R22,R23,R31,R32,R33) of module m, host memory. */
+int pt_ctx_set_frames(pt_ctx *ctx, const float *rot, uint32_t n_modules);
+
+/* Drop-in for make_tuples (ca_hit_ntuplet_generator.jl:65-97 / CAHitNtuplet.produce,
+ * ca_hit_ntuplet.jl:10-16): host hits in, host TrackSOA out, synchronous.
+ * b_field is the `bf` argument (1/field in GeV, 0.0114256972711507).
+ * out_zeroed != 0 tells the library that out->state/covariance are already
+ * zero (Julia's TrackSOA() constructor zero-fills them); otherwise the
+ * library zero-fills them like the constructor does. */
+int pt_make_tuples(pt_ctx *ctx, const pt_hits_view *hits, double b_field,
+                   pt_track_soa *out, int out_zeroed, pt_counters *cnt);
+
+/* Same, for n events whose hit arrays and TrackSOAs are already in device
+ * memory.  Enqueued on the context's stream; returns before completion.
+ * Call pt_ctx_sync, then pt_get_counters. */
+int pt_make_tuples_device_async(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits,
+                                double b_field, const pt_track_soa *out, int out_zeroed);
+int pt_ctx_sync(pt_ctx *ctx);
+int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
+
+/* Internal CA state of event `event` of the last batch (device -> host copy). */
+int pt_get_ca_debug(pt_ctx *ctx, uint32_t event, pt_ca_debug *dbg);
+
+/* Stage timing of the last batch, milliseconds, measured with HIP events on
+ * the context stream when enabled (enable=1 before the batch). */
+#define PT_N_STAGES 16
+int pt_ctx_enable_timing(pt_ctx *ctx, int enable);
+int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages);
+
+/* ---------------------------------------------------------------------------
+ * Broken-line fit on pre-gathered tracks (BrokenLineFitOnGPU kernelBLFastFit +
+ * kernelBLFit, broken_line_fit_on_gpu.jl:24-163), for fit-only workloads.
+ * Layout, all device pointers, track-minor (coalesced):
+ *   hits[(c*5 + k)*n + t]   c = x,y,z          (float, cm)
+ *   ge  [(c*5 + k)*n + t]   c = xx,xy,yy,xz,yz,zz global covariance (float)
+ *   nhit[t]                 3, 4 or 5 (5 is fitted on its first 4 hits, fit5as4)
+ * Outputs (device): state[k*n + t] (5), cov[k*n + t] (15), pt/eta/chi2[t].
+ * ------------------------------------------------------------------------- */
+#define PT_FIT_FP64   0   /* plain double-precision fit, the reference's method        */
+#define PT_FIT_MIXED  1   /* float bulk, double pre-fit curvature and band solve        */
+int pt_fit_bl_device_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
+                           const uint8_t *nhit, double b_field, int precision,
+                           float *state, float *cov, float *pt, float *eta, float *chi2);
+
+/* Device memory helpers (so hosts without a device allocator can stage data). */
+int pt_malloc(pt_ctx *ctx, void **ptr, size_t bytes);
+int pt_free(pt_ctx *ctx, void *ptr);
+int pt_memcpy_h2d(pt_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pt_memcpy_d2h(pt_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pt_memset(pt_ctx *ctx, void *dst, int value, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIXELTRACK_H */

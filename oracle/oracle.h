@@ -1,0 +1,51 @@
+/* oracle.h - TEST INFRASTRUCTURE.  C ABI of the CPU oracle (a serial restatement
+ * of the reference Julia pixel-track stage).  Loaded by tests/ and by the
+ * cpu_baseline leg of bench.py only; the product never links it. */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+#include <stdint.h>
+#include "../include/pixeltrack.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void oracle_params_default(pt_params *p);
+
+/* make_tuples restated (ca_hit_ntuplet_generator.jl:65-97) on host memory.
+ * frames: rot[9*module]; stop_after: 0 = full stage, 1 = after doublets,
+ * 2 = after connect, 3 = after fishbone, 4 = after ntuplets/multiplicity
+ * (no fit, no classification).  out->state/covariance must be zeroed by the
+ * caller (TrackSOA() does the same). */
+int oracle_make_tuples(const pt_params *p, const pt_hits_view *hits, const float *frames, double b_field,
+                       pt_track_soa *out, pt_counters *cnt, pt_ca_debug *dbg, int stop_after);
+
+/* fill_many_from_vector (hist_to_container.jl:265-270) for the 10-layer phi histogram */
+int oracle_build_phi_hist(const int16_t *iphi, uint32_t n_hits, const uint32_t *layer_start, uint32_t *off,
+                          uint16_t *bins);
+
+/* FP64 broken-line fit of pre-gathered tracks; same input layout as
+ * pt_fit_bl_device_async.  Double outputs: out64[t*23 + k] = state[0..4],
+ * cov15[5..19], pt, eta, chi2.  flags[t] = oracle fit flags. */
+int oracle_fit_bl(uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
+                  double *out64, uint32_t *flags, int n_threads);
+
+/* One track with every intermediate (for the known-answer checks).
+ * hits: 3xN column-major doubles (x,y,z per hit), ge: 6xN column-major floats.
+ * Returns the number of doubles written to out (see tests/oracle_layout.py). */
+int oracle_bl_detail(int N, const double *hits, const float *ge, double B, double *out, int cap);
+
+/* Count the arithmetic of one fit of N hits: counts[0]=add/sub, [1]=mul,
+ * [2]=div, [3]=sqrt, [4]=transcendental calls, [5]=comparisons. */
+int oracle_fit_opcount(int N, const double *hits, const float *ge, double B, uint64_t *counts);
+
+/* Forward-mode derivative of L = chi2_line + chi2_circle (+ ||params - truth||^2
+ * if truth != NULL) with respect to (x, y, z, sigma_rphi, sigma_z) of each of
+ * the N hits, through local variances rotated by the module frames rot[9*k].
+ * grad[5*k + c]; value[0] = L. */
+int oracle_fit_grad(int N, const double *xyz, const double *sig, const float *rot, double B, const double *truth,
+                    double *value, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
