@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the pixel-track stage (CA doublets -> connect -> fishbone ->
+ntuplets -> broken-line fit) on one MI355X.
+
+One step = one pass of the stage over a batch of PU50-like synthetic events
+resident in HBM (default 100 events, seeds 1..100: 1500 helices from 50
+vertices, pT power law above 0.3 GeV, |eta| < 2.5, +5% noise hits).
+Prints ONE JSON line (see the keys below).  Per-kernel durations come from HIP
+events recorded on the library's stream after every kernel of the timed steps;
+the roofline entry is computed for the kernel with the largest total time.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cms-julia_amd"))
+
+METRIC = "events/s CA+fit at PU50 (1 GPU); FP32 fit+adjoint tracks/s vs FP64 fit @1e-4"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Flops of one FP64 broken-line fit (add/sub + mul + div + sqrt), counted by
+# running the fit restatement with an op-counting scalar; transcendental calls
+# are excluded.  N=3 and N=4 hits used.
+FIT_FLOPS = {3: None, 4: None}
+
+
+def kernel_bytes(name, c):
+    """Algorithmic HBM bytes of one launch over the batch (sum of per-event
+    figures).  c: dict of per-event counter arrays."""
+    H, S, D = c["n_hits"], c["n_inner_slots"], c["n_cells"]
+    C, T = c["n_connections"], np.minimum(c["n_tuples"], 32768)
+    Th = np.minimum(c["n_tuple_hits"], 163840)
+    F = c["n_fitted"]
+    hist = 1281 * 4 + 11 * 4
+    if name == "k_doublet_count":
+        # each inner/outer hit field read once (det, z, r, iphi, ysize = 14 B,
+        # histogram bin 2 B), histogram offsets, one count per inner slot
+        b = 16 * H + hist + 4 * S
+    elif name == "k_doublet_fill":
+        # same reads + slot offsets, writes 7 B per cell and one list count per hit
+        b = 16 * H + hist + 4 * S + 7 * D + 4 * H
+    elif name in ("k_connect_count", "k_connect_fill"):
+        # per cell: inner/outer id 4 B, list entry 4 B; per hit x,y,z,r,det 18 B;
+        # slot offsets; count: neighbour count + used flag, fill: neighbour ids
+        b = 18 * H + 4 * S + 8 * D + (5 * D if name == "k_connect_count" else 4 * C + 4 * D)
+    elif name == "k_fishbone":
+        b = 22 * H + 8 * D
+    elif name in ("k_dfs_count", "k_dfs_emit"):
+        b = 4 * D + 4 * C + 2 * D + (8 * D if name == "k_dfs_count" else 12 * T + 2 * Th + 16 * T)
+    elif name.startswith("k_fit_tuples"):
+        b = None
+    else:
+        b = None
+    return None if b is None else int(np.sum(b))
+
+
+def run_cpu_baseline(events, frames, budget_s=12.0, max_threads=16):
+    """Oracle (serial restatement of the reference stage, incl. classification)
+    on host threads, one event per thread at a time like the reference's
+    --numberOfStreams runner."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O  # the checker, used here only as the CPU baseline
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    # warm-up (excluded, main.jl:148-151)
+    O.make_tuples(events[0], frames, stop_after=0, debug=False)
+    # size the sample to about budget_s of CPU work
+    t0 = time.perf_counter()
+    for h in events[:4]:
+        O.make_tuples(h, frames, stop_after=0, debug=False)
+    per_event = (time.perf_counter() - t0) / 4
+    n_target = int(max(len(events), budget_s / max(per_event, 1e-4)))
+    work = [events[i % len(events)] for i in range(n_target)]
+    lock = threading.Lock()
+    idx = [0]
+
+    def worker():
+        while True:
+            with lock:
+                i = idx[0]
+                idx[0] += 1
+            if i >= len(work):
+                return
+            O.make_tuples(work[i], frames, stop_after=0, debug=False)
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    return dict(value=round(len(work) / wall, 2), unit="events/s", cores=threads, kind="port",
+                sample=f"{len(work)} PU50 events (the {len(events)} benchmark events cycled), oracle = serial C++ "
+                       f"restatement of the Julia stage incl. classification, {threads} threads, "
+                       f"{per_event * 1e3:.1f} ms/event single-thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--events", type=int, default=100)
+    ap.add_argument("--preset", default="pu50")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the pixel-track stage shards by event with no exchange; "
+              f"--gpus {args.gpus} is not supported in this build", file=sys.stderr)
+        sys.exit(2)
+
+    from pixeltrack import ca, generator as G
+
+    frames = G.frames()
+    seed0 = 1 if args.preset == "pu50" else 1001
+    events = [G.generate_event(args.preset, seed=seed0 + i) for i in range(args.events)]
+    print(f"[bench] generated {len(events)} {args.preset} events, mean hits "
+          f"{np.mean([e.n_hits for e in events]):.0f}", file=sys.stderr, flush=True)
+
+    ctx = ca.Context(max_events=len(events), frames=frames)
+    batch = ca.DeviceBatch(ctx, events)
+    for _ in range(args.warmup):
+        batch.run_async(out_zeroed=True)
+    ctx.sync()
+    counters = [ctx.counters(i) for i in range(len(events))]
+    cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0] if k != "n_by_len"}
+
+    ctx.enable_timing(True)
+    per_kernel = {}
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run_async(out_zeroed=True)
+        for k, v in ctx.stage_times().items():  # synchronises the step
+            per_kernel[k] = per_kernel.get(k, 0.0) + v
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = len(events) * args.steps / wall
+    avg = {k: v / args.steps for k, v in per_kernel.items()}
+    dom = max(avg, key=avg.get)
+    dom_ms = avg[dom]
+    roof = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None, traffic=None,
+                avg_launch_ms=round(dom_ms, 5))
+    b = kernel_bytes(dom, cnt)
+    if b is not None and dom_ms > 0:
+        ach = b / (dom_ms * 1e-3) / 1e9
+        roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b)
+    kernels_ms = {k: round(v, 4) for k, v in sorted(avg.items(), key=lambda kv: -kv[1])}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = run_cpu_baseline(events, frames, budget_s=args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+int16 CA, f64 fit",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.preset}: {len(events)} events/step (seeds {seed0}..{seed0 + len(events) - 1}), "
+                        f"mean {np.mean(cnt['n_hits']):.0f} hits, {np.mean(cnt['n_cells']):.0f} doublets, "
+                        f"{np.mean(cnt['n_tuples']):.0f} tuples, {np.mean(cnt['n_fitted']):.0f} fits per event",
+            "events_per_step": len(events),
+            "parallelism": "events batched in one launch sequence (blockIdx.y = event)",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_ms_per_step": kernels_ms,
+    }
+    print(json.dumps(line))
+    batch.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,552 @@
+// pt_capi.cpp - C ABI of the pixel-track library (include/pixeltrack.h):
+// contexts, the event-slot arena, descriptor upload and kernel sequencing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pixeltrack.h"
+#include "pt_device.hpp"
+
+namespace pt {
+void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32_t max_hits, uint32_t max_slots,
+               int zero_state, hipStream_t st, Marks *tm);
+void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm);
+void launch_fit_flat(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double B, int precision,
+                     float *state, float *cov, float *pt, float *eta, float *chi2, hipStream_t st);
+}  // namespace pt
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(PT_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                       \
+    } while (0)
+
+// gpu_pixel_doublets.jl:20-56
+const int kLP[38] = {0, 1, 0, 4, 0, 7, 1, 2, 1, 4, 1, 7, 4, 5, 7, 8, 2, 3, 2, 4,
+                     2, 7, 5, 6, 8, 9, 0, 2, 1, 3, 0, 5, 0, 8, 4, 6, 7, 9};
+const int16_t kPhi[19] = {522, 730, 730, 522, 626, 626, 522, 522, 626, 626, 626, 522, 522, 522, 522, 522, 522, 522, 522};
+const float kMinZ[19] = {-20., 0., -30., -22., 10., -30., -70., -70., -22., 15., -30, -70., -70., -20., -22., 0, -30., -70., -70.};
+const float kMaxZ[19] = {20., 30., 0., 22., 30., -10., 70., 70., 22., 30., -15., 70., 70., 20., 22., 30., 0., 70., 70.};
+const float kMaxR[19] = {20., 9., 9., 20., 7., 7., 5., 5., 20., 6., 6., 5., 5., 20., 20., 9., 9., 9., 9.};
+
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Carve one slot's scratch out of base; returns bytes used.
+size_t carve(char *base, pt::Scratch *s) {
+    using namespace pt;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char * {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    Scratch t;
+    t.slot_cnt = (uint32_t *)take((kMaxSlots + 1) * 4);
+    t.cell_inner = (uint16_t *)take(kMaxCells * 2);
+    t.cell_outer = (uint16_t *)take(kMaxCells * 2);
+    t.cell_pair = (uint8_t *)take(kMaxCells);
+    t.cell_used = (uint8_t *)take(kMaxCells);
+    t.cell_killed = (uint8_t *)take(kMaxCells);
+    t.ohc_cnt = (uint32_t *)take((kMaxHits + 1) * 4);
+    t.ohc_cursor = (uint32_t *)take(kMaxHits * 4);
+    t.ohc_list = (uint32_t *)take(kMaxCells * 4);
+    t.nn = (uint32_t *)take((kMaxCells + 1) * 4);
+    t.nbr = (uint32_t *)take((size_t)kNbrCap * 4);
+    t.tcnt = (uint32_t *)take((kMaxCells + 1) * 4);
+    t.hcnt = (uint32_t *)take((kMaxCells + 1) * 4);
+    t.tuple_cells = (uint32_t *)take(kMaxTuples * 4 * 4);
+    t.trk_cnt = (uint32_t *)take((kMaxCells + 1) * 4);
+    t.trk_cursor = (uint32_t *)take(kMaxCells * 4);
+    t.trk_list = (uint16_t *)take(kTrkCap * 2);
+    t.tm_off = (uint32_t *)take(16 * 4);
+    t.tm_bins = (uint16_t *)take(kMaxTuples * 2);
+    t.h2t_off = (uint32_t *)take((kMaxHits + 2) * 4);
+    t.h2t_bins = (uint16_t *)take(4u * 49152u * 2);
+    t.h2t_cursor = (uint32_t *)take(kMaxHits * 4);
+    t.misc = (uint32_t *)take(16 * 4);
+    if (s) *s = t;
+    return off;
+}
+
+struct Staging {
+    // device copies for the host-memory entry point (one event)
+    float *xg = nullptr, *yg, *zg, *rg, *xerr, *yerr;
+    int16_t *iphi, *ysize, *det;
+    uint32_t *hist_off;
+    uint16_t *hist_bins;
+    int32_t *quality;
+    float *chi2, *eta, *pt, *state, *cov;
+    uint32_t *hit_off, *det_off;
+    uint16_t *hit_bins, *det_bins;
+    void *block = nullptr;
+};
+
+}  // namespace
+
+struct pt_ctx {
+    int device = 0;
+    pt_params p;
+    uint32_t max_events = 0;
+    hipStream_t stream = nullptr;
+    float *d_frames = nullptr;
+    bool frames_set = false;
+    char *arena = nullptr;
+    size_t slot_bytes = 0;
+    std::vector<pt::Scratch> scratch;
+    pt_counters *d_cnt = nullptr;
+    pt_counters *h_cnt = nullptr;  // pinned
+    pt::EvDesc *d_evs = nullptr;
+    pt::EvDesc *h_evs = nullptr;  // pinned
+    Staging stg;
+    pt::Marks marks;
+    bool timing = false;
+    bool have_marks = false;
+    uint32_t last_n = 0;
+};
+
+extern "C" {
+
+const char *pt_last_error(void) { return g_err.c_str(); }
+
+void pt_params_default(pt_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->min_hits_per_ntuplet = 3;
+    p->max_num_of_doublets = 458752;
+    p->use_riemann_fit = 0;
+    p->fit_5_as_4 = 1;
+    p->include_jumping_forward_doublets = 1;
+    p->early_fish_bone = 1;
+    p->late_fish_bone = 0;
+    p->ideal_conditions = 1;
+    p->do_stats = 0;
+    p->do_cluster_cut = 1;
+    p->do_z0_cut = 1;
+    p->do_pt_cut = 1;
+    p->pt_min = (float)0.899999976158;
+    p->ca_theta_cut_barrel = (float)0.00200000009499;
+    p->ca_theta_cut_forward = (float)0.00300000002608;
+    p->hard_curv_cut = (float)0.0328407224959;
+    p->dca_cut_inner_triplet = (float)0.15000000596;
+    p->dca_cut_outer_triplet = (float)0.25;
+    p->chi2_coeff[0] = (float)0.68177776;
+    p->chi2_coeff[1] = (float)0.74609577;
+    p->chi2_coeff[2] = (float)-0.08035491;
+    p->chi2_coeff[3] = (float)0.00315399;
+    p->chi2_max_pt = 10.0f;
+    p->chi2_scale = 30.0f;
+    p->triplet_max_tip = 0.3f;
+    p->triplet_min_pt = 0.5f;
+    p->triplet_max_zip = 12.0f;
+    p->quadruplet_max_tip = 0.5f;
+    p->quadruplet_min_pt = 0.3f;
+    p->quadruplet_max_zip = 12.0f;
+}
+
+int pt_device_count(int *n) {
+    if (!n) return fail(PT_ERR_ARG, "null pointer");
+    HIP_OK(hipGetDeviceCount(n));
+    return PT_OK;
+}
+
+int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_events) {
+    if (!out) return fail(PT_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (max_events == 0 || max_events > 65535) return fail(PT_ERR_ARG, "max_events must be in 1..65535");
+    pt_params prm;
+    if (p)
+        prm = *p;
+    else
+        pt_params_default(&prm);
+    if (prm.use_riemann_fit) return fail(PT_ERR_UNSUPPORTED, "the Riemann fit is not wired in the reference");
+    if (!prm.fit_5_as_4) return fail(PT_ERR_UNSUPPORTED, "fit_5_as_4 = false throws in the reference");
+    if (prm.min_hits_per_ntuplet < 3 || prm.min_hits_per_ntuplet > 5)
+        return fail(PT_ERR_ARG, "min_hits_per_ntuplet must be 3..5");
+    if (prm.max_num_of_doublets == 0 || prm.max_num_of_doublets > PT_MAX_DOUBLETS)
+        return fail(PT_ERR_ARG, "max_num_of_doublets must be 1..%u", PT_MAX_DOUBLETS);
+    HIP_OK(hipSetDevice(device));
+    pt_ctx *c = new pt_ctx();
+    c->device = device;
+    c->p = prm;
+    c->max_events = max_events;
+    auto cleanup = [&](int rc) {
+        pt_ctx_destroy(c);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(PT_ERR_HIP, "hipStreamCreate failed"));
+    c->slot_bytes = carve(nullptr, nullptr);
+    const size_t arena = c->slot_bytes * max_events;
+    if (hipMalloc((void **)&c->arena, arena) != hipSuccess)
+        return cleanup(fail(PT_ERR_HIP, "hipMalloc of %zu bytes of event scratch failed", arena));
+    c->scratch.resize(max_events);
+    for (uint32_t i = 0; i < max_events; ++i) carve(c->arena + (size_t)i * c->slot_bytes, &c->scratch[i]);
+    if (hipMalloc((void **)&c->d_cnt, sizeof(pt_counters) * max_events) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_cnt, sizeof(pt_counters) * max_events, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&c->d_evs, sizeof(pt::EvDesc) * max_events) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_evs, sizeof(pt::EvDesc) * max_events, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&c->d_frames, sizeof(float) * 9 * PT_N_MODULES) != hipSuccess)
+        return cleanup(fail(PT_ERR_HIP, "context allocation failed"));
+    std::memset(c->h_cnt, 0, sizeof(pt_counters) * max_events);
+    for (int k = 0; k < pt::Marks::kMax; ++k)
+        if (hipEventCreate(&c->marks.ev[k]) != hipSuccess) return cleanup(fail(PT_ERR_HIP, "hipEventCreate failed"));
+    c->have_marks = true;
+    *out = c;
+    return PT_OK;
+}
+
+int pt_ctx_destroy(pt_ctx *c) {
+    if (!c) return PT_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->have_marks)
+        for (int k = 0; k < pt::Marks::kMax; ++k) (void)hipEventDestroy(c->marks.ev[k]);
+    (void)hipFree(c->arena);
+    (void)hipFree(c->d_cnt);
+    (void)hipHostFree(c->h_cnt);
+    (void)hipFree(c->d_evs);
+    (void)hipHostFree(c->h_evs);
+    (void)hipFree(c->d_frames);
+    (void)hipFree(c->stg.block);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PT_OK;
+}
+
+int pt_ctx_set_frames(pt_ctx *c, const float *rot, uint32_t n_modules) {
+    if (!c || !rot) return fail(PT_ERR_ARG, "null pointer");
+    if (n_modules != PT_N_MODULES) return fail(PT_ERR_ARG, "expected %u module frames", PT_N_MODULES);
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipMemcpy(c->d_frames, rot, sizeof(float) * 9 * PT_N_MODULES, hipMemcpyHostToDevice));
+    c->frames_set = true;
+    return PT_OK;
+}
+
+static int build_desc(pt_ctx *c, uint32_t i, const pt_hits_view &h, const pt_track_soa &o, uint32_t &n_slots,
+                      uint32_t &n_pairs) {
+    pt::EvDesc &e = c->h_evs[i];
+    std::memset(&e, 0, sizeof(e));
+    if (h.n_hits > PT_MAX_HITS)
+        return fail(PT_ERR_CAPACITY, "event %u has %u hits, above MAX_NUMBER_OF_HITS = %u", i, h.n_hits, PT_MAX_HITS);
+    if (h.layer_start[0] != 0 || h.layer_start[10] != h.n_hits)
+        return fail(PT_ERR_ARG, "event %u: layer_start must start at 0 and end at n_hits", i);
+    for (int L = 0; L < 10; ++L)
+        if (h.layer_start[L + 1] < h.layer_start[L]) return fail(PT_ERR_ARG, "event %u: layer_start not sorted", i);
+    if (h.n_hits && (!h.xg || !h.yg || !h.zg || !h.rg || !h.xerr || !h.yerr || !h.iphi || !h.ysize || !h.det_ind ||
+                     !h.hist_off || !h.hist_bins))
+        return fail(PT_ERR_ARG, "event %u: null hit array", i);
+    if (!o.quality || !o.chi2 || !o.eta || !o.pt || !o.state || !o.covariance || !o.hit_off || !o.hit_bins ||
+        !o.det_off || !o.det_bins)
+        return fail(PT_ERR_ARG, "event %u: null TrackSOA array", i);
+    e.xg = h.xg, e.yg = h.yg, e.zg = h.zg, e.rg = h.rg, e.xerr = h.xerr, e.yerr = h.yerr;
+    e.iphi = h.iphi, e.ysize = h.ysize, e.det = h.det_ind;
+    e.hist_off = h.hist_off, e.hist_bins = h.hist_bins;
+    e.n_hits = h.n_hits;
+    for (int L = 0; L < 11; ++L) e.layer_start[L] = h.layer_start[L];
+    n_pairs = 19;
+    if (!c->p.include_jumping_forward_doublets) n_pairs = 15;
+    if (c->p.min_hits_per_ntuplet > 3) n_pairs = 13;
+    e.n_pairs = n_pairs;
+    uint32_t cum = 0;
+    for (uint32_t p = 0; p < 20; ++p) {
+        if (p < n_pairs) cum += h.layer_start[kLP[2 * p] + 1] - h.layer_start[kLP[2 * p]];
+        e.slot_cum[p] = (p < n_pairs) ? cum : 0xFFFFFFFFu;
+    }
+    e.n_slots = (h.n_hits == 0) ? 0u : cum;
+    n_slots = e.n_slots;
+    e.quality = o.quality, e.chi2 = o.chi2, e.eta = o.eta, e.pt = o.pt, e.state = o.state, e.cov = o.covariance;
+    e.hit_off = o.hit_off, e.hit_bins = o.hit_bins, e.det_off = o.det_off, e.det_bins = o.det_bins;
+    e.cnt = c->d_cnt + i;
+    e.s = c->scratch[i];
+    return PT_OK;
+}
+
+static pt::Config make_config(const pt_ctx *c, double b_field) {
+    pt::Config cfg;
+    cfg.p = c->p;
+    cfg.B = (double)(float)b_field;  // HelixFitOnGPU(Float32(b_field)) then Float64
+    cfg.frames = c->d_frames;
+    for (int p = 0; p < 19; ++p) {
+        cfg.pairs.inner[p] = (int8_t)kLP[2 * p];
+        cfg.pairs.outer[p] = (int8_t)kLP[2 * p + 1];
+        cfg.pairs.phi_cut[p] = kPhi[p];
+        cfg.pairs.min_z[p] = kMinZ[p];
+        cfg.pairs.max_z[p] = kMaxZ[p];
+        cfg.pairs.max_r[p] = kMaxR[p];
+    }
+    return cfg;
+}
+
+static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_track_soa *outs, double b_field,
+                     int out_zeroed) {
+    if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
+    if (n == 0) return PT_OK;
+    if (n > c->max_events) return fail(PT_ERR_ARG, "%u events exceed the context's %u slots", n, c->max_events);
+    uint32_t max_hits = 0, max_slots = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t ns, np;
+        int rc = build_desc(c, i, hits[i], outs[i], ns, np);
+        if (rc) return rc;
+        max_hits = std::max(max_hits, hits[i].n_hits);
+        max_slots = std::max(max_slots, ns);
+    }
+    HIP_OK(hipMemcpyAsync(c->d_evs, c->h_evs, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, c->stream));
+    const pt::Config cfg = make_config(c, b_field);
+    pt::Marks *marks = c->timing ? &c->marks : nullptr;
+    if (marks) marks->n = 0;
+    pt::launch_ca(c->d_evs, n, cfg, max_hits, max_slots, out_zeroed ? 0 : 1, c->stream, marks);
+    pt::launch_fit_tuples(c->d_evs, n, cfg, c->stream, marks);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(pt_counters) * n, hipMemcpyDeviceToHost, c->stream));
+    c->last_n = n;
+    return PT_OK;
+}
+
+int pt_make_tuples_device_async(pt_ctx *c, uint32_t n_events, const pt_hits_view *hits, double b_field,
+                                const pt_track_soa *out, int out_zeroed) {
+    if (!c || (n_events && (!hits || !out))) return fail(PT_ERR_ARG, "null pointer");
+    HIP_OK(hipSetDevice(c->device));
+    return run_batch(c, n_events, hits, out, b_field, out_zeroed);
+}
+
+int pt_ctx_sync(pt_ctx *c) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_get_counters(pt_ctx *c, uint32_t event, pt_counters *cnt) {
+    if (!c || !cnt) return fail(PT_ERR_ARG, "null pointer");
+    if (event >= c->last_n) return fail(PT_ERR_ARG, "event %u not in the last batch (%u events)", event, c->last_n);
+    *cnt = c->h_cnt[event];
+    return PT_OK;
+}
+
+static int ensure_staging(pt_ctx *c) {
+    if (c->stg.block) return PT_OK;
+    Staging &s = c->stg;
+    const size_t H = PT_MAX_HITS, T = PT_MAX_TUPLES, M = PT_MAX_TUPLE_HITS;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += align256(b);
+        return o;
+    };
+    const size_t o_xg = take(6 * H * 4), o_i16 = take(3 * H * 2), o_ho = take(PT_HIST_NOFF * 4), o_hb = take(H * 2);
+    const size_t o_q = take(T * 4), o_f = take(3 * T * 4), o_st = take(5 * T * 4), o_cv = take(15 * T * 4);
+    const size_t o_off = take(2 * (T + 1) * 4), o_bins = take(2 * M * 2);
+    HIP_OK(hipMalloc(&s.block, off));
+    char *b = (char *)s.block;
+    float *f = (float *)(b + o_xg);
+    s.xg = f, s.yg = f + H, s.zg = f + 2 * H, s.rg = f + 3 * H, s.xerr = f + 4 * H, s.yerr = f + 5 * H;
+    int16_t *i16 = (int16_t *)(b + o_i16);
+    s.iphi = i16, s.ysize = i16 + H, s.det = i16 + 2 * H;
+    s.hist_off = (uint32_t *)(b + o_ho);
+    s.hist_bins = (uint16_t *)(b + o_hb);
+    s.quality = (int32_t *)(b + o_q);
+    float *g = (float *)(b + o_f);
+    s.chi2 = g, s.eta = g + T, s.pt = g + 2 * T;
+    s.state = (float *)(b + o_st);
+    s.cov = (float *)(b + o_cv);
+    uint32_t *u = (uint32_t *)(b + o_off);
+    s.hit_off = u, s.det_off = u + (T + 1);
+    uint16_t *u16 = (uint16_t *)(b + o_bins);
+    s.hit_bins = u16, s.det_bins = u16 + M;
+    return PT_OK;
+}
+
+int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track_soa *out, int out_zeroed,
+                   pt_counters *cnt) {
+    if (!c || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
+    HIP_OK(hipSetDevice(c->device));
+    if (hits->n_hits > PT_MAX_HITS)
+        return fail(PT_ERR_CAPACITY, "%u hits above MAX_NUMBER_OF_HITS = %u", hits->n_hits, PT_MAX_HITS);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    Staging &s = c->stg;
+    const size_t n = hits->n_hits;
+    hipStream_t st = c->stream;
+    if (n) {
+        HIP_OK(hipMemcpyAsync(s.xg, hits->xg, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.yg, hits->yg, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.zg, hits->zg, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.rg, hits->rg, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.xerr, hits->xerr, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.yerr, hits->yerr, n * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.iphi, hits->iphi, n * 2, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.ysize, hits->ysize, n * 2, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.det, hits->det_ind, n * 2, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(s.hist_bins, hits->hist_bins, n * 2, hipMemcpyHostToDevice, st));
+    }
+    if (hits->hist_off)
+        HIP_OK(hipMemcpyAsync(s.hist_off, hits->hist_off, PT_HIST_NOFF * 4, hipMemcpyHostToDevice, st));
+    pt_hits_view dv = *hits;
+    dv.xg = s.xg, dv.yg = s.yg, dv.zg = s.zg, dv.rg = s.rg, dv.xerr = s.xerr, dv.yerr = s.yerr;
+    dv.iphi = s.iphi, dv.ysize = s.ysize, dv.det_ind = s.det, dv.hist_off = s.hist_off, dv.hist_bins = s.hist_bins;
+    pt_track_soa dout;
+    dout.quality = s.quality, dout.chi2 = s.chi2, dout.eta = s.eta, dout.pt = s.pt, dout.state = s.state;
+    dout.covariance = s.cov, dout.hit_off = s.hit_off, dout.hit_bins = s.hit_bins, dout.det_off = s.det_off;
+    dout.det_bins = s.det_bins;
+    (void)out_zeroed;  // the device copy is always zero-filled; the full state/covariance is copied back
+    rc = run_batch(c, 1, &dv, &dout, b_field, 0);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(st));
+    const pt_counters &k = c->h_cnt[0];
+    const size_t T = PT_MAX_TUPLES;
+    const size_t m = std::min<size_t>(k.n_tuples, T);
+    const size_t nb = std::min<size_t>(k.n_tuple_hits, PT_MAX_TUPLE_HITS);
+    if (m) {
+        HIP_OK(hipMemcpyAsync(out->quality, s.quality, m * 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(out->chi2, s.chi2, m * 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(out->eta, s.eta, m * 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(out->pt, s.pt, m * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIP_OK(hipMemcpyAsync(out->state, s.state, 5 * T * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(out->covariance, s.cov, 15 * T * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(out->hit_off, s.hit_off, (T + 1) * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(out->det_off, s.det_off, (T + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (nb) {
+        HIP_OK(hipMemcpyAsync(out->hit_bins, s.hit_bins, nb * 2, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(out->det_bins, s.det_bins, nb * 2, hipMemcpyDeviceToHost, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    if (cnt) *cnt = k;
+    return PT_OK;
+}
+
+int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
+    if (!c || !d) return fail(PT_ERR_ARG, "null pointer");
+    if (event >= c->last_n) return fail(PT_ERR_ARG, "event %u not in the last batch", event);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const pt::EvDesc &e = c->h_evs[event];
+    const pt::Scratch &s = e.s;
+    const pt_counters &k = c->h_cnt[event];
+    const uint32_t nc = k.n_cells, nh = e.n_hits;
+    d->n_cells = nc;
+    if (nc > d->cap_cells) return fail(PT_ERR_CAPACITY, "debug capacity %u < %u cells", d->cap_cells, nc);
+    std::vector<uint16_t> u16(nc);
+    std::vector<uint8_t> u8(nc);
+    if (d->cell_inner) HIP_OK(hipMemcpy(d->cell_inner, s.cell_inner, nc * 2, hipMemcpyDeviceToHost));
+    if (d->cell_outer) HIP_OK(hipMemcpy(d->cell_outer, s.cell_outer, nc * 2, hipMemcpyDeviceToHost));
+    if (d->cell_pair) HIP_OK(hipMemcpy(d->cell_pair, s.cell_pair, nc, hipMemcpyDeviceToHost));
+    if (d->cell_used) HIP_OK(hipMemcpy(d->cell_used, s.cell_used, nc, hipMemcpyDeviceToHost));
+    if (d->cell_doublet_id) {
+        HIP_OK(hipMemcpy(u8.data(), s.cell_killed, nc, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < nc; ++i) d->cell_doublet_id[i] = u8[i] ? -1 : (int32_t)(i + 1);
+    }
+    auto csr = [&](const uint32_t *d_off, uint32_t n, uint32_t *h_off, auto *h_vals, const auto *d_vals,
+                   uint32_t cap, uint32_t limit, bool to1) -> int {
+        std::vector<uint32_t> off(n + 1);
+        HIP_OK(hipMemcpy(off.data(), d_off, (n + 1) * 4, hipMemcpyDeviceToHost));
+        using V = typename std::remove_pointer<decltype(h_vals)>::type;
+        const uint32_t total = off[n];
+        std::vector<V> vals(total);
+        if (total) HIP_OK(hipMemcpy(vals.data(), d_vals, (size_t)total * sizeof(V), hipMemcpyDeviceToHost));
+        uint32_t pos = 0;
+        h_off[0] = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t a = off[i], b = off[i + 1];
+            if (limit && b - a > limit) b = a + limit;
+            for (uint32_t q = a; q < b && pos < cap; ++q) h_vals[pos++] = (V)(vals[q] + (to1 ? 1 : 0));
+            h_off[i + 1] = pos;
+        }
+        return PT_OK;
+    };
+    int rc = PT_OK;
+    if (d->ohc_off && d->ohc) rc = csr(s.ohc_cnt, nh, d->ohc_off, d->ohc, s.ohc_list, d->cap_ohc, PT_MAX_CELLS_PER_HIT, true);
+    if (!rc && d->nbr_off && d->nbr) rc = csr(s.nn, nc, d->nbr_off, d->nbr, s.nbr, d->cap_nbr, 0, true);
+    if (!rc && d->trk_off && d->trk) rc = csr(s.trk_cnt, nc, d->trk_off, d->trk, s.trk_list, d->cap_trk, PT_MAX_CELL_TRACKS, false);
+    return rc;
+}
+
+int pt_ctx_enable_timing(pt_ctx *c, int enable) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    c->timing = enable != 0;
+    return PT_OK;
+}
+
+int pt_get_stage_times(pt_ctx *c, float *ms, const char **names, int *n_stages) {
+    if (!c || !ms || !n_stages) return fail(PT_ERR_ARG, "null pointer");
+    if (!c->timing) return fail(PT_ERR_ARG, "timing not enabled");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const int ns = std::min(c->marks.n - 1, *n_stages);
+    for (int k = 0; k < ns; ++k) {
+        float t = 0;
+        HIP_OK(hipEventElapsedTime(&t, c->marks.ev[k], c->marks.ev[k + 1]));
+        ms[k] = t;
+        if (names) names[k] = c->marks.name[k + 1];
+    }
+    *n_stages = ns < 0 ? 0 : ns;
+    return PT_OK;
+}
+
+int pt_fit_bl_device_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                           double b_field, int precision, float *state, float *cov, float *pt, float *eta,
+                           float *chi2) {
+    if (!c || (n && (!hits || !ge || !nhit || !state || !cov || !pt || !eta || !chi2)))
+        return fail(PT_ERR_ARG, "null pointer");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    HIP_OK(hipSetDevice(c->device));
+    pt::launch_fit_flat(n, hits, ge, nhit, (double)(float)b_field, precision, state, cov, pt, eta, chi2, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+int pt_malloc(pt_ctx *c, void **p, size_t bytes) {
+    if (!c || !p) return fail(PT_ERR_ARG, "null pointer");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipMalloc(p, bytes ? bytes : 1));
+    return PT_OK;
+}
+int pt_free(pt_ctx *c, void *p) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    HIP_OK(hipFree(p));
+    return PT_OK;
+}
+int pt_memcpy_h2d(pt_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+int pt_memcpy_d2h(pt_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+__global__ void k_fill_bytes(uint8_t *p, uint8_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+int pt_memset(pt_ctx *c, void *dst, int value, size_t bytes) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (!bytes) return PT_OK;
+    size_t blocks = std::min<size_t>((bytes + 255) / 256, 65535);
+    k_fill_bytes<<<(unsigned)blocks, 256, 0, c->stream>>>((uint8_t *)dst, (uint8_t)value, bytes);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+}  // extern "C"

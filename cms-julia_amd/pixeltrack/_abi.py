@@ -26,7 +26,7 @@ MAX_FITS = 24576
 N_LAYERS = 10
 HIST_NOFF = 1281
 N_MODULES = 1856
-N_STAGES = 16
+N_STAGES = 48
 
 PT_OK = 0
 PT_ERR_ARG = -1

@@ -1,0 +1,276 @@
+"""Host mirror of the reference's pixel-track producer interface over the HIP C ABI.
+
+Reference interface mirrored here:
+  CAHitNtuplet (EDProducer "CAHitNtupletCUDA")     plugin-PixelTriplets/ca_hit_ntuplet.jl:1-19
+  CAHitNtupletGeneratorOnGPU / make_tuples          plugin-PixelTriplets/ca_hit_ntuplet_generator.jl:35-97
+  Params / QualityCuts                              plugin-PixelTriplets/ca_hit_ntuplet_generator_kernels.jl:31-87
+
+All compute runs in libpixeltrack_hip.so on the GPU; there is no CPU fallback:
+a missing library or device raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .formats import HitsSoA, TrackSOA
+
+
+class ReferenceAssertion(AssertionError):
+    """The input is one on which the reference stops with an exception."""
+
+
+def default_params(**overrides):
+    lib = A.hip_lib()
+    p = A.PtParams()
+    lib.pt_params_default(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One pt_ctx: device scratch for `max_events` events in flight and a stream.
+
+    Mirrors the per-task CAHitNTupletGeneratorKernels cache (TaskLocalValue,
+    ca_hit_ntuplet_generator.jl:9): contexts are not shared across threads."""
+
+    def __init__(self, params=None, device=0, max_events=1, frames=None):
+        self.lib = A.hip_lib()
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        A.check(self.lib.pt_ctx_create(C.byref(h), C.byref(self.params), device, max_events), "pt_ctx_create")
+        self.h = h
+        self.max_events = max_events
+        if frames is not None:
+            self.set_frames(frames)
+
+    def set_frames(self, frames):
+        fr = np.ascontiguousarray(frames, np.float32).reshape(-1)
+        A.check(self.lib.pt_ctx_set_frames(self.h, A.ptr(fr, C.c_float), A.N_MODULES), "pt_ctx_set_frames")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device memory helpers
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        A.check(self.lib.pt_malloc(self.h, C.byref(p), nbytes), "pt_malloc")
+        return p.value
+
+    def free(self, p):
+        A.check(self.lib.pt_free(self.h, C.c_void_p(p)), "pt_free")
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        A.check(self.lib.pt_memcpy_h2d(self.h, C.c_void_p(dst), arr.ctypes.data, arr.nbytes), "pt_memcpy_h2d")
+
+    def d2h(self, arr, src):
+        A.check(self.lib.pt_memcpy_d2h(self.h, arr.ctypes.data, C.c_void_p(src), arr.nbytes), "pt_memcpy_d2h")
+        return arr
+
+    def memset(self, dst, value, nbytes):
+        A.check(self.lib.pt_memset(self.h, C.c_void_p(dst), value, nbytes), "pt_memset")
+
+    def sync(self):
+        A.check(self.lib.pt_ctx_sync(self.h), "pt_ctx_sync")
+
+    def counters(self, event):
+        c = A.PtCounters()
+        A.check(self.lib.pt_get_counters(self.h, event, C.byref(c)), "pt_get_counters")
+        return c.as_dict()
+
+    def enable_timing(self, on=True):
+        A.check(self.lib.pt_ctx_enable_timing(self.h, 1 if on else 0), "pt_ctx_enable_timing")
+
+    def stage_times(self):
+        ms = (C.c_float * A.N_STAGES)()
+        names = (C.c_char_p * A.N_STAGES)()
+        n = C.c_int(A.N_STAGES)
+        A.check(self.lib.pt_get_stage_times(self.h, ms, names, C.byref(n)), "pt_get_stage_times")
+        out = {}
+        for i in range(n.value):
+            out[names[i].decode()] = out.get(names[i].decode(), 0.0) + float(ms[i])
+        return out
+
+    def ca_debug(self, event, n_hits, cap_nbr=4 * 1024 * 1024, cap_trk=1024 * 1024):
+        """Internal CA state (cells, outer-hit lists, neighbour and track lists)."""
+        d = _DebugBuffers(n_hits, cap_nbr=cap_nbr, cap_trk=cap_trk)
+        A.check(self.lib.pt_get_ca_debug(self.h, event, C.byref(d.struct)), "pt_get_ca_debug")
+        return d
+
+
+class _DebugBuffers:
+    def __init__(self, n_hits, cap_cells=A.MAX_DOUBLETS, cap_nbr=4 * 1024 * 1024, cap_trk=1024 * 1024):
+        self.cell_inner = np.zeros(cap_cells, np.uint16)
+        self.cell_outer = np.zeros(cap_cells, np.uint16)
+        self.cell_pair = np.zeros(cap_cells, np.uint8)
+        self.cell_doublet_id = np.zeros(cap_cells, np.int32)
+        self.cell_used = np.zeros(cap_cells, np.uint8)
+        self.nbr_off = np.zeros(cap_cells + 1, np.uint32)
+        self.nbr = np.zeros(cap_nbr, np.uint32)
+        self.ohc_off = np.zeros(max(1, n_hits) + 1, np.uint32)
+        self.ohc = np.zeros(cap_cells, np.uint32)
+        self.trk_off = np.zeros(cap_cells + 1, np.uint32)
+        self.trk = np.zeros(cap_trk, np.uint16)
+        d = A.PtCaDebug()
+        d.cap_cells, d.cap_nbr, d.cap_ohc, d.cap_trk = cap_cells, cap_nbr, cap_cells, cap_trk
+        for name, ct in (("cell_inner", C.c_uint16), ("cell_outer", C.c_uint16), ("cell_pair", C.c_uint8),
+                         ("cell_doublet_id", C.c_int32), ("cell_used", C.c_uint8), ("nbr_off", C.c_uint32),
+                         ("nbr", C.c_uint32), ("ohc_off", C.c_uint32), ("ohc", C.c_uint32),
+                         ("trk_off", C.c_uint32), ("trk", C.c_uint16)):
+            setattr(d, name, A.ptr(getattr(self, name), ct))
+        self.struct = d
+        self.n_hits = n_hits
+
+    @property
+    def n_cells(self):
+        return int(self.struct.n_cells)
+
+    def cells(self):
+        n = self.n_cells
+        return dict(inner=self.cell_inner[:n].copy(), outer=self.cell_outer[:n].copy(),
+                    pair=self.cell_pair[:n].copy(), doublet_id=self.cell_doublet_id[:n].copy(),
+                    used=self.cell_used[:n].copy())
+
+    def lists(self, name):
+        n = self.n_cells if name in ("nbr", "trk") else self.n_hits
+        off = getattr(self, name + "_off")[:n + 1]
+        vals = getattr(self, name)
+        return [vals[off[i]:off[i + 1]].copy() for i in range(n)]
+
+
+class CAHitNtupletGeneratorOnGPU:
+    """make_tuples on the GPU with the reference's parameters (ca_hit_ntuplet_generator.jl:35-58)."""
+
+    def __init__(self, frames, params=None, device=0, strict=False):
+        self.ctx = Context(params, device=device, max_events=1, frames=frames)
+        self.strict = strict
+        self.last_counters = None
+
+    def make_tuples(self, hits: HitsSoA, b_field=A.B_FIELD_INV_GEV) -> TrackSOA:
+        out = TrackSOA()
+        view = hits.view()
+        tv = out.view()
+        cnt = A.PtCounters()
+        A.check(self.ctx.lib.pt_make_tuples(self.ctx.h, C.byref(view), b_field, C.byref(tv), 1, C.byref(cnt)),
+                "pt_make_tuples")
+        self.last_counters = cnt.as_dict()
+        if self.strict and cnt.ref_assert:
+            bad = [k for k, v in A.REF.items() if cnt.ref_assert & v]
+            raise ReferenceAssertion(f"the reference would throw on this event: {bad}")
+        return out
+
+
+class CAHitNtuplet:
+    """EDProducer mirror (ca_hit_ntuplet.jl:1-19): consumes hits, produces TrackSOA.
+
+    `produce(event)` reads event["TrackingRecHit2DHeterogeneous"] and stores
+    event["TrackSOA"], with bf = 0.0114256972711507 as in the reference."""
+    name = "CAHitNtupletCUDA"
+    bf = A.B_FIELD_INV_GEV
+
+    def __init__(self, frames, params=None, device=0):
+        self.gpu_algo = CAHitNtupletGeneratorOnGPU(frames, params, device, strict=True)
+
+    def produce(self, event, setup=None):
+        hits = event["TrackingRecHit2DHeterogeneous"]
+        event["TrackSOA"] = self.gpu_algo.make_tuples(hits, self.bf)
+        return event["TrackSOA"]
+
+
+class DeviceBatch:
+    """Events resident in device memory plus their device TrackSOAs, for the
+    batched asynchronous entry point (pt_make_tuples_device_async)."""
+
+    HIT_FIELDS = (("xg", np.float32), ("yg", np.float32), ("zg", np.float32), ("rg", np.float32),
+                  ("xerr", np.float32), ("yerr", np.float32), ("iphi", np.int16), ("ysize", np.int16),
+                  ("det_ind", np.int16), ("hist_off", np.uint32), ("hist_bins", np.uint16))
+    OUT_FIELDS = (("quality", np.int32, A.MAX_TUPLES), ("chi2", np.float32, A.MAX_TUPLES),
+                  ("eta", np.float32, A.MAX_TUPLES), ("pt", np.float32, A.MAX_TUPLES),
+                  ("state", np.float32, 5 * A.MAX_TUPLES), ("covariance", np.float32, 15 * A.MAX_TUPLES),
+                  ("hit_off", np.uint32, A.MAX_TUPLES + 1), ("hit_bins", np.uint16, A.MAX_TUPLE_HITS),
+                  ("det_off", np.uint32, A.MAX_TUPLES + 1), ("det_bins", np.uint16, A.MAX_TUPLE_HITS))
+
+    def __init__(self, ctx: Context, events):
+        self.ctx = ctx
+        self.n = len(events)
+        self.events = events
+        self.ptrs = []
+        self.views = (A.PtHitsView * self.n)()
+        self.outs = (A.PtTrackSoA * self.n)()
+        for i, h in enumerate(events):
+            v = self.views[i]
+            for name, dt in self.HIT_FIELDS:
+                arr = np.ascontiguousarray(getattr(h, name), dt)
+                p = ctx.malloc(max(arr.nbytes, 4))
+                self.ptrs.append(p)
+                if arr.nbytes:
+                    ctx.h2d(p, arr)
+                setattr(v, name, C.cast(C.c_void_p(p), type(getattr(v, name))))
+            v.n_hits = h.n_hits
+            for L in range(11):
+                v.layer_start[L] = int(h.layer_start[L])
+            o = self.outs[i]
+            for name, dt, count in self.OUT_FIELDS:
+                nbytes = np.dtype(dt).itemsize * count
+                p = ctx.malloc(nbytes)
+                self.ptrs.append(p)
+                if name in ("state", "covariance"):
+                    ctx.memset(p, 0, nbytes)
+                setattr(o, name, C.cast(C.c_void_p(p), type(getattr(o, name))))
+
+    def run_async(self, b_field=A.B_FIELD_INV_GEV, out_zeroed=False):
+        A.check(self.ctx.lib.pt_make_tuples_device_async(self.ctx.h, self.n, self.views, b_field, self.outs,
+                                                         1 if out_zeroed else 0), "pt_make_tuples_device_async")
+
+    def download(self, i) -> TrackSOA:
+        """Copy event i's device TrackSOA back to a host TrackSOA."""
+        t = TrackSOA()
+        o = self.outs[i]
+        for name, dt, count in self.OUT_FIELDS:
+            arr = getattr(t, name)
+            flat = arr.reshape(-1)
+            src = C.cast(getattr(o, name), C.c_void_p).value
+            self.ctx.d2h(flat, src)
+        return t
+
+    def free(self):
+        for p in self.ptrs:
+            self.ctx.free(p)
+        self.ptrs = []
+
+
+def fit_broken_line(ctx: Context, hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, precision=A.FIT_FP64):
+    """Fit pre-gathered tracks on the GPU (pt_fit_bl_device_async); returns host arrays."""
+    n = int(nhit.shape[0])
+    h = np.ascontiguousarray(hits, np.float32)
+    g = np.ascontiguousarray(ge, np.float32)
+    nh = np.ascontiguousarray(nhit, np.uint8)
+    bufs = {k: ctx.malloc(max(4, a.nbytes)) for k, a in (("h", h), ("g", g), ("n", nh))}
+    outs = {k: ctx.malloc(max(4, 4 * n * m)) for k, m in (("state", 5), ("cov", 15), ("pt", 1), ("eta", 1),
+                                                         ("chi2", 1))}
+    try:
+        ctx.h2d(bufs["h"], h)
+        ctx.h2d(bufs["g"], g)
+        ctx.h2d(bufs["n"], nh)
+        A.check(ctx.lib.pt_fit_bl_device_async(ctx.h, n, bufs["h"], bufs["g"], bufs["n"], b_field, precision,
+                                               outs["state"], outs["cov"], outs["pt"], outs["eta"], outs["chi2"]),
+                "pt_fit_bl_device_async")
+        ctx.sync()
+        res = dict(state=np.zeros((5, n), np.float32), cov=np.zeros((15, n), np.float32),
+                   pt=np.zeros(n, np.float32), eta=np.zeros(n, np.float32), chi2=np.zeros(n, np.float32))
+        for k in res:
+            ctx.d2h(res[k], outs[k])
+        return res
+    finally:
+        for p in list(bufs.values()) + list(outs.values()):
+            ctx.free(p)

@@ -215,9 +215,10 @@ int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
 /* Internal CA state of event `event` of the last batch (device -> host copy). */
 int pt_get_ca_debug(pt_ctx *ctx, uint32_t event, pt_ca_debug *dbg);
 
-/* Stage timing of the last batch, milliseconds, measured with HIP events on
- * the context stream when enabled (enable=1 before the batch). */
-#define PT_N_STAGES 16
+/* Per-kernel timing of the last batch, milliseconds, from HIP events recorded
+ * on the context stream after every kernel when enabled (enable=1 before the
+ * batch).  *n_stages: capacity of ms/names on input, entries on output. */
+#define PT_N_STAGES 48
 int pt_ctx_enable_timing(pt_ctx *ctx, int enable);
 int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages);
 

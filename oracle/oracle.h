@@ -14,7 +14,7 @@ void oracle_params_default(pt_params *p);
 /* make_tuples restated (ca_hit_ntuplet_generator.jl:65-97) on host memory.
  * frames: rot[9*module]; stop_after: 0 = full stage, 1 = after doublets,
  * 2 = after connect, 3 = after fishbone, 4 = after ntuplets/multiplicity
- * (no fit, no classification).  out->state/covariance must be zeroed by the
+ * (no fit, no classification), 5 = after the fit (no classification).  out->state/covariance must be zeroed by the
  * caller (TrackSOA() does the same). */
 int oracle_make_tuples(const pt_params *p, const pt_hits_view *hits, const float *frames, double b_field,
                        pt_track_soa *out, pt_counters *cnt, pt_ca_debug *dbg, int stop_after);

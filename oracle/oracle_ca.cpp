@@ -51,11 +51,13 @@ template <class T> struct PreAllocMatrix {
     uint32_t nrowmax = 0, ncolmax = 0, ncol = 0;
     std::vector<T> data;
     std::vector<uint32_t> len;
+    // (re)allocation keeps old storage: like Matrix{T}(undef, ...) only the
+    // column lengths are initialised (simple_matrix.jl:1-11)
     void init(uint32_t nr, uint32_t nc) {
         nrowmax = nr;
         ncolmax = nc;
         ncol = 0;
-        data.assign((size_t)nr * nc, T(0));
+        if (data.size() < (size_t)nr * nc) data.resize((size_t)nr * nc);
         len.assign(nc, 0);
     }
     // extend!: returns the last column (not failure) when exhausted (simple_matrix.jl:13-21)
@@ -696,7 +698,15 @@ void oracle_params_default(pt_params *p) {
 int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *frames, double b_field,
                        pt_track_soa *out, pt_counters *cnt, pt_ca_debug *dbg, int stop_after) {
     if (!P || !hv || !out) return PT_ERR_ARG;
-    State st;
+    // per-thread scratch reused across events, like CACHED_KERNELS (a TaskLocalValue,
+    // ca_hit_ntuplet_generator.jl:9) reset by resetCAHitNTupletGeneratorKernels
+    thread_local State st_tls;
+    State &st = st_tls;
+    st.n_cells = 0;
+    st.n_found = 0;
+    st.apc[0] = st.apc[1] = 0;
+    st.overflow = st.ref_assert = 0;
+    st.n_connect_pairs = st.n_connections_raw = st.n_nbr_pushes_dropped = 0;
     Hits hh{hv};
     const uint32_t n_hits = hv->n_hits;
     std::memset(cnt, 0, sizeof(*cnt));
@@ -704,14 +714,16 @@ int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *
     if (n_hits > PT_MAX_HITS) st.ref_assert |= PT_REF_TOO_MANY_HITS;
     // TrackSOA(): off zeroed by zero(tuples_d)
     st.tup_off.assign(PT_MAX_TUPLES + 1, 0);
-    st.tup_bins.assign(PT_MAX_TUPLE_HITS, 0);
-    st.quality.assign(PT_MAX_TUPLES, PT_QUALITY_BAD);
+    if (st.tup_bins.size() < PT_MAX_TUPLE_HITS) st.tup_bins.resize(PT_MAX_TUPLE_HITS);
+    if (st.quality.size() < PT_MAX_TUPLES) st.quality.resize(PT_MAX_TUPLES);
     st.quality_set.assign(PT_MAX_TUPLES, 0);
-    st.tm_bins.assign(48 * 1024, 0);
-    st.cells.resize(P->max_num_of_doublets);
+    if (st.tm_bins.size() < 48 * 1024) st.tm_bins.resize(48 * 1024);
+    if (st.cells.size() < P->max_num_of_doublets) st.cells.resize(P->max_num_of_doublets);
     st.ohc.init(PT_MAX_CELLS_PER_HIT, std::max<uint32_t>(n_hits, 1));
-    st.nbr.init(PT_MAX_NEIGHBORS, PT_MAX_ACTIVE_LISTS);
-    st.trk.init(PT_MAX_CELL_TRACKS, PT_MAX_ACTIVE_LISTS);
+    if (st.nbr.ncolmax == 0) st.nbr.init(PT_MAX_NEIGHBORS, PT_MAX_ACTIVE_LISTS);
+    if (st.trk.ncolmax == 0) st.trk.init(PT_MAX_CELL_TRACKS, PT_MAX_ACTIVE_LISTS);
+    st.nbr.reset();
+    st.trk.reset();
     // init_doublets (gpu_pixel_doublets.jl:91-106)
     st.nbr.extend();
     st.trk.extend();
@@ -747,8 +759,10 @@ int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *
         for (uint32_t i = 0; i < st.n_cells; ++i) users += (st.cells[i].tracks == st.trk.ncolmax);
         if (users > 1) st.overflow |= PT_OVF_TRACK_LISTS;
     }
-    const bool full = stop_after <= 0;
-    if (full) {
+    const bool write_out = stop_after <= 0 || stop_after >= 4;
+    const bool do_fit = stop_after <= 0 || stop_after == 5;
+    const bool do_classify = stop_after <= 0;
+    if (write_out) {
         // output hit_indices (the TrackSOA container) and quality
         for (uint32_t i = 0; i <= PT_MAX_TUPLES; ++i) out->hit_off[i] = st.tup_off[i];
         const uint32_t nbins_written = std::min<uint32_t>(st.apc[1], PT_MAX_TUPLE_HITS);
@@ -763,14 +777,14 @@ int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *
         // fit (launchBrokenLineKernelsOnCPU, broken_line_fit_on_gpu.jl:166-203)
         const double B = (double)(float)b_field;
         uint32_t n_fitted = 0;
-        if (frames) {
+        if (frames && do_fit) {
             fit_bucket<3>(hh, frames, 3, B, st, out, n_fitted, st.ref_assert);
             fit_bucket<4>(hh, frames, 4, B, st, out, n_fitted, st.ref_assert);
             fit_bucket<4>(hh, frames, 5, B, st, out, n_fitted, st.ref_assert);
         }
         cnt->n_fitted = n_fitted;
         // classify_tuples (ca_hit_ntuplet_generator_kernels.jl:222-231)
-        if (frames) {
+        if (frames && do_fit && do_classify) {
             kernel_classify_tracks(*P, st, out);
             kernel_fast_duplicate_remover(st, out);
             hit_to_tuple_and_triplet_cleaner(st, out, st.overflow);

@@ -1,0 +1,110 @@
+"""GPU parity: the HIP pixel-track stage against the CPU oracle, stage by stage.
+
+Integer/index work (doublets, outer-hit lists, neighbour lists, fishbone,
+tuples, cell->tuple lists, multiplicity, module indices) must be bit-exact.
+The FP64 fit is compared after the float store of TrackSOA: rtol 2e-6
+(float rounding of double results that agree to ~1e-12), atol 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from pixeltrack import _abi as A
+from pixeltrack import ca, generator as G
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2)]
+FIT_RTOL, FIT_ATOL = 2e-6, 1e-12
+COUNTER_KEYS = ("n_hits", "n_inner_slots", "n_cells", "n_cells_found", "n_connections", "n_connect_pairs",
+                "n_killed", "n_tuples", "n_tuple_hits", "n_by_len", "n_fitted", "overflow", "ref_assert")
+
+
+@pytest.fixture(scope="module")
+def gen_gpu(frames):
+    return ca.CAHitNtupletGeneratorOnGPU(frames)
+
+
+def run_both(gen_gpu, frames, preset, seed):
+    hits = G.generate_event(preset, seed=seed)
+    print(f"[parity] {preset} seed={seed} hits={hits.n_hits}", flush=True)
+    ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=5)
+    out = gen_gpu.make_tuples(hits)
+    gcnt = gen_gpu.last_counters
+    gdbg = gen_gpu.ctx.ca_debug(0, hits.n_hits)
+    return hits, ref, rcnt, rdbg, out, gcnt, gdbg
+
+
+@pytest.mark.parametrize("preset,seed", CASES)
+def test_stage_parity(gen_gpu, frames, preset, seed):
+    hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
+    for k in COUNTER_KEYS:
+        assert gcnt[k] == rcnt[k], (k, gcnt[k], rcnt[k])
+    rc, gc = rdbg.cells(), gdbg.cells()
+    for k in ("inner", "outer", "pair", "doublet_id", "used"):
+        np.testing.assert_array_equal(gc[k], rc[k], err_msg=k)
+    for name in ("ohc", "nbr", "trk"):
+        rl, gl = rdbg.lists(name), gdbg.lists(name)
+        assert len(rl) == len(gl), name
+        bad = [i for i in range(len(rl)) if not np.array_equal(rl[i], gl[i])]
+        assert not bad, f"{name}: {len(bad)} lists differ, first {bad[:5]}"
+    m = min(rcnt["n_tuples"], A.MAX_TUPLES)
+    nb = min(rcnt["n_tuple_hits"], A.MAX_TUPLE_HITS)
+    np.testing.assert_array_equal(out.hit_off, ref.hit_off)
+    np.testing.assert_array_equal(out.hit_bins[:nb], ref.hit_bins[:nb])
+    np.testing.assert_array_equal(out.det_off, ref.det_off)
+    sz = int(ref.hit_off[-1]) - 1
+    np.testing.assert_array_equal(out.det_bins[:sz], ref.det_bins[:sz])
+    np.testing.assert_array_equal(out.quality[:m], ref.quality[:m])
+
+
+@pytest.mark.parametrize("preset,seed", CASES)
+def test_fit_parity(gen_gpu, frames, preset, seed):
+    hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
+    assert gcnt["n_fitted"] == rcnt["n_fitted"] > 0
+    fitted = np.nonzero(ref.state[2] != 0)[0]
+    assert fitted.size == rcnt["n_fitted"]
+    np.testing.assert_allclose(out.state[:, fitted], ref.state[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+    np.testing.assert_allclose(out.covariance[:, fitted], ref.covariance[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+    for k in ("pt", "eta", "chi2"):
+        np.testing.assert_allclose(getattr(out, k)[fitted], getattr(ref, k)[fitted], rtol=FIT_RTOL, atol=FIT_ATOL,
+                                   err_msg=k)
+    unfit = np.setdiff1d(np.arange(A.MAX_TUPLES), fitted)
+    assert not out.state[:, unfit].any() and not out.covariance[:, unfit].any()
+
+
+def test_device_batch_matches_oracle(frames):
+    events = [G.generate_event("pu50", seed=s) for s in (11, 12, 13, 14)]
+    ctx = ca.Context(max_events=len(events), frames=frames)
+    batch = ca.DeviceBatch(ctx, events)
+    print("[parity] device batch of 4 pu50 events", flush=True)
+    batch.run_async(out_zeroed=True)
+    ctx.sync()
+    for i, h in enumerate(events):
+        ref, rcnt, _ = O.make_tuples(h, frames, stop_after=5, debug=False)
+        g = batch.download(i)
+        gcnt = ctx.counters(i)
+        for k in COUNTER_KEYS:
+            assert gcnt[k] == rcnt[k], (i, k)
+        np.testing.assert_array_equal(g.hit_off, ref.hit_off)
+        m = min(rcnt["n_tuples"], A.MAX_TUPLES)
+        np.testing.assert_array_equal(g.quality[:m], ref.quality[:m])
+        fitted = np.nonzero(ref.state[2] != 0)[0]
+        np.testing.assert_allclose(g.state[:, fitted], ref.state[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+    batch.free()
+    ctx.close()
+
+
+def test_fit_only_fp64_matches_oracle():
+    tr = G.fit_tracks(20000, seed=7)
+    ctx = ca.Context(max_events=1)
+    print("[parity] fit-only fp64 20000 tracks", flush=True)
+    res = ca.fit_broken_line(ctx, tr["hits"], tr["ge"], tr["nhit"])
+    out64, flags = O.fit_bl(tr["hits"], tr["ge"], tr["nhit"])
+    ok = flags == 0
+    assert ok.mean() > 0.99
+    np.testing.assert_allclose(res["state"][:, ok], out64[ok, 0:5].T, rtol=FIT_RTOL, atol=1e-9)
+    np.testing.assert_allclose(res["cov"][:, ok], out64[ok, 5:20].T, rtol=FIT_RTOL, atol=1e-12)
+    np.testing.assert_allclose(res["pt"][ok], out64[ok, 20], rtol=FIT_RTOL)
+    np.testing.assert_allclose(res["chi2"][ok], out64[ok, 22], rtol=FIT_RTOL, atol=1e-9)
+    ctx.close()
