@@ -471,9 +471,10 @@ __device__ __forceinline__ bool is_start(const EvDesc &e, const Config &cfg, uin
 }
 
 __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t ref = 0;
+    uint32_t ref = 0, killed = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
     PT_EVENT_LOOP(e, c, e.cnt->n_cells) {
+        killed += e.s.cell_killed[c];
         uint32_t nt = 0, nh = 0;
         if (is_start(e, cfg, c)) {
             dfs(e, c, min_d, ref, [&](const uint32_t *, uint32_t len) {
@@ -487,6 +488,7 @@ __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ ev
         e.s.trk_cursor[c] = 0;
     }
     wave_or_atomic(&evs[blockIdx.y].cnt->ref_assert, ref);
+    wave_add_atomic(&evs[blockIdx.y].cnt->n_killed, killed);
 }
 
 __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs, Config cfg) {

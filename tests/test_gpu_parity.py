@@ -35,27 +35,46 @@ def run_both(gen_gpu, frames, preset, seed):
     return hits, ref, rcnt, rdbg, out, gcnt, gdbg
 
 
+def diff_report(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return f"shape {a.shape} vs {b.shape}"
+    bad = np.nonzero(a != b)[0]
+    return "" if bad.size == 0 else f"{bad.size} differ, first at {bad[:5].tolist()}: {a[bad[:5]].tolist()} vs " \
+                                    f"{b[bad[:5]].tolist()}"
+
+
 @pytest.mark.parametrize("preset,seed", CASES)
 def test_stage_parity(gen_gpu, frames, preset, seed):
     hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
+    errors = []
     for k in COUNTER_KEYS:
-        assert gcnt[k] == rcnt[k], (k, gcnt[k], rcnt[k])
+        if gcnt[k] != rcnt[k]:
+            errors.append(f"counter {k}: gpu {gcnt[k]} oracle {rcnt[k]}")
     rc, gc = rdbg.cells(), gdbg.cells()
     for k in ("inner", "outer", "pair", "doublet_id", "used"):
-        np.testing.assert_array_equal(gc[k], rc[k], err_msg=k)
+        d = diff_report(gc[k], rc[k])
+        if d:
+            errors.append(f"cells.{k}: {d}")
     for name in ("ohc", "nbr", "trk"):
         rl, gl = rdbg.lists(name), gdbg.lists(name)
-        assert len(rl) == len(gl), name
+        if len(rl) != len(gl):
+            errors.append(f"{name}: {len(gl)} lists vs {len(rl)}")
+            continue
         bad = [i for i in range(len(rl)) if not np.array_equal(rl[i], gl[i])]
-        assert not bad, f"{name}: {len(bad)} lists differ, first {bad[:5]}"
+        if bad:
+            errors.append(f"{name}: {len(bad)} lists differ, first {bad[:3]}: "
+                          f"{[gl[i].tolist() for i in bad[:3]]} vs {[rl[i].tolist() for i in bad[:3]]}")
     m = min(rcnt["n_tuples"], A.MAX_TUPLES)
     nb = min(rcnt["n_tuple_hits"], A.MAX_TUPLE_HITS)
-    np.testing.assert_array_equal(out.hit_off, ref.hit_off)
-    np.testing.assert_array_equal(out.hit_bins[:nb], ref.hit_bins[:nb])
-    np.testing.assert_array_equal(out.det_off, ref.det_off)
     sz = int(ref.hit_off[-1]) - 1
-    np.testing.assert_array_equal(out.det_bins[:sz], ref.det_bins[:sz])
-    np.testing.assert_array_equal(out.quality[:m], ref.quality[:m])
+    for name, g, r in (("hit_off", out.hit_off, ref.hit_off), ("hit_bins", out.hit_bins[:nb], ref.hit_bins[:nb]),
+                       ("det_off", out.det_off, ref.det_off), ("det_bins", out.det_bins[:sz], ref.det_bins[:sz]),
+                       ("quality", out.quality[:m], ref.quality[:m])):
+        d = diff_report(g, r)
+        if d:
+            errors.append(f"{name}: {d}")
+    assert not errors, "\n".join(errors)
 
 
 @pytest.mark.parametrize("preset,seed", CASES)
