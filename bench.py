@@ -33,30 +33,35 @@ FIT_FLOPS = {3: None, 4: None}
 
 
 def kernel_bytes(name, c):
-    """Algorithmic HBM bytes of one launch over the batch (sum of per-event
-    figures).  c: dict of per-event counter arrays."""
+    """Algorithmic HBM bytes of one launch over the batch: the bytes the kernel
+    must read and write at least once (unique inputs and outputs, no re-reads),
+    summed over the events.  c: dict of per-event counter arrays."""
     H, S, D = c["n_hits"], c["n_inner_slots"], c["n_cells"]
     C, T = c["n_connections"], np.minimum(c["n_tuples"], 32768)
     Th = np.minimum(c["n_tuple_hits"], 163840)
-    F = c["n_fitted"]
     hist = 1281 * 4 + 11 * 4
     if name == "k_doublet_count":
-        # each inner/outer hit field read once (det, z, r, iphi, ysize = 14 B,
-        # histogram bin 2 B), histogram offsets, one count per inner slot
-        b = 16 * H + hist + 4 * S
+        # inner-hit record per slot 16 B, histogram-ordered hit records 16 B per hit,
+        # histogram offsets; out: count 4 + mask 8 + candidate range 16 B per slot
+        b = 16 * S + 16 * H + hist + 28 * S
     elif name == "k_doublet_fill":
-        # same reads + slot offsets, writes 7 B per cell and one list count per hit
-        b = 16 * H + hist + 4 * S + 7 * D + 4 * H
-    elif name in ("k_connect_count", "k_connect_fill"):
-        # per cell: inner/outer id 4 B, list entry 4 B; per hit x,y,z,r,det 18 B;
-        # slot offsets; count: neighbour count + used flag, fill: neighbour ids
-        b = 18 * H + 4 * S + 8 * D + (5 * D if name == "k_connect_count" else 4 * C + 4 * D)
+        # per slot: count, mask, range (28 B) in, first cell id (4 B) out; per cell:
+        # outer-hit record + xy (24 B) in, cell record 4 + pair 1 + flags 1 + count 1
+        # + outer-hit coordinates 16 + outer-hit list entry 8 + cursor 4 B out
+        b = 32 * S + 24 * D + 35 * D
+    elif name == "k_connect":
+        # per hit: list cursor 4 B, slot ranges 8 B, coordinates 16 B; per cell: list
+        # entry 8 B, inner-hit coordinates 16 B + module 2 B, outer-hit coordinates of
+        # the cell as a candidate 16 B, neighbour count 1 B; per neighbour entry 4 B
+        b = 28 * H + 43 * D + 4 * C
     elif name == "k_fishbone":
-        b = 22 * H + 8 * D
+        # per hit: cursor + coordinates 20 B; per cell: list entry 8 B, inner-hit
+        # coordinates 16 B + module 2 B, flags 1 B
+        b = 20 * H + 27 * D
     elif name in ("k_dfs_count", "k_dfs_emit"):
-        b = 4 * D + 4 * C + 2 * D + (8 * D if name == "k_dfs_count" else 12 * T + 2 * Th + 16 * T)
-    elif name.startswith("k_fit_tuples"):
-        b = None
+        # per neighbour entry 4 B + neighbour flags 1 B; per stored tuple: offsets 4 B,
+        # hits 2 B each, cells 16 B, quality 4 B (emit)
+        b = 5 * C + 4 * D // 10 + ((4 + 16 + 4) * T + 2 * Th if name == "k_dfs_emit" else 0)
     else:
         b = None
     return None if b is None else int(np.sum(b))
@@ -110,6 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=100)
     ap.add_argument("--preset", default="pu50")
+    ap.add_argument("--streams", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
@@ -134,22 +140,30 @@ def main():
     counters = [ctx.counters(i) for i in range(len(events))]
     cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0] if k != "n_by_len"}
 
-    ctx.enable_timing(True)
-    per_kernel = {}
+    # timed region: production mode (events split over streams, no timing marks)
+    ctx.set_streams(args.streams)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.run_async(out_zeroed=True)
-        for k, v in ctx.stage_times().items():  # synchronises the step
-            per_kernel[k] = per_kernel.get(k, 0.0) + v
     ctx.sync()
     wall = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {wall:.4f} s ({args.streams} streams)", file=sys.stderr, flush=True)
+
+    # per-kernel durations: HIP events recorded after every kernel on the
+    # stream the kernels run on (a timed batch runs on one stream)
+    ctx.enable_timing(True)
+    per_kernel = {}
+    n_inst = max(3, args.steps // 2)
+    for _ in range(n_inst):
+        batch.run_async(out_zeroed=True)
+        for k, v in ctx.stage_times().items():
+            per_kernel[k] = per_kernel.get(k, 0.0) + v
     ctx.enable_timing(False)
-    print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
 
     ms_per_step = wall * 1e3 / args.steps
     value = len(events) * args.steps / wall
-    avg = {k: v / args.steps for k, v in per_kernel.items()}
+    avg = {k: v / n_inst for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
     dom_ms = avg[dom]
     roof = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None, traffic=None,
@@ -187,6 +201,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": kernels_ms,
+        "kernels_sum_ms_single_stream": round(sum(avg.values()), 4),
+        "streams": args.streams,
     }
     print(json.dumps(line))
     batch.free()

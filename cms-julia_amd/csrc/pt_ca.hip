@@ -2,33 +2,150 @@
 // lists, cell connection, fishbone, ntuplet search, early duplicate removal,
 // tuple multiplicity and hit->module indices.
 //
-// Every stage reproduces the serial creation order of the reference (so ids,
-// caps and list contents are identical) without serialising the work:
-//   * doublets: count per inner slot -> exclusive scan -> fill at the scanned
-//     offset; cell id = serial doublet_id - 1 (gpu_pixel_doublets.jl:187-355)
-//   * neighbour lists are built from the outer cell's side ("o-centric"): the
-//     candidates of an outer cell are the cells starting on its outer hit,
-//     which the doublet scan already lays out in ascending id order per layer
-//     pair, so each list comes out ascending and capped at 64 exactly as the
-//     serial push! order gives (ca_hit_ntuplet_generator_kernels_impl.jl:45-114)
-//   * ntuplets: per start cell count -> scan -> emit, so tuple ids equal the
-//     serial DFS emission order (gpu_ca_cell.jl:232-270, impl.jl:116-132).
+// Every stage reproduces the serial creation order of the reference, so ids,
+// caps and list contents are identical, without serialising the work:
+//   * doublets (gpu_pixel_doublets.jl:187-355): count the passing candidates of
+//     each inner slot, then give slot j the cell ids after those of slots < j
+//     (cell id = serial doublet id - 1) and write the cells;
+//   * kernel_connect (ca_hit_ntuplet_generator_kernels_impl.jl:45-114) is laid
+//     out per hit h: every (cell ending on h) x (cell starting on h) pair is one
+//     lane's work, and a cell's neighbours come out in ascending id order, the
+//     serial push! order, capped at 64;
+//   * fishbone (gpu_fishbone.jl:4-62) likewise per hit, one lane per cell pair;
+//   * ntuplets (gpu_ca_cell.jl:232-270, impl.jl:116-132): count the tuples of
+//     each start cell, scan in cell order, emit, so tuple ids equal the serial
+//     DFS emission order.
+// The kernels are latency-bound gathers: work is spread so that every lane of
+// a wavefront is busy and several independent loads are in flight per lane,
+// and prefix sums use per-tile totals instead of waiting on other tiles.
 #include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
 
 #include "pt_device.hpp"
 
 namespace pt {
 
 // ------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
 __device__ __forceinline__ void wave_add_atomic(uint32_t *addr, uint32_t v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(addr, v);
+    if (lane_id() == 0 && v) atomicAdd(addr, v);
 }
 __device__ __forceinline__ void wave_or_atomic(uint32_t *addr, uint32_t v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicOr(addr, v);
+    if (lane_id() == 0 && v) atomicOr(addr, v);
+}
+// orders the wavefront's own LDS accesses (one wavefront, no block barrier)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+    return v;
+}
+// bits [s0, s1) of a 64-bit mask, 0 <= s0 <= s1 <= 64
+__device__ __forceinline__ uint64_t range_mask(uint32_t s0, uint32_t s1) {
+    const uint64_t hi = (s1 >= 64u) ? ~0ull : ((1ull << s1) - 1ull);
+    const uint64_t lo = (s0 >= 64u) ? ~0ull : ((1ull << s0) - 1ull);
+    return hi & ~lo;
+}
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// One ticket per wavefront from a per-event counter.
+__device__ __forceinline__ uint32_t take_ticket(uint32_t *ctr) {
+    uint32_t t = 0;
+    if (lane_id() == 0) t = atomicAdd(ctr, 1u);
+    return __shfl(t, 0, 64);
+}
+
+// owner of item g when lane l owns items [incl_l - n_l, incl_l): the first lane
+// whose inclusive prefix exceeds g
+__device__ __forceinline__ int owner_of(uint32_t incl, uint32_t g) {
+    int lo = 0, hi = 63;
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+        const int mid = (lo + hi) >> 1;
+        if ((uint32_t)__shfl(incl, mid, 64) > g)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// position of the r-th set bit (r < popcount(m))
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint32_t lo = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
+        if (r >= lo) {
+            r -= lo;
+            m >>= w;
+            pos += (uint32_t)w;
+        }
+    }
+    return pos;
+}
+
+// Wavefront owners hold lane-contiguous items; after a round of 64 items
+// starting at `base`, fold the pass bits of this lane's own items (lanes
+// [s0, s1), item k at lane k - (excl - base)) into its count and 64-bit mask.
+__device__ __forceinline__ void fold_round(uint64_t pass, uint32_t excl, uint32_t incl, uint32_t base, uint32_t &cnt,
+                                           uint64_t &msk) {
+    const int64_t d = (int64_t)excl - (int64_t)base;
+    const int64_t d1 = (int64_t)incl - (int64_t)base;
+    const uint32_t s0 = (uint32_t)(d < 0 ? 0 : (d > 64 ? 64 : d));
+    const uint32_t s1 = (uint32_t)(d1 < 0 ? 0 : (d1 > 64 ? 64 : d1));
+    if (s1 > s0) {
+        const uint64_t seg = pass & range_mask(s0, s1);
+        cnt += (uint32_t)__popcll(seg);
+        msk |= (d >= 0) ? (seg >> d) : ((-d < 64) ? (seg << (-d)) : 0ull);
+    }
+}
+
+// Block-wide (256 threads) sums and exclusive scans; lds holds 4 words.
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T *lds) {
+    const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane_id() == 0) lds[wid] = v;
+    __syncthreads();
+    const T r = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return r;
+}
+template <class T>
+__device__ __forceinline__ T block_excl_scan4(T v, T *lds, T &total) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    T inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T o = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += o;
+    }
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    T pre = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k < wid) pre += lds[k];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return pre + inc - v;
 }
 
 // ------------------------------------------------------------------ doublet cuts
@@ -43,93 +160,141 @@ __device__ __forceinline__ bool pt_cut(float ro, int16_t idphi, float ri) {
     return dphi * dphi * (r2t4 - (double)rr) > (double)d2;
 }
 
-// Walk the phi window of inner slot j (gpu_pixel_doublets.jl:187-355) and call
-// on_pass(outer_hit_1based) for every candidate passing all cuts, in the
-// serial order: bins kl..kh-1 with wrap, stored (descending-id) order inside a bin.
-template <class F>
-__device__ __forceinline__ void walk_slot(const EvDesc &e, const Config &cfg, uint32_t j, uint32_t &ref_bits,
-                                          F &&on_pass) {
-    const pt_params &P = cfg.p;
+// Inner-hit part of the doublet search for inner slot j (gpu_pixel_doublets.jl:187-308):
+// the inner-hit cuts and the candidate range(s): bins kl..kh-1 of the outer
+// layer are contiguous histogram positions (two ranges when the window wraps
+// past bin 128), visited by the serial loop in position order.
+struct SlotCtx {
+    float me_z, me_r;
+    int32_t me_s;
+    uint32_t pk;          // me_p (16 b) | pair << 16
+    uint32_t a0, na, b0;  // candidates: positions [a0, a0+na) then [b0, ...)
+};
+
+__device__ __forceinline__ int slot_pair(const SlotCtx &s) { return (int)(s.pk >> 16); }
+__device__ __forceinline__ int16_t slot_me_p(const SlotCtx &s) { return (int16_t)(s.pk & 0xFFFFu); }
+
+__device__ __forceinline__ int pair_of_slot(const EvDesc &e, uint32_t j) {
     int p = 0;
     while (j >= e.slot_cum[p]) ++p;
+    return p;
+}
+
+// returns the number of candidates (0 when the inner hit fails its cuts)
+__device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cfg, uint32_t j, uint32_t &ref_bits,
+                                               SlotCtx &s) {
+    const pt_params &P = cfg.p;
+    const int p = pair_of_slot(e, j);
     const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
     const uint32_t i0 = ((p == 0) ? j : j - e.slot_cum[p - 1]) + e.layer_start[inner];
-    const int16_t mi = e.det[i0];
-    if (mi > 2000) return;
-    const float me_z = e.zg[i0];
-    if (me_z < cfg.pairs.min_z[p] || me_z > cfg.pairs.max_z[p]) return;
+    const uint4 q = e.s.hdbl[i0];
+    const int16_t mi = (int16_t)(q.z >> 16);
+    s.pk = (uint32_t)p << 16;
+    if (mi > 2000) return 0;
+    s.me_z = __uint_as_float(q.x);
+    if (s.me_z < cfg.pairs.min_z[p] || s.me_z > cfg.pairs.max_z[p]) return 0;
     int me_s = -1;
     if (P.do_cluster_cut) {
         if (inner == 0 && !(mi < 96)) ref_bits |= PT_REF_BAD_MODULE;
         const bool is_outer_ladder = P.ideal_conditions ? true : (((mi / 8) % 2) == 0);
-        me_s = (inner > 0 || is_outer_ladder) ? (int)e.ysize[i0] : -1;
-        if (inner == 0 && outer > 3 && me_s > 0 && me_s < 36) return;
-        if (inner == 1 && outer > 3 && me_s > 0 && me_s < 28) return;
+        me_s = (inner > 0 || is_outer_ladder) ? (int)(int16_t)q.w : -1;
+        if (inner == 0 && outer > 3 && me_s > 0 && me_s < 36) return 0;
+        if (inner == 1 && outer > 3 && me_s > 0 && me_s < 28) return 0;
     }
-    const int16_t me_p = e.iphi[i0];
-    const float me_r = e.rg[i0];
+    s.me_s = me_s;
+    const int16_t me_p = (int16_t)(q.z & 0xFFFFu);
+    s.me_r = __uint_as_float(q.y);
+    s.pk = (uint32_t)(uint16_t)me_p | ((uint32_t)p << 16);
     const int16_t cut = cfg.pairs.phi_cut[p];
-    const float max_r = cfg.pairs.max_r[p];
-    const float dz_dr_fact = (float)(8 * 0.0285 / 0.015);
-    const int dy = (inner == 0) ? 28 : 20;
-    const uint32_t lo = e.layer_start[outer], hi = e.layer_start[outer + 1];
-    int kl = hist_bin((int16_t)(me_p - cut));
+    const int kl = hist_bin((int16_t)(me_p - cut));
     int kh = hist_bin((int16_t)(me_p + cut)) + 1;
     if (kh == 129) kh = 1;
+    if (kl == kh) return 0;  // while (current_bin != kh) visits no bin
     const int h_off = 128 * outer;
-    for (int cb = kl; cb != kh; cb = (cb == 128) ? 1 : cb + 1) {
-        const uint32_t pb = e.hist_off[cb + h_off - 1];
-        uint32_t pe = e.hist_off[cb + h_off];
-        if (pe > e.n_hits) {  // malformed histogram: never read past the hit arrays
+    auto off = [&](int k) {  // Julia off[k + h_off], clamped to the hit count
+        uint32_t v = e.hist_off[k + h_off - 1];
+        if (v > e.n_hits) {
             ref_bits |= PT_REF_HIST_LAYER;
-            pe = e.n_hits;
+            v = e.n_hits;
         }
-        for (uint32_t pos = pb; pos < pe; ++pos) {
-            const uint32_t oi = e.hist_bins[pos];  // 1-based
-            if (!(oi > lo && oi <= hi)) {
-                ref_bits |= PT_REF_HIST_LAYER;
-                if (oi == 0 || oi > e.n_hits) continue;
+        return v;
+    };
+    uint32_t nb;
+    s.a0 = off(kl);
+    if (kl < kh) {
+        const uint32_t a1 = off(kh);
+        s.na = a1 > s.a0 ? a1 - s.a0 : 0u;
+        s.b0 = 0;
+        nb = 0;
+    } else {  // bins kl..128 then 1..kh-1
+        const uint32_t a1 = off(129);
+        s.na = a1 > s.a0 ? a1 - s.a0 : 0u;
+        s.b0 = off(1);
+        const uint32_t b1 = off(kh);
+        nb = b1 > s.b0 ? b1 - s.b0 : 0u;
+    }
+    return s.na + nb;
+}
+
+__device__ __forceinline__ uint32_t cand_pos(uint32_t a0, uint32_t na, uint32_t b0, uint32_t k) {
+    return (k < na) ? a0 + k : b0 + (k - na);
+}
+__device__ __forceinline__ uint32_t cand_pos(const SlotCtx &s, uint32_t k) { return cand_pos(s.a0, s.na, s.b0, k); }
+
+// Candidate cuts (gpu_pixel_doublets.jl:317-340) on the hit record q stored at
+// a histogram position (q.w >> 16 = 1-based hit id).
+__device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, const SlotCtx &s, uint4 q,
+                                          uint32_t &ref_bits) {
+    const pt_params &P = cfg.p;
+    const int p = slot_pair(s);
+    const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
+    const uint32_t oi = q.w >> 16;
+    if (!(oi > e.layer_start[outer] && oi <= e.layer_start[outer + 1])) {
+        ref_bits |= PT_REF_HIST_LAYER;
+        if (oi == 0 || oi > e.n_hits) return false;
+    }
+    if ((int16_t)(q.z >> 16) > 2000) return false;
+    const float zo = __uint_as_float(q.x), ro = __uint_as_float(q.y);
+    if (P.do_z0_cut) {
+        // gpu_pixel_doublets.jl:69-74
+        const float dr = ro - s.me_r;
+        if (dr > cfg.pairs.max_r[p] || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr) return false;
+    }
+    const int16_t cut = cfg.pairs.phi_cut[p];
+    int16_t i_dphi = (int16_t)((int16_t)(q.z & 0xFFFFu) - slot_me_p(s));
+    i_dphi = (int16_t)(i_dphi < 0 ? -i_dphi : i_dphi);  // Int16 abs wraps at -32768
+    if (i_dphi > cut) return false;
+    if (P.do_cluster_cut) {
+        // gpu_pixel_doublets.jl:79-88
+        const int me_s = s.me_s;
+        if (outer < 4) {
+            const int so = (int16_t)(q.w & 0xFFFFu);
+            if (me_s > 0 && so > 0 && abs(so - me_s) > ((inner == 0) ? 28 : 20)) return false;
+        } else if (inner < 4 && me_s > 0) {
+            const float qq = fabsf((s.me_z - zo) / (s.me_r - ro)) * (float)(8 * 0.0285 / 0.015);
+            const double t = trunc((double)qq + 0.5);
+            if (!isfinite(t) || fabs(t) > 9.0e15) {
+                ref_bits |= PT_REF_SIZE_CUT_INF;
+                return false;
             }
-            const uint32_t o0 = oi - 1;
-            if (e.det[o0] > 2000) continue;
-            const float ro = e.rg[o0];
-            const float zo = e.zg[o0];
-            if (P.do_z0_cut) {
-                // gpu_pixel_doublets.jl:69-74
-                const float dr = ro - me_r;
-                if (dr > max_r || dr < 0.0f || fabsf(me_z * ro - me_r * zo) > 12.0f * dr) continue;
-            }
-            const int16_t mo_p = e.iphi[o0];
-            int16_t i_dphi = (int16_t)(mo_p - me_p);
-            i_dphi = (int16_t)(i_dphi < 0 ? -i_dphi : i_dphi);  // Int16 abs wraps at -32768
-            if (i_dphi > cut) continue;
-            if (P.do_cluster_cut) {
-                // gpu_pixel_doublets.jl:79-88
-                if (outer < 4) {
-                    const int so = e.ysize[o0];
-                    if (me_s > 0 && so > 0 && abs(so - me_s) > dy) continue;
-                } else if (inner < 4 && me_s > 0) {
-                    const float q = fabsf((me_z - zo) / (me_r - ro)) * dz_dr_fact;
-                    const double t = trunc((double)q + 0.5);
-                    if (!isfinite(t) || fabs(t) > 9.0e15) {
-                        ref_bits |= PT_REF_SIZE_CUT_INF;
-                        continue;
-                    }
-                    if (llabs((long long)me_s - (long long)t) > 20) continue;
-                }
-            }
-            if (P.do_pt_cut && pt_cut(ro, i_dphi, me_r)) continue;
-            on_pass(oi, i0);
+            if (llabs((long long)me_s - (long long)t) > 20) return false;
         }
     }
+    if (P.do_pt_cut && pt_cut(ro, i_dphi, s.me_r)) return false;
+    return true;
 }
 
 // ------------------------------------------------------------------ K0: reset per event
-// Clears the counters, the per-hit list heads and (unless the caller's
-// TrackSOA is already zero, as TrackSOA() makes it) state and covariance.
-__global__ void k_reset(const EvDesc *__restrict__ evs, int zero_state) {
+// Clears the counters, list cursors and per-tile totals; packs the per-hit
+// fields the CA reads into 16-byte records, once in hit order and once in
+// histogram order (so a slot's candidates are contiguous records); and
+// (unless the caller's TrackSOA is already zero, as TrackSOA() makes it)
+// zeroes state and covariance.
+__global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, EvGrid grid, int zero_state) {
     constexpr uint32_t kHead = 32u;
-    PT_EVENT_LOOP(e, i, kHead + e.n_hits + 1u + (zero_state ? 20u * kMaxTuples : 0u)) {
+    constexpr uint32_t kBits = (kMaxCells + 63u) / 64u;
+    PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kAggTile - 1) / kAggTile + kDfsAgg + kBits +
+                            (zero_state ? 20u * kMaxTuples : 0u)) {
         uint32_t k = i;
         if (k < kHead) {
             if (k < sizeof(pt_counters) / 4) reinterpret_cast<uint32_t *>(e.cnt)[k] = 0u;
@@ -139,12 +304,44 @@ __global__ void k_reset(const EvDesc *__restrict__ evs, int zero_state) {
             continue;
         }
         k -= kHead;
-        if (k < e.n_hits + 1u) {
-            e.s.ohc_cnt[k] = 0u;
-            if (k < e.n_hits) e.s.ohc_cursor[k] = 0u;
+        if (k < e.n_hits) {
+            e.s.ohc_cursor[k] = 0u;
+            const float z = e.zg[k], r = e.rg[k];
+            e.s.hxyzr[k] = make_float4(e.xg[k], e.yg[k], z, r);
+            e.s.hdbl[k] = make_uint4(__float_as_uint(z), __float_as_uint(r),
+                                     (uint32_t)(uint16_t)e.iphi[k] | ((uint32_t)(uint16_t)e.det[k] << 16),
+                                     (uint32_t)(uint16_t)e.ysize[k]);
+            const uint32_t h = e.hist_bins[k];  // 1-based hit at histogram position k
+            uint4 rec = make_uint4(0u, 0u, 0x7FFFu << 16, h << 16);
+            float2 xy = make_float2(0.f, 0.f);
+            if (h >= 1u && h <= e.n_hits) {
+                const uint32_t h0 = h - 1u;
+                rec = make_uint4(__float_as_uint(e.zg[h0]), __float_as_uint(e.rg[h0]),
+                                 (uint32_t)(uint16_t)e.iphi[h0] | ((uint32_t)(uint16_t)e.det[h0] << 16),
+                                 (uint32_t)(uint16_t)e.ysize[h0] | (h << 16));
+                xy = make_float2(e.xg[h0], e.yg[h0]);
+            }
+            e.s.binrec[k] = rec;
+            e.s.binxy[k] = xy;
             continue;
         }
-        k -= e.n_hits + 1u;
+        k -= e.n_hits;
+        const uint32_t n_dbl_agg = (e.n_slots + kAggTile - 1) / kAggTile;
+        if (k < n_dbl_agg) {
+            e.s.dbl_agg[k] = 0u;
+            continue;
+        }
+        k -= n_dbl_agg;
+        if (k < kDfsAgg) {
+            e.s.dfs_agg[k] = 0ull;
+            continue;
+        }
+        k -= kDfsAgg;
+        if (k < kBits) {
+            e.s.start_bits[k] = 0ull;
+            continue;
+        }
+        k -= kBits;
         if (k < 5u * kMaxTuples)
             e.state[k] = 0.0f;
         else
@@ -153,130 +350,203 @@ __global__ void k_reset(const EvDesc *__restrict__ evs, int zero_state) {
 }
 
 // ------------------------------------------------------------------ K1: doublet count
-__global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict__ evs, Config cfg) {
+// A wavefront takes a tile of 64 consecutive inner slots.  Their candidates
+// (about 16 per slot) are spread over all 64 lanes, kEvalU rounds of loads in
+// flight at a time; the pass bits come back to the slot's lane through the
+// ballot: the slot's count and, for its first 64 candidates, a bit mask.  Per
+// tile of kAggTile slots the counts are also summed.
+constexpr int kEvalU = 4;
+
+__global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    __shared__ SlotCtx ctx[4][64];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t n_slots = e.n_slots;
+    const uint32_t n_tiles = (n_slots + 63u) / 64u;
     uint32_t ref = 0;
-    PT_EVENT_LOOP(e, j, e.n_slots) {
-        uint32_t c = 0;
-        walk_slot(e, cfg, j, ref, [&](uint32_t, uint32_t) { ++c; });
-        e.s.slot_cnt[j] = c;
+    for (;;) {
+        const uint32_t t = take_ticket(&e.s.misc[kMiscTkCount]);
+        if (t >= n_tiles) break;
+        const uint32_t j0 = t * 64u, j = j0 + lane;
+        wave_sync();  // the previous tile's reads of ctx are done
+        SlotCtx s = {};
+        const uint32_t n = (j < n_slots) ? slot_setup(e, cfg, j, ref, s) : 0u;
+        ctx[w][lane] = s;
+        wave_sync();
+        const uint32_t incl = wave_incl_scan(n);
+        const uint32_t excl = incl - n;
+        const uint32_t total = __shfl(incl, 63, 64);
+        uint32_t cnt = 0;
+        uint64_t msk = 0;
+        for (uint32_t base = 0; base < total; base += 64u * kEvalU) {
+            SlotCtx c[kEvalU];
+            uint4 q[kEvalU];
+            uint32_t g[kEvalU];
+#pragma unroll
+            for (int u = 0; u < kEvalU; ++u) {
+                g[u] = base + 64u * u + lane;
+                const int o = owner_of(incl, g[u]);
+                const uint32_t x = (uint32_t)__shfl(excl, o, 64);
+                q[u] = make_uint4(0u, 0u, 0u, 0u);
+                c[u] = ctx[w][o];
+                if (g[u] < total) q[u] = e.s.binrec[cand_pos(c[u], g[u] - x)];
+            }
+#pragma unroll
+            for (int u = 0; u < kEvalU; ++u) {
+                const bool ok = g[u] < total && cand_pass(e, cfg, c[u], q[u], ref);
+                fold_round(__ballot(ok), excl, incl, base + 64u * u, cnt, msk);
+            }
+        }
+        if (j < n_slots) {
+            e.s.slot_cnt[j] = cnt;
+            e.s.slot_mask[j] = msk;
+            e.s.slot_pos[j] = make_uint4(s.a0, s.na, s.b0, n | (s.pk & 0xFFFF0000u));
+        }
+        const uint32_t sum = wave_sum(cnt);
+        if (lane == 0 && sum) atomicAdd(&e.s.dbl_agg[j0 / kAggTile], sum);
     }
-    wave_or_atomic(&evs[blockIdx.y].cnt->ref_assert, ref);
+    wave_or_atomic(&e.cnt->ref_assert, ref);
 }
 
-// ------------------------------------------------------------------ K3: doublet fill
-__global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t ref = 0;
-    PT_EVENT_LOOP(e, j, e.n_slots) {
-        const uint32_t n_cells = e.cnt->n_cells;
-        uint32_t base = e.s.slot_cnt[j];
-        const uint32_t end = e.s.slot_cnt[j + 1];
-        if (base >= n_cells || base == end) continue;
-        int p = 0;
-        while (j >= e.slot_cum[p]) ++p;
-        walk_slot(e, cfg, j, ref, [&](uint32_t oi, uint32_t i0) {
-            if (base < n_cells) {
-                e.s.cell_inner[base] = (uint16_t)(i0 + 1);
-                e.s.cell_outer[base] = (uint16_t)oi;
-                e.s.cell_pair[base] = (uint8_t)p;
-                e.s.cell_used[base] = 0;
-                e.s.cell_killed[base] = 0;
-                atomicAdd(&e.s.ohc_cnt[oi - 1], 1u);
+// ------------------------------------------------------------------ K2: doublet fill
+// A block takes a tile of kAggTile slots: its first cell id is the sum of the
+// preceding tiles' counts, a block scan gives each slot's first id (stored in
+// place of its count).  Then, per wavefront, the passing candidates are spread
+// over the lanes and each lane writes one cell: the (inner, outer) record, the
+// pair, the outer hit's coordinates, cleared flags and neighbour count, and an
+// entry in the outer hit's list.  Doublets beyond max_num_of_doublets are
+// counted but not stored, like the serial loop.
+__global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    __shared__ uint32_t lsum[4];
+    __shared__ uint32_t ltk;
+    __shared__ uint32_t lid[4][64];
+    __shared__ unsigned long long lmsk[4][64];
+    __shared__ uint4 lpos[4][64];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t n_slots = e.n_slots;
+    const uint32_t n_agg = (n_slots + kAggTile - 1) / kAggTile;
+    const uint32_t cap = cfg.p.max_num_of_doublets;
+    uint32_t ref = 0, ovf = 0, err = 0;
+    for (;;) {
+        if (threadIdx.x == 0) ltk = atomicAdd(&e.s.misc[kMiscTkFill], 1u);
+        __syncthreads();
+        const uint32_t T = ltk;
+        __syncthreads();
+        if (T >= n_agg) break;
+        uint32_t part = 0;
+        for (uint32_t k = threadIdx.x; k < T; k += 256u) part += e.s.dbl_agg[k];
+        const uint32_t pre = block_sum(part, lsum);
+        const uint32_t j = T * kAggTile + threadIdx.x;
+        const uint32_t cnt = (j < n_slots) ? e.s.slot_cnt[j] : 0u;
+        uint64_t msk = 0;
+        uint4 sp = make_uint4(0u, 0u, 0u, 0u);
+        if (j < n_slots) {
+            msk = e.s.slot_mask[j];
+            sp = e.s.slot_pos[j];
+        }
+        uint32_t tot;
+        const uint32_t id = pre + block_excl_scan4(cnt, lsum, tot);
+        if (j < n_slots) e.s.slot_cnt[j] = id;
+        if (T == n_agg - 1 && threadIdx.x == 0) {
+            const uint32_t all = pre + tot;
+            e.s.slot_cnt[n_slots] = all;
+            e.cnt->n_cells_found = all;
+            e.cnt->n_cells = all < cap ? all : cap;
+            if (all > cap) ovf |= PT_OVF_DOUBLETS;
+        }
+        if (pre >= cap) continue;  // nothing of this tile is stored (uniform in the block)
+        lid[w][lane] = id;
+        lmsk[w][lane] = msk;
+        lpos[w][lane] = sp;
+        wave_sync();
+        auto emit = [&](uint32_t js, int p, uint32_t pos, uint32_t cid) {
+            const uint4 q = e.s.binrec[pos];
+            const float2 xy = e.s.binxy[pos];
+            const uint32_t i1 = ((p == 0) ? js : js - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]] + 1u;
+            const uint32_t oi = q.w >> 16;
+            e.s.cell_io[cid] = i1 | (oi << 16);
+            e.s.cell_flags[cid] = 0;
+            e.s.cell_pair[cid] = (uint8_t)p;
+            e.s.cell_nn[cid] = 0;
+            e.s.cell_oxyzr[cid] = make_float4(xy.x, xy.y, __uint_as_float(q.x), __uint_as_float(q.y));
+            const uint32_t r = atomicAdd(&e.s.ohc_cursor[oi - 1u], 1u);
+            if (r < kOhcStride) {
+                e.s.ohc_list[(size_t)(oi - 1u) * kOhcStride + r] = make_uint2(cid, i1 - 1u);
+            } else if (r == kOhcStride) {
+                ovf |= PT_OVF_OUTER_HIT_CELLS;
+                const uint32_t k = atomicAdd(&e.s.misc[kMiscOhcOvf], 1u);
+                if (k < kOhcOvfCap)
+                    e.s.ohc_ovf[k] = oi - 1u;
+                else
+                    err |= PT_INT_OVF_LIST;
             }
-            ++base;
-        });
+        };
+        const uint32_t nm = (uint32_t)__popcll(msk);
+        const uint32_t fincl = wave_incl_scan(nm);
+        const uint32_t fexcl = fincl - nm;
+        const uint32_t ftotal = __shfl(fincl, 63, 64);
+        const uint32_t jw = T * kAggTile + w * 64u;
+        for (uint32_t base = 0; base < ftotal; base += 64u) {
+            const uint32_t g = base + lane;
+            const int o = owner_of(fincl, g);
+            const uint32_t r = g - (uint32_t)__shfl(fexcl, o, 64);
+            if (g < ftotal) {
+                const uint32_t cid = lid[w][o] + r;
+                if (cid < cap) {
+                    const uint4 so = lpos[w][o];
+                    const uint32_t k = nth_set_bit(lmsk[w][o], r);
+                    emit(jw + (uint32_t)o, (int)(so.w >> 16), cand_pos(so.x, so.y, so.z, k), cid);
+                }
+            }
+        }
+        if (j < n_slots && (sp.w & 0xFFFFu) > 64u) {  // candidates beyond a slot's first 64: serial, rare
+            SlotCtx s;
+            const uint32_t n = slot_setup(e, cfg, j, ref, s);
+            uint32_t cid = id + nm;
+            for (uint32_t k = 64; k < n && cid < cap; ++k) {
+                const uint32_t pos = cand_pos(s, k);
+                if (cand_pass(e, cfg, s, e.s.binrec[pos], ref)) emit(j, slot_pair(s), pos, cid++);
+            }
+        }
+        wave_sync();
     }
     (void)ref;
+    wave_or_atomic(&e.cnt->overflow, ovf);
+    wave_or_atomic(&e.cnt->internal_error, err);
 }
 
-// ------------------------------------------------------------------ scans
-// Exclusive scan in place of arr[0..n), total to arr[n].  One block per event,
-// tiles of kScanBlock*kScanItems staged through LDS for coalesced access.
-enum ScanWhat : int { kScanSlots = 0, kScanOhc = 1, kScanNn = 2, kScanTcnt = 3, kScanHcnt = 4, kScanTrk = 5 };
-
-template <int W>
-__global__ void __launch_bounds__(kScanBlock) k_scan(const EvDesc *__restrict__ evs, uint32_t max_doublets) {
-    __shared__ uint32_t tile[kScanBlock * kScanItems];
-    __shared__ uint32_t lds[kScanBlock / 64 + 1];
-    const EvDesc &e = evs[blockIdx.y];
-    uint32_t *arr;
-    uint32_t n;
-    if (W == kScanSlots) {
-        arr = e.s.slot_cnt;
-        n = e.n_slots;
-    } else if (W == kScanOhc) {
-        arr = e.s.ohc_cnt;
-        n = e.n_hits;
-    } else {
-        n = e.cnt->n_cells;
-        arr = (W == kScanNn) ? e.s.nn : (W == kScanTcnt) ? e.s.tcnt : (W == kScanHcnt) ? e.s.hcnt : e.s.trk_cnt;
-    }
-    const uint32_t T = kScanBlock * kScanItems;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += T) {
-        for (uint32_t k = threadIdx.x; k < T; k += kScanBlock) tile[k] = (base + k < n) ? arr[base + k] : 0u;
-        __syncthreads();
-        uint32_t sum = 0;
-        const uint32_t t0 = threadIdx.x * kScanItems;
-#pragma unroll
-        for (uint32_t q = 0; q < kScanItems; ++q) sum += tile[t0 + q];
-        uint32_t total;
-        uint32_t run = block_excl_scan(sum, lds, total) + carry;
-#pragma unroll
-        for (uint32_t q = 0; q < kScanItems; ++q) {
-            const uint32_t v = tile[t0 + q];
-            tile[t0 + q] = run;
-            run += v;
-        }
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < T; k += kScanBlock)
-            if (base + k < n) arr[base + k] = tile[k];
-        carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        arr[n] = carry;
-        if (W == kScanSlots) {
-            e.cnt->n_cells_found = carry;
-            e.cnt->n_cells = carry < max_doublets ? carry : max_doublets;
-            if (carry > max_doublets) e.cnt->overflow |= PT_OVF_DOUBLETS;
-        } else if (W == kScanNn) {
-            e.cnt->n_connections = carry;
-        } else if (W == kScanTcnt) {
-            e.cnt->n_tuples = carry;
-            if (carry > kMaxTuples) e.cnt->overflow |= PT_OVF_TUPLES;
-        } else if (W == kScanHcnt) {
-            e.cnt->n_tuple_hits = carry;
+// Outer-hit lists that overflowed kOhcStride: rebuild them as the first 128
+// cells in id order (the entries push! keeps), one wavefront per event
+// scanning the cells.  Exits at once when no list overflowed.
+__global__ void __launch_bounds__(64) k_ohc_rebuild(const EvDesc *__restrict__ evs, EvGrid grid) {
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    uint32_t err = 0;
+    const uint32_t n_ovf = e.s.misc[kMiscOhcOvf];
+    if (n_ovf > kOhcOvfCap) err |= PT_INT_OVF_LIST;
+    const uint32_t n = min(n_ovf, kOhcOvfCap);
+    const uint32_t n_cells = e.cnt->n_cells;
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t h = e.s.ohc_ovf[q];
+        uint2 *L = e.s.ohc_list + (size_t)h * kOhcStride;
+        uint32_t r = 0;
+        for (uint32_t c0 = 0; c0 < n_cells && r < kOhcStride; c0 += 64u) {
+            const uint32_t c = c0 + lane_id();
+            const uint32_t io = (c < n_cells) ? e.s.cell_io[c] : 0u;
+            const bool match = c < n_cells && (io >> 16) == h + 1u;
+            const uint64_t bal = __ballot(match);
+            const uint32_t rank = r + (uint32_t)__popcll(bal & lanes_below());
+            if (match && rank < kOhcStride) L[rank] = make_uint2(c, (io & 0xFFFFu) - 1u);
+            r += (uint32_t)__popcll(bal);
         }
     }
-}
-
-// ------------------------------------------------------------------ outer-hit lists
-// is_outer_hit_of_cell (OuterHitOfCellVector(128, n_hits)): ascending cell ids per
-// outer hit; consumers use the first 128 entries like push! keeps them.
-__global__ void __launch_bounds__(256) k_ohc_fill(const EvDesc *__restrict__ evs) {
-    PT_EVENT_LOOP(e, c, e.cnt->n_cells) {
-        const uint32_t h0 = e.s.cell_outer[c] - 1u;
-        const uint32_t pos = atomicAdd(&e.s.ohc_cursor[h0], 1u);
-        e.s.ohc_list[e.s.ohc_cnt[h0] + pos] = c;
-    }
-}
-__global__ void __launch_bounds__(256) k_ohc_sort(const EvDesc *__restrict__ evs) {
-    uint32_t ovf = 0;
-    PT_EVENT_LOOP(e, h, e.n_hits) {
-        const uint32_t a = e.s.ohc_cnt[h], b = e.s.ohc_cnt[h + 1];
-        uint32_t *L = e.s.ohc_list;
-        for (uint32_t x = a + 1; x < b; ++x) {
-            const uint32_t v = L[x];
-            uint32_t y = x;
-            while (y > a && L[y - 1] > v) {
-                L[y] = L[y - 1];
-                --y;
-            }
-            L[y] = v;
-        }
-        if (b - a > PT_MAX_CELLS_PER_HIT) ovf |= PT_OVF_OUTER_HIT_CELLS;
-    }
-    wave_or_atomic(&evs[blockIdx.y].cnt->overflow, ovf);
+    if (err && lane_id() == 0) atomicOr(&e.cnt->internal_error, err);
 }
 
 // ------------------------------------------------------------------ connect
@@ -317,273 +587,669 @@ __device__ __forceinline__ bool dca_cut(float x1, float y1, float x2, float y2, 
     return fabsf(dca0) < region * fabsf(m_c);
 }
 
-// Candidates of outer cell o: the cells whose inner hit is o's outer hit, in
-// ascending cell id, provided o is among the first 128 cells of that hit's
-// outer list (the only ones kernel_connect visits).
-template <class F>
-__device__ __forceinline__ void for_each_candidate(const EvDesc &e, const Config &cfg, uint32_t o, uint32_t n_cells,
-                                                   uint32_t &n_eval, F &&f) {
-    const uint32_t h0 = e.s.cell_outer[o] - 1u;
-    const uint32_t la = e.s.ohc_cnt[h0], lb = e.s.ohc_cnt[h0 + 1];
-    if (lb - la > PT_MAX_CELLS_PER_HIT && o > e.s.ohc_list[la + PT_MAX_CELLS_PER_HIT - 1]) return;
-    const int L = layer_of(e, h0);
-    const uint32_t i1 = e.s.cell_inner[o] - 1u;
-    const float r1 = e.rg[i1], z1 = e.zg[i1], x1 = e.xg[i1], y1 = e.yg[i1];
-    const float region = (e.det[i1] < 96) ? cfg.p.dca_cut_inner_triplet : cfg.p.dca_cut_outer_triplet;
-    const float ri = e.rg[h0], zi = e.zg[h0], x2 = e.xg[h0], y2 = e.yg[h0];
-    const float theta = (e.det[h0] < 1184) ? cfg.p.ca_theta_cut_barrel : cfg.p.ca_theta_cut_forward;
-    const uint32_t lpos = h0 - e.layer_start[L];
-    for (uint32_t p = 0; p < e.n_pairs; ++p) {
-        if (cfg.pairs.inner[p] != L) continue;
-        const uint32_t j = ((p == 0) ? 0u : e.slot_cum[p - 1]) + lpos;
-        uint32_t a = e.s.slot_cnt[j], b = e.s.slot_cnt[j + 1];
-        if (b > n_cells) b = n_cells;
-        for (uint32_t t = a; t < b; ++t) {
-            const uint32_t h3 = e.s.cell_outer[t] - 1u;
-            const float ro = e.rg[h3], zo = e.zg[h3], x3 = e.xg[h3], y3 = e.yg[h3];
-            ++n_eval;
-            const bool ok = are_aligned(r1, z1, ri, zi, ro, zo, cfg.p.pt_min, theta) &&
-                            dca_cut(x1, y1, x2, y2, x3, y3, region, cfg.p.hard_curv_cut);
-            if (ok && !f(t)) return;
-        }
-    }
+struct ConnCtx {
+    float x1, y1, z1, r1, x2, y2, zi, ri, region, theta;
+};
+
+__device__ __forceinline__ bool conn_pass(const ConnCtx &c, const Config &cfg, float4 h) {
+    return are_aligned(c.r1, c.z1, c.ri, c.zi, h.w, h.z, cfg.p.pt_min, c.theta) &&
+           dca_cut(c.x1, c.y1, c.x2, c.y2, h.x, h.y, c.region, cfg.p.hard_curv_cut);
 }
 
-__global__ void __launch_bounds__(256) k_connect_count(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t n_eval = 0, ovf = 0;
-    PT_EVENT_LOOP(e, o, e.cnt->n_cells) {
-        uint32_t npass = 0;
-        for_each_candidate(e, cfg, o, e.cnt->n_cells, n_eval, [&](uint32_t t) {
-            e.s.cell_used[t] = 1;
-            ++npass;
-            return true;
-        });
-        if (npass) e.s.cell_used[o] = 1;
-        if (npass > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
-        e.s.nn[o] = npass < PT_MAX_NEIGHBORS ? npass : PT_MAX_NEIGHBORS;
-    }
-    const EvDesc &e0 = evs[blockIdx.y];
-    wave_add_atomic(&e0.cnt->n_connect_pairs, n_eval);
-    wave_or_atomic(&e0.cnt->overflow, ovf);
+__device__ __forceinline__ float theta_cut_of(const EvDesc &e, const Config &cfg, uint32_t h0) {
+    return ((int16_t)(e.s.hdbl[h0].z >> 16) < 1184) ? cfg.p.ca_theta_cut_barrel : cfg.p.ca_theta_cut_forward;
+}
+__device__ __forceinline__ float region_of(const Config &cfg, uint32_t det_inner) {
+    return ((int16_t)det_inner < 96) ? cfg.p.dca_cut_inner_triplet : cfg.p.dca_cut_outer_triplet;
 }
 
-__global__ void __launch_bounds__(256) k_connect_fill(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t dummy = 0, ovf = 0;
-    PT_EVENT_LOOP(e, o, e.cnt->n_cells) {
-        const uint32_t a = e.s.nn[o], b = e.s.nn[o + 1];
-        if (a == b) continue;
-        if (b > kNbrCap) {
-            ovf |= 1u;
-            continue;
-        }
-        uint32_t w = a;
-        for_each_candidate(e, cfg, o, e.cnt->n_cells, dummy, [&](uint32_t t) {
-            e.s.nbr[w++] = t;
-            return w < b;
-        });
-    }
-    if (ovf) atomicOr(&evs[blockIdx.y].s.misc[1], 1u);
+// candidate cells of the pair by_inner[L][q] starting on hit h (lpos = h's index in L)
+__device__ __forceinline__ void pair_range(const EvDesc &e, const Config &cfg, int L, uint32_t lpos, int q,
+                                           uint32_t n_cells, uint32_t &a, uint32_t &b) {
+    const int p = cfg.pairs.by_inner[L][q];
+    const uint32_t j = ((p == 0) ? 0u : e.slot_cum[p - 1]) + lpos;
+    a = e.s.slot_cnt[j];
+    b = e.s.slot_cnt[j + 1];
+    if (b > n_cells) b = n_cells;
+    if (a > b) a = b;
 }
 
-// ------------------------------------------------------------------ fishbone (gpu_fishbone.jl:4-62)
-__global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs) {
-    uint32_t killed = 0;
-    PT_EVENT_LOOP(e, h, e.n_hits) {
-        const uint32_t la = e.s.ohc_cnt[h], lb = e.s.ohc_cnt[h + 1];
-        uint32_t m = lb - la;
-        if (m < 2) continue;
-        if (m > PT_MAX_CELLS_PER_HIT) m = PT_MAX_CELLS_PER_HIT;
-        const float xo = e.xg[h], yo = e.yg[h], zo = e.zg[h];
-        const uint32_t *L = e.s.ohc_list + la;
-        for (uint32_t a = 0; a + 1 < m; ++a) {
-            const uint32_t ca = L[a];
-            if (!e.s.cell_used[ca]) continue;
-            const uint32_t ia = e.s.cell_inner[ca] - 1u;
-            const float dxa = e.xg[ia] - xo, dya = e.yg[ia] - yo, dza = e.zg[ia] - zo;
-            const float na = dxa * dxa + dya * dya + dza * dza;
-            const int16_t da = e.det[ia];
-            for (uint32_t b = a + 1; b < m; ++b) {
-                const uint32_t cb = L[b];
-                if (!e.s.cell_used[cb]) continue;
-                const uint32_t ib = e.s.cell_inner[cb] - 1u;
-                const float dxb = e.xg[ib] - xo, dyb = e.yg[ib] - yo, dzb = e.zg[ib] - zo;
-                const float nb = dxb * dxb + dyb * dyb + dzb * dzb;
-                const float cos12 = dxa * dxb + dya * dyb + dza * dzb;
-                if (da != e.det[ib] && cos12 * cos12 >= 0.99999f * na * nb) {
-                    if (na > nb) {
-                        if (!e.s.cell_killed[ca]) ++killed;
-                        e.s.cell_killed[ca] = 1;
-                        break;
-                    } else {
-                        if (!e.s.cell_killed[cb]) ++killed;
-                        e.s.cell_killed[cb] = 1;
-                    }
-                }
-            }
-        }
-    }
-    (void)killed;
-}
-
-// ------------------------------------------------------------------ ntuplets (gpu_ca_cell.jl:232-270)
-// Iterative DFS with the serial visiting order; leaf(cells, n_cells) is called
-// at every emission point of the recursive reference.
-template <class F>
-__device__ __forceinline__ void dfs(const EvDesc &e, uint32_t s, uint32_t min_doublets, uint32_t &ref_bits,
-                                    F &&leaf) {
-    uint32_t stk[4], pos[4];
-    bool child[4];
-    int d = 0;
-    stk[0] = s;
-    pos[0] = 0;
-    child[0] = false;
-    const uint32_t *nn = e.s.nn;
-    while (d >= 0) {
-        const uint32_t c = stk[d];
-        const uint32_t a = nn[c], cnt = nn[c + 1] - a;
-        bool pushed = false;
-        while (pos[d] < cnt) {
-            const uint32_t n = e.s.nbr[a + pos[d]];
-            pos[d]++;
-            if (n >= e.cnt->n_cells || e.s.cell_killed[n]) continue;
-            child[d] = true;
-            if (d + 1 >= 4) {  // the reference asserts: at most 4 doublets in a chain
-                ref_bits |= PT_REF_DEPTH;
-                continue;
-            }
-            ++d;
-            stk[d] = n;
-            pos[d] = 0;
-            child[d] = false;
-            pushed = true;
-            break;
-        }
-        if (pushed) continue;
-        if (!child[d] && (uint32_t)(d + 1) >= min_doublets) leaf(stk, (uint32_t)(d + 1));
-        --d;
-    }
-}
-
-__device__ __forceinline__ bool is_start(const EvDesc &e, const Config &cfg, uint32_t c) {
-    if (e.s.cell_killed[c]) return false;
-    const int p = e.s.cell_pair[c];
+// ntuplet start cells: pair < 8 || pair > 12 (pair < 3 when min_hits > 3)
+// (ca_hit_ntuplet_generator_kernels_impl.jl:116-132)
+__device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
     return (cfg.p.min_hits_per_ntuplet > 3) ? (p < 3) : (p < 8 || p > 12);
 }
 
-__global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t ref = 0, killed = 0;
+// Per hit h, the rows are the cells ending on h that kernel_connect visits (the
+// first 128 of h's list) and the columns the cells starting on h, in id order
+// (one contiguous id range per layer pair).  A wavefront takes kHitTile hits
+// per ticket and packs them into groups of at most kRowsGrp rows; the rows of
+// a group are loaded once into LDS, and every (row, column) pair of the group
+// is one lane's work, kEvalU rounds of column loads in flight.  The passing
+// columns of a row, in order, are the row cell's outer neighbours: the rank of
+// a pass within its row is the passes of earlier rounds (per-row count in LDS)
+// plus those of lower lanes of the same row.
+constexpr uint32_t kHitTile = 16;
+constexpr uint32_t kRowsGrp = 128;
+
+struct ConnLds {
+    // per hit of the tile
+    float hx[kHitTile], hy[kHitTile], hz[kHitTile], hr[kHitTile], theta[kHitTile], rcpc[kHitTile];
+    uint32_t cols[kHitTile], a[6][kHitTile], cum[6][kHitTile];
+    // per row of the group
+    uint32_t rcell[kRowsGrp], rcnt[kRowsGrp];
+    float rx[kRowsGrp], ry[kRowsGrp], rz[kRowsGrp], rr[kRowsGrp], rreg[kRowsGrp];
+};
+
+__global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    __shared__ ConnLds lds_all[4];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    ConnLds &S = lds_all[w];
+    const uint32_t n_cells = e.cnt->n_cells, n_hits = e.n_hits;
+    uint32_t *nbr4u = reinterpret_cast<uint32_t *>(e.s.nbr4);
+    uint32_t n_eval = 0, n_conn = 0, n_lists = 0, ovf = 0;
+    for (;;) {
+        const uint32_t h0 = take_ticket(&e.s.misc[kMiscTkConn]) * kHitTile;
+        if (h0 >= n_hits) break;
+        wave_sync();  // the previous tile's reads of S are done
+        const uint32_t h = h0 + lane;
+        uint32_t R = 0, C = 0;
+        if (lane < kHitTile && h < n_hits) {
+            R = min(e.s.ohc_cursor[h], kOhcStride);
+            if (R) {
+                const int L = layer_of(e, h);
+                const int np = cfg.pairs.n_by_inner[L];
+                const uint32_t lpos = h - e.layer_start[L];
+                uint32_t ra[6], rb[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    ra[q] = rb[q] = 0;
+                    if (q < np) pair_range(e, cfg, L, lpos, q, n_cells, ra[q], rb[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    S.a[q][lane] = ra[q];
+                    C += rb[q] - ra[q];
+                    S.cum[q][lane] = C;
+                }
+                if (C) {
+                    const float4 p = e.s.hxyzr[h];
+                    S.hx[lane] = p.x;
+                    S.hy[lane] = p.y;
+                    S.hz[lane] = p.z;
+                    S.hr[lane] = p.w;
+                    S.theta[lane] = theta_cut_of(e, cfg, h);
+                    S.rcpc[lane] = 1.0f / (float)C;
+                }
+            }
+            if (!C) R = 0;
+            S.cols[lane] = C;
+        }
+        wave_sync();
+        for (uint32_t first = 0; first < kHitTile;) {
+            const uint32_t v = (lane >= first) ? R : 0u;
+            const uint32_t rincl = wave_incl_scan(v);
+            const uint64_t fits = __ballot(lane >= first && lane < kHitTile && rincl <= kRowsGrp);
+            const uint32_t last = 63u - (uint32_t)__clzll(fits);
+            const uint32_t rows = __shfl(rincl, (int)last, 64);
+            if (rows) {
+                const uint32_t rexcl = rincl - v;
+                // stage the rows: cell, inner-hit coordinates, dca region
+#pragma unroll
+                for (uint32_t u = 0; u < kRowsGrp / 64u; ++u) {
+                    const uint32_t rr = 64u * u + lane;
+                    const int o = owner_of(rincl, rr);
+                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    if (rr < rows) {
+                        const uint2 en = e.s.ohc_list[(size_t)(h0 + (uint32_t)o) * kOhcStride + (rr - s0)];
+                        const float4 p1 = e.s.hxyzr[en.y];
+                        S.rcell[rr] = en.x;
+                        S.rcnt[rr] = 0u;
+                        S.rx[rr] = p1.x;
+                        S.ry[rr] = p1.y;
+                        S.rz[rr] = p1.z;
+                        S.rr[rr] = p1.w;
+                        S.rreg[rr] = region_of(cfg, e.s.hdbl[en.y].z >> 16);
+                    }
+                }
+                wave_sync();
+                const uint32_t items = (lane >= first && lane <= last) ? R * C : 0u;
+                const uint32_t iincl = wave_incl_scan(items);
+                const uint32_t iexcl = iincl - items;
+                const uint32_t total = __shfl(iincl, 63, 64);
+                for (uint32_t base = 0; base < total; base += 64u * kEvalU) {
+                    uint32_t t[kEvalU], rs[kEvalU], rst[kEvalU], cl[kEvalU];
+                    int ow[kEvalU];
+                    float4 o3[kEvalU];
+#pragma unroll
+                    for (int u = 0; u < kEvalU; ++u) {
+                        const uint32_t g = base + 64u * u + lane;
+                        const int o = owner_of(iincl, g);
+                        const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), orb = (uint32_t)__shfl(rexcl, o, 64);
+                        ow[u] = o;
+                        t[u] = 0;
+                        rs[u] = 0;
+                        rst[u] = 0;
+                        cl[u] = 0;
+                        o3[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (g < total) {
+                            const uint32_t k = g - ox, cols = S.cols[o];
+                            uint32_t row = (uint32_t)((float)k * S.rcpc[o]);
+                            if (row * cols > k) --row;
+                            if ((row + 1u) * cols <= k) ++row;
+                            const uint32_t col = k - row * cols;
+                            uint32_t q = 0, q0 = 0;
+                            while (col >= S.cum[q][o]) {
+                                q0 = S.cum[q][o];
+                                ++q;
+                            }
+                            t[u] = S.a[q][o] + col - q0;
+                            rs[u] = orb + row;
+                            rst[u] = ox + row * cols;
+                            cl[u] = cols;
+                            o3[u] = e.s.cell_oxyzr[t[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kEvalU; ++u) {
+                        const uint32_t rbase = base + 64u * u;
+                        const uint32_t g = rbase + lane;
+                        const bool act = g < total;
+                        bool pass = false;
+                        if (act) {
+                            const int o = ow[u];
+                            const uint32_t r = rs[u];
+                            const ConnCtx c{S.rx[r], S.ry[r], S.rz[r], S.rr[r], S.hx[o], S.hy[o], S.hz[o], S.hr[o],
+                                            S.rreg[r], S.theta[o]};
+                            pass = conn_pass(c, cfg, o3[u]);
+                        }
+                        const uint64_t pm = __ballot(pass);
+                        const uint32_t lo = (act && rst[u] > rbase) ? rst[u] - rbase : 0u;
+                        const uint32_t prev = act ? S.rcnt[rs[u]] : 0u;
+                        wave_sync();
+                        if (pass) {
+                            const uint32_t cell = S.rcell[rs[u]];
+                            const uint32_t rk = prev + (uint32_t)__popcll(pm & lanes_below() & ~range_mask(0u, lo));
+                            if (rk < 4u) nbr4u[(size_t)cell * 4u + rk] = t[u];
+                            e.s.cell_flags[t[u]] = kUsed;
+                            e.s.cell_flags[cell] = kUsed;
+                            e.s.trk[t[u]] = make_uint2(0u, 0u);
+                            e.s.trk[cell] = make_uint2(0u, 0u);
+                        }
+                        if (act && (g + 1u == rst[u] + cl[u] || lane == 63u))
+                            S.rcnt[rs[u]] = prev + (uint32_t)__popcll(pm & range_mask(lo, lane + 1u));
+                        wave_sync();
+                    }
+                }
+                // the group's rows: store the neighbour count, queue long lists and start cells
+                for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
+                    const uint32_t rr = r0 + lane;
+                    const uint32_t c = (rr < rows) ? S.rcnt[rr] : 0u;
+                    const uint32_t cell = c ? S.rcell[rr] : 0u;
+                    bool st = false;
+                    if (c) {
+                        const uint32_t nn = c < PT_MAX_NEIGHBORS ? c : PT_MAX_NEIGHBORS;
+                        e.s.cell_nn[cell] = (uint8_t)nn;
+                        n_conn += nn;
+                        ++n_lists;
+                        if (c > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
+                        if (nn > 4u) e.s.ext_list[atomicAdd(&e.s.misc[kMiscExt], 1u)] = cell;
+                        st = start_pair(cfg, e.s.cell_pair[cell]);
+                    }
+                    const uint64_t sb = __ballot(st);
+                    if (sb) {
+                        uint32_t sbase = 0;
+                        if (lane == 0) sbase = atomicAdd(&e.s.misc[kMiscStart], (uint32_t)__popcll(sb));
+                        sbase = __shfl(sbase, 0, 64);
+                        if (st) {
+                            e.s.start_list[sbase + (uint32_t)__popcll(sb & lanes_below())] = cell;
+                            atomicOr(reinterpret_cast<unsigned long long *>(&e.s.start_bits[cell >> 6]),
+                                     1ull << (cell & 63u));
+                        }
+                    }
+                }
+                if (lane == 0) n_eval += total;
+                wave_sync();
+            }
+            first = last + 1u;
+        }
+    }
+    wave_add_atomic(&e.cnt->n_connect_pairs, n_eval);
+    wave_add_atomic(&e.cnt->n_connections, n_conn);
+    wave_add_atomic(&e.s.misc[kMiscNbrLists], n_lists);
+    wave_or_atomic(&e.cnt->overflow, ovf);
+}
+
+// Cells with more than 4 neighbours (rare): walk the candidates again in id
+// order and put neighbours 5..64 in the per-event pool.
+__global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    uint32_t err = 0;
+    PT_EVENT_LOOP(e, i, e.s.misc[kMiscExt]) {
+        const uint32_t o = e.s.ext_list[i];
+        const uint32_t nkeep = e.s.cell_nn[o];
+        const uint32_t base = atomicAdd(&e.s.misc[kMiscPool], nkeep - 4u);
+        if (base + (nkeep - 4u) > kNbrPool) {
+            err |= PT_INT_NBR_POOL;
+            e.s.cell_nn[o] = 4;
+            continue;
+        }
+        e.s.nbr_ext[o] = base;
+        const uint32_t n_cells = e.cnt->n_cells;
+        const uint32_t io = e.s.cell_io[o];
+        const uint32_t h0 = (io >> 16) - 1u, i1 = (io & 0xFFFFu) - 1u;
+        const int L = layer_of(e, h0);
+        const int np = cfg.pairs.n_by_inner[L];
+        const uint32_t lpos = h0 - e.layer_start[L];
+        const float4 p1 = e.s.hxyzr[i1], p2 = e.s.hxyzr[h0];
+        const ConnCtx c{p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w, region_of(cfg, e.s.hdbl[i1].z >> 16),
+                        theta_cut_of(e, cfg, h0)};
+        uint32_t k = 0;
+        for (int q = 0; q < np && k < nkeep; ++q) {
+            uint32_t a, bq;
+            pair_range(e, cfg, L, lpos, q, n_cells, a, bq);
+            for (uint32_t t = a; t < bq && k < nkeep; ++t) {
+                if (!conn_pass(c, cfg, e.s.cell_oxyzr[t])) continue;
+                if (k >= 4u) e.s.nbr_pool[base + k - 4u] = t;
+                ++k;
+            }
+        }
+    }
+    wave_or_atomic(&e.cnt->internal_error, err);
+}
+
+// ------------------------------------------------------------------ fishbone (gpu_fishbone.jl:4-62)
+// The serial loop, per hit, over the ascending cell list L (first 128):
+//   for ic, for jc > ic, both used: if collinear and on different modules:
+//     if |ic| > |jc| (inner-hit distance): kill ic and stop; else kill jc.
+// Kills never feed back into the loop, so it is evaluated per cell pair: with
+// first(ic) = smallest qualifying jc with |jc| < |ic|, ic dies if first(ic)
+// exists, and jc dies if some ic qualifies with it and jc < first(ic).
+// A wavefront takes kHitTile hits per ticket, packs them into groups of at
+// most kFishRows list entries (rows, loaded once into LDS) and spreads every
+// unordered row pair of a group over the lanes; a second step decides each row.
+constexpr uint32_t kFishRows = 128;
+
+struct FishLds {
+    uint32_t id[kFishRows], ud[kFishRows], first[kFishRows];  // ud: module (16 b) | used << 16
+    float dx[kFishRows], dy[kFishRows], dz[kFishRows], n[kFishRows];
+    unsigned long long q[2][kFishRows];  // rows (index in the hit) qualifying with this one as ic
+};
+
+// pair k of the b-major enumeration (0,1), (0,2), (1,2), (0,3), ...: (k - b(b-1)/2, b)
+__device__ __forceinline__ uint32_t tri_b(uint32_t k) {
+    uint32_t bb = (uint32_t)((1.0f + sqrtf(8.0f * (float)k + 1.0f)) * 0.5f);
+    if (bb * (bb - 1u) / 2u > k) --bb;
+    if ((bb + 1u) * bb / 2u <= k) ++bb;
+    return bb;
+}
+
+__global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ FishLds lds_all[4];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    FishLds &S = lds_all[w];
+    const uint32_t n_hits = e.n_hits;
+    uint32_t killed = 0;
+    for (;;) {
+        const uint32_t h0 = take_ticket(&e.s.misc[kMiscTkFish]) * kHitTile;
+        if (h0 >= n_hits) break;
+        const uint32_t h = h0 + lane;
+        const uint32_t cur = (lane < kHitTile && h < n_hits) ? e.s.ohc_cursor[h] : 0u;
+        const uint32_t R = (cur >= 2u) ? (cur < kOhcStride ? cur : kOhcStride) : 0u;
+        for (uint32_t first = 0; first < kHitTile;) {
+            const uint32_t v = (lane >= first) ? R : 0u;
+            const uint32_t rincl = wave_incl_scan(v);
+            const uint64_t fits = __ballot(lane >= first && lane < kHitTile && rincl <= kFishRows);
+            const uint32_t last = 63u - (uint32_t)__clzll(fits);
+            const uint32_t rows = __shfl(rincl, (int)last, 64);
+            if (rows) {
+                const uint32_t rexcl = rincl - v;
+                wave_sync();  // the previous group's reads of S are done
+#pragma unroll
+                for (uint32_t u = 0; u < kFishRows / 64u; ++u) {
+                    const uint32_t rr = 64u * u + lane;
+                    const int o = owner_of(rincl, rr);
+                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    if (rr < rows) {
+                        const uint32_t hh = h0 + (uint32_t)o;
+                        const uint2 en = e.s.ohc_list[(size_t)hh * kOhcStride + (rr - s0)];
+                        const float4 po = e.s.hxyzr[hh];
+                        const float4 pi = e.s.hxyzr[en.y];
+                        const float dx = pi.x - po.x, dy = pi.y - po.y, dz = pi.z - po.z;
+                        S.id[rr] = en.x;
+                        S.ud[rr] = (e.s.hdbl[en.y].z >> 16) | ((e.s.cell_flags[en.x] & kUsed) ? (1u << 16) : 0u);
+                        S.dx[rr] = dx;
+                        S.dy[rr] = dy;
+                        S.dz[rr] = dz;
+                        S.n[rr] = dx * dx + dy * dy + dz * dz;
+                        S.first[rr] = ~0u;
+                        S.q[0][rr] = 0ull;
+                        S.q[1][rr] = 0ull;
+                    }
+                }
+                wave_sync();
+                // every unordered pair of rows of the same hit
+                const uint32_t items = (lane >= first && lane <= last) ? R * (R - 1u) / 2u : 0u;
+                const uint32_t iincl = wave_incl_scan(items);
+                const uint32_t iexcl = iincl - items;
+                const uint32_t total = __shfl(iincl, 63, 64);
+                for (uint32_t base = 0; base < total; base += 64u) {
+                    const uint32_t g = base + lane;
+                    const int o = owner_of(iincl, g);
+                    const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    if (g < total) {
+                        const uint32_t k = g - ox;
+                        const uint32_t bb = tri_b(k), aa = k - bb * (bb - 1u) / 2u;
+                        const uint32_t ra = s0 + aa, rb = s0 + bb;
+                        const uint32_t ida = S.id[ra], idb = S.id[rb];
+                        const bool a_ic = ida < idb;
+                        const uint32_t ri = a_ic ? ra : rb, rj = a_ic ? rb : ra;  // ic, jc rows
+                        const uint32_t ui = S.ud[ri], uj = S.ud[rj];
+                        if (((ui & uj) >> 16) && (ui & 0xFFFFu) != (uj & 0xFFFFu)) {
+                            const float ni = S.n[ri], nj = S.n[rj];
+                            const float cos12 = S.dx[ri] * S.dx[rj] + S.dy[ri] * S.dy[rj] + S.dz[ri] * S.dz[rj];
+                            if (cos12 * cos12 >= 0.99999f * ni * nj) {
+                                const uint32_t jl = rj - s0;
+                                atomicOr(&S.q[jl >> 6][ri], 1ull << (jl & 63u));
+                                if (ni > nj) atomicMin(&S.first[ri], a_ic ? idb : ida);
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+                // decide each row
+                for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
+                    const uint32_t rr = r0 + lane;
+                    const int o = owner_of(rincl, rr);
+                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64), n_o = (uint32_t)__shfl(v, o, 64);
+                    if (rr < rows) {
+                        const uint32_t myid = S.id[rr];
+                        bool kill = S.first[rr] != ~0u;
+                        const uint32_t ml = rr - s0;
+                        const unsigned long long bit = 1ull << (ml & 63u);
+                        for (uint32_t x = s0; x < s0 + n_o && !kill; ++x)
+                            if ((S.q[ml >> 6][x] & bit) && myid < S.first[x]) kill = true;
+                        if (kill) {
+                            e.s.cell_flags[myid] = kUsed | kKilled;
+                            ++killed;
+                        }
+                    }
+                }
+            }
+            first = last + 1u;
+        }
+    }
+    wave_add_atomic(&e.cnt->n_killed, killed);
+}
+
+// ------------------------------------------------------------------ ntuplets (gpu_ca_cell.jl:232-270)
+// DFS with the serial visiting order, one template level per depth (the
+// reference asserts at most 4 doublets per chain, gpu_ca_cell.jl:234), so the
+// path lives in registers.  leaf(path, n_cells) is called at every emission
+// point of the recursive reference.
+__device__ __forceinline__ uint32_t nbr_at(const EvDesc &e, uint4 v, uint32_t ext, uint32_t i) {
+    return (i == 0) ? v.x : (i == 1) ? v.y : (i == 2) ? v.z : (i == 3) ? v.w : e.s.nbr_pool[ext + i - 4u];
+}
+
+template <int D, class F>
+__device__ __forceinline__ void dfs_level(const EvDesc &e, uint32_t (&stk)[4], uint32_t min_doublets,
+                                          uint32_t &ref_bits, F &leaf) {
+    const uint32_t c = stk[D];
+    const uint32_t nn = e.s.cell_nn[c];
+    bool child = false;
+    if (nn) {
+        const uint4 v = e.s.nbr4[c];
+        const uint32_t ext = (nn > 4u) ? e.s.nbr_ext[c] : 0u;
+        for (uint32_t i = 0; i < nn; ++i) {
+            const uint32_t n = nbr_at(e, v, ext, i);
+            if (e.s.cell_flags[n] & kKilled) continue;
+            child = true;
+            if constexpr (D + 1 < 4) {
+                stk[D + 1] = n;
+                dfs_level<D + 1>(e, stk, min_doublets, ref_bits, leaf);
+            } else {
+                ref_bits |= PT_REF_DEPTH;  // a fifth doublet: the reference's @assert fails
+            }
+        }
+    }
+    if (!child && (uint32_t)(D + 1) >= min_doublets) leaf(stk, (uint32_t)(D + 1));
+}
+
+template <class F>
+__device__ __forceinline__ void dfs(const EvDesc &e, uint32_t s, uint32_t min_doublets, uint32_t &ref_bits, F &&leaf) {
+    uint32_t stk[4] = {s, 0u, 0u, 0u};
+    dfs_level<0>(e, stk, min_doublets, ref_bits, leaf);
+}
+
+// Start candidates (cells with neighbours on a start pair) that the fishbone
+// left alive: count their tuples and tuple hits, one lane per candidate, and
+// sum them per tile of kAggTile cells.
+__global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    uint32_t ref = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
-    PT_EVENT_LOOP(e, c, e.cnt->n_cells) {
-        killed += e.s.cell_killed[c];
-        uint32_t nt = 0, nh = 0;
-        if (is_start(e, cfg, c)) {
+    PT_EVENT_LOOP(e, i, e.s.misc[kMiscStart]) {
+        const uint32_t c = e.s.start_list[i];
+        uint64_t v = 0;
+        if (!(e.s.cell_flags[c] & kKilled)) {
+            uint32_t nt = 0, nh = 0;
             dfs(e, c, min_d, ref, [&](const uint32_t *, uint32_t len) {
                 ++nt;
                 nh += len + 1;
             });
+            v = ((uint64_t)nt << 32) | nh;
         }
-        e.s.tcnt[c] = nt;
-        e.s.hcnt[c] = nh;
-        e.s.trk_cnt[c] = 0;
-        e.s.trk_cursor[c] = 0;
+        e.s.tup[c] = v;
+        if (v) atomicAdd(reinterpret_cast<unsigned long long *>(&e.s.dfs_agg[c / kAggTile]), (unsigned long long)v);
     }
-    wave_or_atomic(&evs[blockIdx.y].cnt->ref_assert, ref);
-    wave_add_atomic(&evs[blockIdx.y].cnt->n_killed, killed);
+    wave_or_atomic(&e.cnt->ref_assert, ref);
 }
 
-__global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs, Config cfg) {
-    uint32_t ref = 0;
+// Exclusive scan, in cell order, of the start cells' (tuples, hits): block per
+// tile of kAggTile cells, the tile's prefix summed from the preceding tiles.
+__global__ void __launch_bounds__(256) k_dfs_scan(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ uint64_t lsum[4];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t n_cells = e.cnt->n_cells;
+    const uint32_t n_tiles = (n_cells + kAggTile - 1) / kAggTile;
+    for (uint32_t T = b; T < n_tiles; T += grid.per_event) {
+        uint64_t part = 0;
+        for (uint32_t k = threadIdx.x; k < T; k += 256u) part += e.s.dfs_agg[k];
+        const uint64_t pre = block_sum(part, lsum);
+        const uint32_t c = T * kAggTile + threadIdx.x;
+        const bool st = c < n_cells && ((e.s.start_bits[c >> 6] >> (c & 63u)) & 1ull);
+        const uint64_t v = st ? e.s.tup[c] : 0ull;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan4(v, lsum, tot);
+        if (v) e.s.tup[c] = pre + ex;
+        if (T == n_tiles - 1 && threadIdx.x == 0) {
+            const uint64_t all = pre + tot;
+            e.cnt->n_tuples = (uint32_t)(all >> 32);
+            e.cnt->n_tuple_hits = (uint32_t)all;
+            if ((uint32_t)(all >> 32) > kMaxTuples) atomicOr(&e.cnt->overflow, PT_OVF_TUPLES);
+        }
+    }
+}
+
+// Emit the tuples of each live start candidate at its scanned position:
+// container offsets and hit ids (bulk_fill), the tuple's cells, quality bad,
+// and per cell the tuple count and largest tuple size for the early duplicate
+// remover.  The first 32768 tuples are stored, like the capped container.
+__global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    uint32_t ref = 0, err = 0, ovf = 0, lists = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
-    PT_EVENT_LOOP(e, c, e.cnt->n_cells) {
-        uint32_t it = e.s.tcnt[c];  // 0-based id of the first tuple of this start cell
-        if (it == e.s.tcnt[c + 1]) continue;
-        uint32_t hp = e.s.hcnt[c];
+    PT_EVENT_LOOP(e, i, e.s.misc[kMiscStart]) {
+        const uint32_t c = e.s.start_list[i];
+        if (e.s.cell_flags[c] & kKilled) continue;
+        const uint64_t pre = e.s.tup[c];
+        uint32_t it = (uint32_t)(pre >> 32);  // 0-based id of this cell's first tuple
+        uint32_t hp = (uint32_t)pre;
         dfs(e, c, min_d, ref, [&](const uint32_t *stk, uint32_t len) {
             ++it;  // 1-based tuple id (bulk_fill)
             if (it <= kMaxTuples) {
                 e.hit_off[it - 1] = hp + 1u;
-                for (uint32_t q = 0; q < len; ++q) e.hit_bins[hp + q] = e.s.cell_inner[stk[q]];
-                e.hit_bins[hp + len] = e.s.cell_outer[stk[len - 1]];
+                uint4 tc = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    if (q < len) {
+                        const uint32_t cq = stk[q];
+                        const uint32_t io = e.s.cell_io[cq];
+                        e.hit_bins[hp + q] = (uint16_t)(io & 0xFFFFu);
+                        if (q + 1 == len) e.hit_bins[hp + len] = (uint16_t)(io >> 16);
+                        const uint32_t before = atomicAdd(&e.s.trk[cq].x, 1u);
+                        atomicMax(&e.s.trk[cq].y, len + 1u);
+                        if (before == 0u) ++lists;
+                        if (before == PT_MAX_CELL_TRACKS) {  // list capped: exact path
+                            ovf |= PT_OVF_CELL_TRACKS;
+                            const uint32_t k = atomicAdd(&e.s.misc[kMiscTrkOvf], 1u);
+                            if (k < kTrkOvfCap)
+                                e.s.trk_ovf[k] = cq;
+                            else
+                                err |= PT_INT_OVF_LIST;
+                        }
+                        if (q == 0) tc.x = cq;
+                        if (q == 1) tc.y = cq;
+                        if (q == 2) tc.z = cq;
+                        if (q == 3) tc.w = cq;
+                    }
+                }
+                reinterpret_cast<uint4 *>(e.s.tuple_cells)[it - 1] = tc;
                 e.quality[it - 1] = PT_QUALITY_BAD;
-                uint32_t *tc = e.s.tuple_cells + 4u * (it - 1u);
-                for (uint32_t q = 0; q < 4; ++q) tc[q] = (q < len) ? stk[q] : 0xFFFFFFFFu;
-                for (uint32_t q = 0; q < len; ++q) atomicAdd(&e.s.trk_cnt[stk[q]], 1u);
             }
             hp += len + 1;
         });
     }
+    wave_or_atomic(&e.cnt->overflow, ovf);
+    wave_or_atomic(&e.cnt->internal_error, err);
+    wave_add_atomic(&e.s.misc[kMiscTrkLists], lists);
     (void)ref;
 }
 
-// bulk_finalize_fill (hist_to_container.jl:150-161)
-__global__ void __launch_bounds__(256) k_finalize_tuples(const EvDesc *__restrict__ evs) {
-    PT_EVENT_LOOP(e, i, kMaxTuples + 1u) {
-        const uint32_t m = e.cnt->n_tuples, n = e.cnt->n_tuple_hits;
-        if (m > kMaxTuples) {
-            if (i == kMaxTuples) e.hit_off[kMaxTuples] = e.hit_off[kMaxTuples - 1];
-        } else if (i >= m) {
-            e.hit_off[i] = n + 1u;
-        }
-    }
-}
-
-// cell -> tuple lists (CellTracksVector), ascending tuple id
-__global__ void __launch_bounds__(256) k_trk_fill(const EvDesc *__restrict__ evs) {
-    PT_EVENT_LOOP(e, t, (e.cnt->n_tuples < kMaxTuples ? e.cnt->n_tuples : kMaxTuples)) {
-        const uint32_t *tc = e.s.tuple_cells + 4u * t;
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t c = tc[q];
-            if (c == 0xFFFFFFFFu) break;
-            const uint32_t pos = atomicAdd(&e.s.trk_cursor[c], 1u);
-            const uint32_t w = e.s.trk_cnt[c] + pos;
-            if (w < kTrkCap) e.s.trk_list[w] = (uint16_t)(t + 1u);
-        }
-    }
-}
-__global__ void __launch_bounds__(256) k_trk_sort_dup(const EvDesc *__restrict__ evs) {
-    uint32_t ovf = 0;
-    PT_EVENT_LOOP(e, c, e.cnt->n_cells) {
-        const uint32_t a = e.s.trk_cnt[c], b = e.s.trk_cnt[c + 1];
-        if (b - a < 2) continue;
-        uint16_t *L = e.s.trk_list;
-        for (uint32_t x = a + 1; x < b; ++x) {
-            const uint16_t v = L[x];
-            uint32_t y = x;
-            while (y > a && L[y - 1] > v) {
-                L[y] = L[y - 1];
-                --y;
+// ------------------------------------------------------------------ after the ntuplets
+// Per stored tuple: kernel_early_duplicate_remover
+// (ca_hit_ntuplet_generator_kernels_impl.jl:143-166) - a tuple is dup if one of
+// its cells carries a longer tuple.  Cells whose list was capped at 64, or
+// that hold the last stored tuple when the container overflowed (its size
+// reads as 0 after finalize), are left to the exact path in k_multiplicity.
+// Per container offset: bulk_finalize_fill (hist_to_container.jl:150-161) and
+// the module-index copy kernel_fill_hit_indices
+// (ca_hit_ntuplet_generator_kernels_impl.jl:25-38).
+__global__ void __launch_bounds__(256) k_tuples_post(const EvDesc *__restrict__ evs, EvGrid grid) {
+    PT_EVENT_LOOP(e, i, kMaxTuples + 1u + min(e.cnt->n_tuple_hits, PT_MAX_TUPLE_HITS)) {
+        const uint32_t m = e.cnt->n_tuples, nhits = e.cnt->n_tuple_hits;
+        const uint32_t mc = m < kMaxTuples ? m : kMaxTuples;
+        if (i <= kMaxTuples) {
+            if (i < mc) {
+                const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[i];
+                uint4 last = make_uint4(~0u, ~0u, ~0u, ~0u);
+                if (m > kMaxTuples) last = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[kMaxTuples - 1];
+                const uint32_t cs[4] = {tc.x, tc.y, tc.z, tc.w};
+                const uint32_t len = (tc.y == ~0u) ? 1u : (tc.z == ~0u) ? 2u : (tc.w == ~0u) ? 3u : 4u;
+                bool dup = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t cq = cs[q];
+                    if (cq == ~0u) break;
+                    if (cq == last.x || cq == last.y || cq == last.z || cq == last.w) continue;
+                    const uint2 tr = e.s.trk[cq];
+                    if (tr.x < 2u || tr.x > PT_MAX_CELL_TRACKS) continue;
+                    if (tr.y != len + 1u) dup = true;
+                }
+                if (dup) e.quality[i] = PT_QUALITY_DUP;
             }
-            L[y] = v;
-        }
-        uint32_t m = b - a;
-        if (m > PT_MAX_CELL_TRACKS) {
-            ovf |= PT_OVF_CELL_TRACKS;
-            m = PT_MAX_CELL_TRACKS;
-        }
-        // kernel_early_duplicate_remover (ca_hit_ntuplet_generator_kernels_impl.jl:143-166)
-        uint32_t maxh = 0;
-        for (uint32_t k = 0; k < m; ++k) {
-            const uint32_t it = L[a + k];
-            const uint32_t sz = e.hit_off[it] - e.hit_off[it - 1];
-            maxh = sz > maxh ? sz : maxh;
-        }
-        for (uint32_t k = 0; k < m; ++k) {
-            const uint32_t it = L[a + k];
-            if (e.hit_off[it] - e.hit_off[it - 1] != maxh) e.quality[it - 1] = PT_QUALITY_DUP;
+            uint32_t off;
+            if (m > kMaxTuples) {
+                off = (i == kMaxTuples) ? e.hit_off[kMaxTuples - 1] : e.hit_off[i];
+                if (i == kMaxTuples) e.hit_off[kMaxTuples] = off;
+            } else if (i >= m) {
+                off = nhits + 1u;
+                e.hit_off[i] = off;
+            } else {
+                off = e.hit_off[i];
+            }
+            e.det_off[i] = off;
+        } else {
+            const uint32_t k = i - (kMaxTuples + 1u);
+            const uint32_t end = (m > kMaxTuples) ? e.hit_off[kMaxTuples - 1] : nhits + 1u;
+            if (k + 1u < end) e.det_bins[k] = (uint16_t)e.det[e.hit_bins[k] - 1u];
         }
     }
-    wave_or_atomic(&evs[blockIdx.y].cnt->overflow, ovf);
 }
 
 // ------------------------------------------------------------------ multiplicity
+// Exact early-duplicate pass for the cells set aside above: the cell's list is
+// its first 64 tuples in id order; tuples shorter than the longest are dup.
+__device__ void trk_exact(const EvDesc &e, uint32_t cell, uint32_t mc) {
+    const uint4 *tcs = reinterpret_cast<const uint4 *>(e.s.tuple_cells);
+    auto size_of = [&](uint32_t t) { return e.hit_off[t + 1] - e.hit_off[t]; };
+    uint32_t r = 0, maxh = 0;
+    for (uint32_t t0 = 0; t0 < mc && r < PT_MAX_CELL_TRACKS; t0 += 64u) {
+        const uint32_t t = t0 + lane_id();
+        bool has = false;
+        if (t < mc) {
+            const uint4 tc = tcs[t];
+            has = tc.x == cell || tc.y == cell || tc.z == cell || tc.w == cell;
+        }
+        const uint64_t bal = __ballot(has);
+        const bool in = has && r + (uint32_t)__popcll(bal & lanes_below()) < PT_MAX_CELL_TRACKS;
+        maxh = max(maxh, wave_max(in ? size_of(t) : 0u));
+        r += (uint32_t)__popcll(bal);
+    }
+    if (r < 2u) return;
+    r = 0;
+    for (uint32_t t0 = 0; t0 < mc && r < PT_MAX_CELL_TRACKS; t0 += 64u) {
+        const uint32_t t = t0 + lane_id();
+        bool has = false;
+        if (t < mc) {
+            const uint4 tc = tcs[t];
+            has = tc.x == cell || tc.y == cell || tc.z == cell || tc.w == cell;
+        }
+        const uint64_t bal = __ballot(has);
+        const bool in = has && r + (uint32_t)__popcll(bal & lanes_below()) < PT_MAX_CELL_TRACKS;
+        if (in && size_of(t) != maxh) e.quality[t] = PT_QUALITY_DUP;
+        r += (uint32_t)__popcll(bal);
+    }
+}
+
 // kernel_countMultiplicity / finalize! / kernel_fillMultiplicity
 // (ca_hit_ntuplet_generator_kernels_impl.jl:174-211): bucket = hit count, each
 // bucket lists tuple ids in DESCENDING order (fill_direct decrements).
-__global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict__ evs) {
+__global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint32_t lds[5][1024 / 64 + 1];
     __shared__ uint32_t tot[5];
-    const EvDesc &e = evs[blockIdx.y];
+    uint32_t ev, blk;
+    if (!ev_block(grid, ev, blk)) return;
+    const EvDesc &e = evs[ev];
     const uint32_t m = e.cnt->n_tuples < kMaxTuples ? e.cnt->n_tuples : kMaxTuples;
+    {
+        const uint32_t n_ovf = min(e.s.misc[kMiscTrkOvf], kTrkOvfCap);
+        const uint32_t extra = (e.cnt->n_tuples > kMaxTuples) ? 4u : 0u;
+        const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (uint32_t k = wv; k < n_ovf + extra; k += nw) {
+            uint32_t cell;
+            if (k < n_ovf) {
+                cell = e.s.trk_ovf[k];
+            } else {
+                cell = e.s.tuple_cells[4u * (kMaxTuples - 1u) + (k - n_ovf)];
+                if (cell == ~0u) continue;
+            }
+            trk_exact(e, cell, m);
+        }
+        if (threadIdx.x == 0) {
+            if (e.cnt->n_cells == 0) atomicOr(&e.cnt->ref_assert, PT_REF_NO_CELLS);
+            uint32_t o = 0;
+            if (e.s.misc[kMiscNbrLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_NEIGHBOR_LISTS;
+            if (e.s.misc[kMiscTrkLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_TRACK_LISTS;
+            if (o) atomicOr(&e.cnt->overflow, o);
+        }
+        __syncthreads();
+    }
     const uint32_t chunk = (m + blockDim.x - 1) / blockDim.x;
     const uint32_t t0 = threadIdx.x * chunk, t1 = min(m, t0 + chunk);
     uint32_t c[5] = {0, 0, 0, 0, 0};
@@ -598,55 +1264,76 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
         c[nh - 3]++;
     }
     uint32_t ex[5];
-    for (int b = 0; b < 5; ++b) {
+    for (int bk = 0; bk < 5; ++bk) {
         uint32_t total;
-        ex[b] = block_excl_scan(c[b], lds[b], total);
-        if (threadIdx.x == 0) tot[b] = total;
+        ex[bk] = block_excl_scan(c[bk], lds[bk], total);
+        if (threadIdx.x == 0) tot[bk] = total;
     }
     __syncthreads();
     // off[] after finalize!: exclusive start of bucket nh at tm_off[nh-1]
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (int b = 0; b < 9; ++b) {
-            e.s.tm_off[b] = run;
-            if (b >= 2 && b < 7) run += tot[b - 2];
+        for (int bk = 0; bk < 9; ++bk) {
+            e.s.tm_off[bk] = run;
+            if (bk >= 2 && bk < 7) run += tot[bk - 2];
         }
-        for (int b = 3; b < 8; ++b) e.cnt->n_by_len[b] = tot[b - 3];
-        for (int b = 3; b <= 5; ++b)
-            if (tot[b - 3] > PT_MAX_FITS) e.cnt->overflow |= PT_OVF_FITS;
+        for (int bk = 3; bk < 8; ++bk) e.cnt->n_by_len[bk] = tot[bk - 3];
+        for (int bk = 3; bk <= 5; ++bk)
+            if (tot[bk - 3] > PT_MAX_FITS) atomicOr(&e.cnt->overflow, PT_OVF_FITS);
     }
     __syncthreads();
     for (uint32_t t = t0; t < t1; ++t) {
         const uint32_t nh = e.hit_off[t + 1] - e.hit_off[t];
         if (nh < 3 || nh >= 8 || e.quality[t] == PT_QUALITY_DUP) continue;
-        const int b = nh - 3;
-        const uint32_t rank = ex[b]++;
-        e.s.tm_bins[e.s.tm_off[nh - 1] + (tot[b] - 1u - rank)] = (uint16_t)(t + 1u);
+        const int bk = nh - 3;
+        const uint32_t rank = ex[bk]++;
+        e.s.tm_bins[e.s.tm_off[nh - 1] + (tot[bk] - 1u - rank)] = (uint16_t)(t + 1u);
     }
     if (ref) atomicOr(&e.cnt->ref_assert, ref);
 }
 
-// ------------------------------------------------------------------ hit -> module indices
-// kernel_fill_hit_indices (ca_hit_ntuplet_generator_kernels_impl.jl:25-38)
-__global__ void __launch_bounds__(256) k_fill_det(const EvDesc *__restrict__ evs) {
-    PT_EVENT_LOOP(e, i, kMaxTuples + 1u + PT_MAX_TUPLE_HITS) {
-        if (i <= kMaxTuples) {
-            e.det_off[i] = e.hit_off[i];
-        } else {
-            const uint32_t k = i - (kMaxTuples + 1u);
-            const uint32_t sz = e.hit_off[kMaxTuples] - 1u;
-            if (k < sz) e.det_bins[k] = (uint16_t)e.det[e.hit_bins[k] - 1u];
-        }
-    }
-}
-
 // ------------------------------------------------------------------ launchers
-static inline dim3 grid_for(uint32_t items, uint32_t n_events, uint32_t bs = 256) {
+// Blocks per event for a grid-stride kernel over `items` items per event:
+// enough to cover the items, at most what keeps about 2048 blocks in flight.
+static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t bs = 256) {
     uint32_t per = (items + bs - 1) / bs;
     const uint32_t target = (2048u + n_events - 1) / n_events;
     if (per > target) per = target;
     if (per < 1) per = 1;
-    return dim3(per, n_events, 1);
+    return EvGrid{n_events, per};
+}
+
+// Blocks per event for a kernel whose wavefronts take work by ticket: exactly
+// the blocks that are resident at once on the events' XCDs (occupancy from
+// the runtime), so every wavefront starts at once and none waits for a slot.
+static std::mutex g_occ_mu;
+static std::map<const void *, int> g_occ;
+template <class K>
+static EvGrid ev_grid_resident(K kernel, uint32_t n_events, int bs = 256) {
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_occ_mu);
+        auto it = g_occ.find((const void *)kernel);
+        if (it == g_occ.end()) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            g_occ[(const void *)kernel] = per_cu;
+        } else {
+            per_cu = it->second;
+        }
+    }
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+                  ? prop.multiProcessorCount
+                  : 256;
+    }
+    const uint32_t ev_per_xcd = (n_events + 7u) / 8u;
+    uint32_t per = (uint32_t)(cus / 8) * (uint32_t)per_cu / ev_per_xcd;
+    if (per < 1) per = 1;
+    return EvGrid{n_events, per};
 }
 
 // Launch sequence of the CA stage.  When `tm` is given, an event is recorded
@@ -657,50 +1344,44 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     auto mark = [&](const char *name) {
         if (tm) tm->record(name, st);
     };
+    const EvGrid one{n_events, 1u};
     mark("start");
-    k_reset<<<grid_for(max_hits + 33u + (zero_state ? 20u * kMaxTuples : 0u), n_events), 256, 0, st>>>(d_evs,
-                                                                                                          zero_state);
+    const EvGrid g_reset = ev_grid(max_hits + max_slots / kAggTile + kDfsAgg + kMaxCells / 64u + 64u +
+                                       (zero_state ? 20u * kMaxTuples : 0u),
+                                   n_events);
+    k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
     mark("k_reset");
-    k_doublet_count<<<grid_for(max_slots, n_events), 256, 0, st>>>(d_evs, cfg);
+    const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events);
+    k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg);
     mark("k_doublet_count");
-    k_scan<kScanSlots><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, cfg.p.max_num_of_doublets);
-    mark("k_scan_slots");
-    k_doublet_fill<<<grid_for(max_slots, n_events), 256, 0, st>>>(d_evs, cfg);
+    const EvGrid g_fill = ev_grid_resident(k_doublet_fill, n_events);
+    k_doublet_fill<<<ev_grid_dim(g_fill), 256, 0, st>>>(d_evs, g_fill, cfg);
     mark("k_doublet_fill");
-    k_scan<kScanOhc><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, 0);
-    mark("k_scan_ohc");
-    k_ohc_fill<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs);
-    mark("k_ohc_fill");
-    k_ohc_sort<<<grid_for(max_hits, n_events), 256, 0, st>>>(d_evs);
-    mark("k_ohc_sort");
-    k_connect_count<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs, cfg);
-    mark("k_connect_count");
-    k_scan<kScanNn><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, 0);
-    mark("k_scan_nn");
-    k_connect_fill<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs, cfg);
-    mark("k_connect_fill");
-    if (cfg.p.early_fish_bone) k_fishbone<<<grid_for(max_hits, n_events), 256, 0, st>>>(d_evs);
+    k_ohc_rebuild<<<ev_grid_dim(one), 64, 0, st>>>(d_evs, one);
+    mark("k_ohc_rebuild");
+    const EvGrid g_conn = ev_grid_resident(k_connect, n_events);
+    k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
+    mark("k_connect");
+    k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
+    mark("k_connect_ext");
+    if (cfg.p.early_fish_bone) {
+        const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events);
+        k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish);
+    }
     mark("k_fishbone");
-    k_dfs_count<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs, cfg);
+    const EvGrid g_start = ev_grid(est_cells / 8u, n_events);
+    k_dfs_count<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_count");
-    k_scan<kScanTcnt><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, 0);
-    mark("k_scan_tcnt");
-    k_scan<kScanHcnt><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, 0);
-    mark("k_scan_hcnt");
-    k_dfs_emit<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs, cfg);
+    const EvGrid g_cells = ev_grid(est_cells, n_events);
+    k_dfs_scan<<<ev_grid_dim(g_cells), 256, 0, st>>>(d_evs, g_cells);
+    mark("k_dfs_scan");
+    k_dfs_emit<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_emit");
-    k_finalize_tuples<<<grid_for(kMaxTuples + 1u, n_events), 256, 0, st>>>(d_evs);
-    mark("k_finalize_tuples");
-    k_scan<kScanTrk><<<dim3(1, n_events), kScanBlock, 0, st>>>(d_evs, 0);
-    mark("k_scan_trk");
-    k_trk_fill<<<grid_for(kMaxTuples, n_events), 256, 0, st>>>(d_evs);
-    mark("k_trk_fill");
-    k_trk_sort_dup<<<grid_for(est_cells, n_events), 256, 0, st>>>(d_evs);
-    mark("k_trk_sort_dup");
-    k_multiplicity<<<dim3(1, n_events), 1024, 0, st>>>(d_evs);
+    const EvGrid g_post = ev_grid(kMaxTuples + 1u + PT_MAX_TUPLE_HITS / 4u, n_events);
+    k_tuples_post<<<ev_grid_dim(g_post), 256, 0, st>>>(d_evs, g_post);
+    mark("k_tuples_post");
+    k_multiplicity<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
     mark("k_multiplicity");
-    k_fill_det<<<grid_for(kMaxTuples + 1u + PT_MAX_TUPLE_HITS, n_events), 256, 0, st>>>(d_evs);
-    mark("k_fill_det");
 }
 
 }  // namespace pt

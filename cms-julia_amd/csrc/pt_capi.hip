@@ -64,28 +64,38 @@ size_t carve(char *base, pt::Scratch *s) {
     };
     Scratch t;
     t.slot_cnt = (uint32_t *)take((kMaxSlots + 1) * 4);
-    t.cell_inner = (uint16_t *)take(kMaxCells * 2);
-    t.cell_outer = (uint16_t *)take(kMaxCells * 2);
+    t.slot_mask = (uint64_t *)take((size_t)kMaxSlots * 8);
+    t.slot_pos = (uint4 *)take((size_t)kMaxSlots * 16);
+    t.dbl_agg = (uint32_t *)take(kDblAgg * 4);
+    t.cell_io = (uint32_t *)take(kMaxCells * 4);
+    t.cell_flags = (uint8_t *)take(kMaxCells);
     t.cell_pair = (uint8_t *)take(kMaxCells);
-    t.cell_used = (uint8_t *)take(kMaxCells);
-    t.cell_killed = (uint8_t *)take(kMaxCells);
-    t.ohc_cnt = (uint32_t *)take((kMaxHits + 1) * 4);
+    t.cell_nn = (uint8_t *)take(kMaxCells);
+    t.cell_oxyzr = (float4 *)take((size_t)kMaxCells * 16);
+    t.nbr4 = (uint4 *)take((size_t)kMaxCells * 16);
+    t.nbr_ext = (uint32_t *)take(kMaxCells * 4);
+    t.nbr_pool = (uint32_t *)take((size_t)kNbrPool * 4);
+    t.ext_list = (uint32_t *)take(kMaxCells * 4);
+    t.trk = (uint2 *)take((size_t)kMaxCells * 8);
+    t.start_list = (uint32_t *)take(kMaxCells * 4);
+    t.start_bits = (uint64_t *)take((size_t)(kMaxCells / 64 + 1) * 8);
+    t.tup = (uint64_t *)take((size_t)kMaxCells * 8);
+    t.dfs_agg = (uint64_t *)take((size_t)kDfsAgg * 8);
     t.ohc_cursor = (uint32_t *)take(kMaxHits * 4);
-    t.ohc_list = (uint32_t *)take(kMaxCells * 4);
-    t.nn = (uint32_t *)take((kMaxCells + 1) * 4);
-    t.nbr = (uint32_t *)take((size_t)kNbrCap * 4);
-    t.tcnt = (uint32_t *)take((kMaxCells + 1) * 4);
-    t.hcnt = (uint32_t *)take((kMaxCells + 1) * 4);
+    t.ohc_list = (uint2 *)take((size_t)kMaxHits * kOhcStride * 8);
+    t.ohc_ovf = (uint32_t *)take(kOhcOvfCap * 4);
+    t.trk_ovf = (uint32_t *)take(kTrkOvfCap * 4);
     t.tuple_cells = (uint32_t *)take(kMaxTuples * 4 * 4);
-    t.trk_cnt = (uint32_t *)take((kMaxCells + 1) * 4);
-    t.trk_cursor = (uint32_t *)take(kMaxCells * 4);
-    t.trk_list = (uint16_t *)take(kTrkCap * 2);
     t.tm_off = (uint32_t *)take(16 * 4);
     t.tm_bins = (uint16_t *)take(kMaxTuples * 2);
     t.h2t_off = (uint32_t *)take((kMaxHits + 2) * 4);
     t.h2t_bins = (uint16_t *)take(4u * 49152u * 2);
     t.h2t_cursor = (uint32_t *)take(kMaxHits * 4);
     t.misc = (uint32_t *)take(16 * 4);
+    t.hxyzr = (float4 *)take(kMaxHits * 16);
+    t.hdbl = (uint4 *)take(kMaxHits * 16);
+    t.binrec = (uint4 *)take(kMaxHits * 16);
+    t.binxy = (float2 *)take(kMaxHits * 8);
     if (s) *s = t;
     return off;
 }
@@ -122,6 +132,13 @@ struct pt_ctx {
     Staging stg;
     pt::Marks marks;
     bool timing = false;
+    // sub-batches on separate streams (default priority) overlap the kernels of
+    // different event groups; joined back into `stream` before results are read
+    static constexpr int kMaxSub = 8;
+    int n_sub = 1;
+    hipStream_t sub[kMaxSub] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kMaxSub] = {};
     bool have_marks = false;
     uint32_t last_n = 0;
 };
@@ -212,6 +229,12 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
     for (int k = 0; k < pt::Marks::kMax; ++k)
         if (hipEventCreate(&c->marks.ev[k]) != hipSuccess) return cleanup(fail(PT_ERR_HIP, "hipEventCreate failed"));
     c->have_marks = true;
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(PT_ERR_HIP, "hipEventCreate failed"));
+    for (int k = 0; k < pt_ctx::kMaxSub; ++k)
+        if (hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(PT_ERR_HIP, "sub-stream creation failed"));
     *out = c;
     return PT_OK;
 }
@@ -220,6 +243,12 @@ int pt_ctx_destroy(pt_ctx *c) {
     if (!c) return PT_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < pt_ctx::kMaxSub; ++k) {
+        if (c->sub[k]) (void)hipStreamSynchronize(c->sub[k]);
+        if (c->sub[k]) (void)hipStreamDestroy(c->sub[k]);
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->have_marks)
         for (int k = 0; k < pt::Marks::kMax; ++k) (void)hipEventDestroy(c->marks.ev[k]);
     (void)hipFree(c->arena);
@@ -287,6 +316,17 @@ static pt::Config make_config(const pt_ctx *c, double b_field) {
     cfg.p = c->p;
     cfg.B = (double)(float)b_field;  // HelixFitOnGPU(Float32(b_field)) then Float64
     cfg.frames = c->d_frames;
+    std::memset(cfg.pairs.by_inner, 0, sizeof(cfg.pairs.by_inner));
+    std::memset(cfg.pairs.pair_of, -1, sizeof(cfg.pairs.pair_of));
+    for (int p = 0; p < 19; ++p) cfg.pairs.pair_of[kLP[2 * p]][kLP[2 * p + 1]] = (int8_t)p;
+    std::memset(cfg.pairs.n_by_inner, 0, sizeof(cfg.pairs.n_by_inner));
+    uint32_t n_pairs = 19;
+    if (!c->p.include_jumping_forward_doublets) n_pairs = 15;
+    if (c->p.min_hits_per_ntuplet > 3) n_pairs = 13;
+    for (uint32_t p = 0; p < n_pairs; ++p) {
+        const int L = kLP[2 * p];
+        cfg.pairs.by_inner[L][cfg.pairs.n_by_inner[L]++] = (int8_t)p;
+    }
     for (int p = 0; p < 19; ++p) {
         cfg.pairs.inner[p] = (int8_t)kLP[2 * p];
         cfg.pairs.outer[p] = (int8_t)kLP[2 * p + 1];
@@ -303,20 +343,38 @@ static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_t
     if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
     if (n == 0) return PT_OK;
     if (n > c->max_events) return fail(PT_ERR_ARG, "%u events exceed the context's %u slots", n, c->max_events);
-    uint32_t max_hits = 0, max_slots = 0;
+    std::vector<uint32_t> nslots(n);
     for (uint32_t i = 0; i < n; ++i) {
-        uint32_t ns, np;
-        int rc = build_desc(c, i, hits[i], outs[i], ns, np);
+        uint32_t np;
+        int rc = build_desc(c, i, hits[i], outs[i], nslots[i], np);
         if (rc) return rc;
-        max_hits = std::max(max_hits, hits[i].n_hits);
-        max_slots = std::max(max_slots, ns);
     }
     HIP_OK(hipMemcpyAsync(c->d_evs, c->h_evs, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, c->stream));
     const pt::Config cfg = make_config(c, b_field);
     pt::Marks *marks = c->timing ? &c->marks : nullptr;
     if (marks) marks->n = 0;
-    pt::launch_ca(c->d_evs, n, cfg, max_hits, max_slots, out_zeroed ? 0 : 1, c->stream, marks);
-    pt::launch_fit_tuples(c->d_evs, n, cfg, c->stream, marks);
+    const uint32_t n_sub = marks ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)c->n_sub, n));
+    auto launch_range = [&](uint32_t e0, uint32_t e1, hipStream_t st) {
+        uint32_t max_hits = 0, max_slots = 0;
+        for (uint32_t i = e0; i < e1; ++i) {
+            max_hits = std::max(max_hits, hits[i].n_hits);
+            max_slots = std::max(max_slots, nslots[i]);
+        }
+        pt::launch_ca(c->d_evs + e0, e1 - e0, cfg, max_hits, max_slots, out_zeroed ? 0 : 1, st, marks);
+        pt::launch_fit_tuples(c->d_evs + e0, e1 - e0, cfg, st, marks);
+    };
+    if (n_sub == 1) {
+        launch_range(0, n, c->stream);
+    } else {
+        HIP_OK(hipEventRecord(c->ev_fork, c->stream));
+        for (uint32_t k = 0; k < n_sub; ++k) {
+            const uint32_t e0 = (uint32_t)((uint64_t)n * k / n_sub), e1 = (uint32_t)((uint64_t)n * (k + 1) / n_sub);
+            HIP_OK(hipStreamWaitEvent(c->sub[k], c->ev_fork, 0));
+            launch_range(e0, e1, c->sub[k]);
+            HIP_OK(hipEventRecord(c->ev_join[k], c->sub[k]));
+            HIP_OK(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
+        }
+    }
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(pt_counters) * n, hipMemcpyDeviceToHost, c->stream));
     c->last_n = n;
@@ -330,10 +388,20 @@ int pt_make_tuples_device_async(pt_ctx *c, uint32_t n_events, const pt_hits_view
     return run_batch(c, n_events, hits, out, b_field, out_zeroed);
 }
 
+// After the batch has completed: an event whose kernels hit one of this
+// library's own limits makes the call fail (its results are not valid).
+static int check_internal(const pt_ctx *c) {
+    for (uint32_t i = 0; i < c->last_n; ++i)
+        if (c->h_cnt[i].internal_error)
+            return fail(PT_ERR_CAPACITY, "event %u: internal limit hit (PT_INT_* bits 0x%x)", i,
+                        c->h_cnt[i].internal_error);
+    return PT_OK;
+}
+
 int pt_ctx_sync(pt_ctx *c) {
     if (!c) return fail(PT_ERR_ARG, "null context");
     HIP_OK(hipStreamSynchronize(c->stream));
-    return PT_OK;
+    return check_internal(c);
 }
 
 int pt_get_counters(pt_ctx *c, uint32_t event, pt_counters *cnt) {
@@ -412,6 +480,8 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     rc = run_batch(c, 1, &dv, &dout, b_field, 0);
     if (rc) return rc;
     HIP_OK(hipStreamSynchronize(st));
+    rc = check_internal(c);
+    if (rc) return rc;
     const pt_counters &k = c->h_cnt[0];
     const size_t T = PT_MAX_TUPLES;
     const size_t m = std::min<size_t>(k.n_tuples, T);
@@ -445,39 +515,89 @@ int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
     const uint32_t nc = k.n_cells, nh = e.n_hits;
     d->n_cells = nc;
     if (nc > d->cap_cells) return fail(PT_ERR_CAPACITY, "debug capacity %u < %u cells", d->cap_cells, nc);
-    std::vector<uint16_t> u16(nc);
-    std::vector<uint8_t> u8(nc);
-    if (d->cell_inner) HIP_OK(hipMemcpy(d->cell_inner, s.cell_inner, nc * 2, hipMemcpyDeviceToHost));
-    if (d->cell_outer) HIP_OK(hipMemcpy(d->cell_outer, s.cell_outer, nc * 2, hipMemcpyDeviceToHost));
-    if (d->cell_pair) HIP_OK(hipMemcpy(d->cell_pair, s.cell_pair, nc, hipMemcpyDeviceToHost));
-    if (d->cell_used) HIP_OK(hipMemcpy(d->cell_used, s.cell_used, nc, hipMemcpyDeviceToHost));
-    if (d->cell_doublet_id) {
-        HIP_OK(hipMemcpy(u8.data(), s.cell_killed, nc, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < nc; ++i) d->cell_doublet_id[i] = u8[i] ? -1 : (int32_t)(i + 1);
+    std::vector<uint32_t> io(nc);
+    std::vector<uint8_t> fl(nc);
+    if (nc) {
+        HIP_OK(hipMemcpy(io.data(), s.cell_io, (size_t)nc * 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(fl.data(), s.cell_flags, nc, hipMemcpyDeviceToHost));
     }
-    auto csr = [&](const uint32_t *d_off, uint32_t n, uint32_t *h_off, auto *h_vals, const auto *d_vals,
-                   uint32_t cap, uint32_t limit, bool to1) -> int {
-        std::vector<uint32_t> off(n + 1);
-        HIP_OK(hipMemcpy(off.data(), d_off, (n + 1) * 4, hipMemcpyDeviceToHost));
-        using V = typename std::remove_pointer<decltype(h_vals)>::type;
-        const uint32_t total = off[n];
-        std::vector<V> vals(total);
-        if (total) HIP_OK(hipMemcpy(vals.data(), d_vals, (size_t)total * sizeof(V), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> pr(nc);
+    if (nc) HIP_OK(hipMemcpy(pr.data(), s.cell_pair, nc, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < nc; ++i) {
+        const uint32_t in = io[i] & 0xFFFFu, on = io[i] >> 16;
+        if (d->cell_inner) d->cell_inner[i] = (uint16_t)in;
+        if (d->cell_outer) d->cell_outer[i] = (uint16_t)on;
+        if (d->cell_pair) d->cell_pair[i] = pr[i];
+        if (d->cell_used) d->cell_used[i] = fl[i] & pt::kUsed;
+        if (d->cell_doublet_id) d->cell_doublet_id[i] = (fl[i] & pt::kKilled) ? -1 : (int32_t)(i + 1);
+    }
+    // outer-hit lists: ascending cell ids, first 128 (as OuterHitOfCellVector keeps them)
+    if (d->ohc_off && d->ohc) {
+        std::vector<uint32_t> cur(nh);
+        if (nh) HIP_OK(hipMemcpy(cur.data(), s.ohc_cursor, (size_t)nh * 4, hipMemcpyDeviceToHost));
+        std::vector<uint2> lst((size_t)nh * pt::kOhcStride);
+        if (nh) HIP_OK(hipMemcpy(lst.data(), s.ohc_list, lst.size() * sizeof(uint2), hipMemcpyDeviceToHost));
         uint32_t pos = 0;
-        h_off[0] = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            uint32_t a = off[i], b = off[i + 1];
-            if (limit && b - a > limit) b = a + limit;
-            for (uint32_t q = a; q < b && pos < cap; ++q) h_vals[pos++] = (V)(vals[q] + (to1 ? 1 : 0));
-            h_off[i + 1] = pos;
+        d->ohc_off[0] = 0;
+        for (uint32_t h = 0; h < nh; ++h) {
+            const uint32_t m = std::min(cur[h], pt::kOhcStride);
+            std::vector<uint32_t> v(m);
+            for (uint32_t q = 0; q < m; ++q) v[q] = lst[(size_t)h * pt::kOhcStride + q].x;
+            std::sort(v.begin(), v.end());
+            for (uint32_t q = 0; q < v.size() && pos < d->cap_ohc; ++q) d->ohc[pos++] = v[q] + 1;
+            d->ohc_off[h + 1] = pos;
         }
-        return PT_OK;
-    };
-    int rc = PT_OK;
-    if (d->ohc_off && d->ohc) rc = csr(s.ohc_cnt, nh, d->ohc_off, d->ohc, s.ohc_list, d->cap_ohc, PT_MAX_CELLS_PER_HIT, true);
-    if (!rc && d->nbr_off && d->nbr) rc = csr(s.nn, nc, d->nbr_off, d->nbr, s.nbr, d->cap_nbr, 0, true);
-    if (!rc && d->trk_off && d->trk) rc = csr(s.trk_cnt, nc, d->trk_off, d->trk, s.trk_list, d->cap_trk, PT_MAX_CELL_TRACKS, false);
-    return rc;
+    }
+    // neighbour lists: 4 inline entries, the rest in the pool
+    if (d->nbr_off && d->nbr) {
+        std::vector<uint8_t> nn(nc);
+        std::vector<uint4> v4(nc);
+        std::vector<uint32_t> ext(nc), pool(pt::kNbrPool);
+        if (nc) {
+            HIP_OK(hipMemcpy(nn.data(), s.cell_nn, nc, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(v4.data(), s.nbr4, (size_t)nc * 16, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(ext.data(), s.nbr_ext, (size_t)nc * 4, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(pool.data(), s.nbr_pool, pool.size() * 4, hipMemcpyDeviceToHost));
+        }
+        uint32_t pos = 0;
+        d->nbr_off[0] = 0;
+        for (uint32_t i = 0; i < nc; ++i) {
+            const uint32_t in4[4] = {v4[i].x, v4[i].y, v4[i].z, v4[i].w};
+            for (uint32_t q = 0; q < nn[i] && pos < d->cap_nbr; ++q) {
+                const uint32_t id = (q < 4) ? in4[q] : pool[std::min<size_t>(ext[i] + q - 4, pool.size() - 1)];
+                d->nbr[pos++] = id + 1;
+            }
+            d->nbr_off[i + 1] = pos;
+        }
+    }
+    // cell -> tuple lists (CellTracksVector): tuple ids through the cell in id
+    // order, first 64, rebuilt from the cells of each stored tuple
+    if (d->trk_off && d->trk) {
+        const uint32_t mc = std::min<uint32_t>(k.n_tuples, PT_MAX_TUPLES);
+        std::vector<uint32_t> tc((size_t)mc * 4);
+        if (mc) HIP_OK(hipMemcpy(tc.data(), s.tuple_cells, tc.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<std::vector<uint16_t>> lists(nc);
+        for (uint32_t t = 0; t < mc; ++t)
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = tc[(size_t)t * 4 + q];
+                if (c == 0xFFFFFFFFu) break;
+                if (c < nc && lists[c].size() < PT_MAX_CELL_TRACKS) lists[c].push_back((uint16_t)(t + 1));
+            }
+        uint32_t pos = 0;
+        d->trk_off[0] = 0;
+        for (uint32_t i = 0; i < nc; ++i) {
+            for (uint32_t q = 0; q < lists[i].size() && pos < d->cap_trk; ++q) d->trk[pos++] = lists[i][q];
+            d->trk_off[i + 1] = pos;
+        }
+    }
+    return PT_OK;
+}
+
+int pt_ctx_set_streams(pt_ctx *c, int n_streams) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (n_streams < 1 || n_streams > pt_ctx::kMaxSub) return fail(PT_ERR_ARG, "n_streams must be 1..%d", pt_ctx::kMaxSub);
+    c->n_sub = n_streams;
+    return PT_OK;
 }
 
 int pt_ctx_enable_timing(pt_ctx *c, int enable) {

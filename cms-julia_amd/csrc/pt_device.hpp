@@ -16,47 +16,78 @@ constexpr uint32_t kMaxHits = PT_MAX_HITS;
 constexpr uint32_t kMaxCells = PT_MAX_DOUBLETS;
 constexpr uint32_t kMaxSlots = 6u * PT_MAX_HITS;   // layer 0 is the inner layer of 6 pairs
 constexpr uint32_t kMaxTuples = PT_MAX_TUPLES;
-constexpr uint32_t kNbrCap = 2u * 1024u * 1024u;    // neighbour entries per event
-constexpr uint32_t kTrkCap = 4u * PT_MAX_TUPLES;    // cell->tuple entries (<= 4 cells per tuple)
-constexpr uint32_t kScanBlock = 1024;
-constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kNbrPool = 1024u * 1024u;   // neighbour entries beyond the 4 kept inline, per event
+constexpr uint32_t kOhcStride = PT_MAX_CELLS_PER_HIT;  // outer-hit list capacity per hit
+constexpr uint32_t kOhcOvfCap = kMaxCells / (PT_MAX_CELLS_PER_HIT + 1) + 1;  // hits that can exceed it
+constexpr uint32_t kTrkOvfCap = 4u * PT_MAX_TUPLES / (PT_MAX_CELL_TRACKS + 1) + 8u;
+constexpr uint32_t kAggTile = 256;                // items per prefix-sum tile (one block)
+constexpr uint32_t kDblAgg = (kMaxSlots + kAggTile - 1) / kAggTile;
+constexpr uint32_t kDfsAgg = (kMaxCells + kAggTile - 1) / kAggTile;
 
 // Layer pairs (gpu_pixel_doublets.jl:20-28) and per-pair cuts (:29-56)
 struct PairTables {
     int8_t inner[19], outer[19];
+    int8_t by_inner[10][6];   // pairs whose inner layer is L, in pair order
+    int8_t n_by_inner[10];
+    int8_t pair_of[10][10];   // layer pair of (inner layer, outer layer), -1 if none
     int16_t phi_cut[19];
     float min_z[19], max_z[19], max_r[19];  // integral values: exact in float
 };
 
 // Scratch of one event slot (device pointers into the context arena).
 struct Scratch {
-    uint32_t *slot_cnt;     // [kMaxSlots+1]  doublets per inner slot -> exclusive offsets
-    uint16_t *cell_inner;   // [kMaxCells]    1-based hit ids
-    uint16_t *cell_outer;   // [kMaxCells]
-    uint8_t *cell_pair;     // [kMaxCells]    0-based layer pair
-    uint8_t *cell_used;     // [kMaxCells]
-    uint8_t *cell_killed;   // [kMaxCells]
-    uint32_t *ohc_cnt;      // [kMaxHits+1]   cells per outer hit -> exclusive offsets
-    uint32_t *ohc_cursor;   // [kMaxHits]
-    uint32_t *ohc_list;     // [kMaxCells]    0-based cell ids, ascending per hit
-    uint32_t *nn;           // [kMaxCells+1]  stored neighbours per cell -> offsets
-    uint32_t *nbr;          // [kNbrCap]      0-based cell ids, ascending
-    uint32_t *tcnt;         // [kMaxCells+1]  tuples per start cell -> offsets
-    uint32_t *hcnt;         // [kMaxCells+1]  tuple hits per start cell -> offsets
+    uint32_t *slot_cnt;     // [kMaxSlots+1]  passing candidates per inner slot, then the
+                            //                slot's first cell id (total at [n_slots])
+    uint64_t *slot_mask;    // [kMaxSlots]    passing candidates among the slot's first 64
+    uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
+    uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kAggTile slots
+    uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
+    uint8_t *cell_flags;    // [kMaxCells]    kUsed (connect), kKilled (fishbone)
+    uint8_t *cell_pair;     // [kMaxCells]    layer pair
+    uint8_t *cell_nn;       // [kMaxCells]    stored outer neighbours (<= 64)
+    float4 *cell_oxyzr;     // [kMaxCells]    (x, y, z, r) of the outer hit
+    uint4 *nbr4;            // [kMaxCells]    first 4 neighbours (0-based cell ids, ascending)
+    uint32_t *nbr_ext;      // [kMaxCells]    pool offset of neighbours 5..64
+    uint32_t *nbr_pool;     // [kNbrPool]
+    uint32_t *ext_list;     // [kMaxCells]    cells with more than 4 neighbours
+    uint2 *trk;             // [kMaxCells]    (tuples through the cell, largest hit count among them)
+    uint32_t *start_list;   // [kMaxCells]    ntuplet start candidates (any order)
+    uint64_t *start_bits;   // [kMaxCells/64] the same as a bit set
+    uint64_t *tup;          // [kMaxCells]    start cell: (tuples << 32 | hits), then the prefix
+    uint64_t *dfs_agg;      // [kDfsAgg]      the same summed per tile of kAggTile cells
+    uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
+    uint2 *ohc_list;        // [kMaxHits*kOhcStride] (cell, inner hit) 0-based, unordered unless
+                            //                the hit overflowed (then the first 128 cells, ascending)
+    uint32_t *ohc_ovf;      // [kOhcOvfCap]   hits with more than kOhcStride cells
+    uint32_t *trk_ovf;      // [kTrkOvfCap]   cells whose tuple list needs the exact path
     uint32_t *tuple_cells;  // [kMaxTuples*4] cells of each stored tuple
-    uint32_t *trk_cnt;      // [kMaxCells+1]  tuples per cell -> offsets
-    uint32_t *trk_cursor;   // [kMaxCells]
-    uint16_t *trk_list;     // [kTrkCap]      1-based tuple ids, ascending per cell
     uint32_t *tm_off;       // [16]           tuple multiplicity (9 used)
     uint16_t *tm_bins;      // [kMaxTuples]
     uint32_t *h2t_off;      // [kMaxHits+2]   hit -> loose tuples (classification)
     uint16_t *h2t_bins;     // [4*PT_MAX_TUPLES... 196608]
     uint32_t *h2t_cursor;   // [kMaxHits]
     uint32_t *misc;         // [16]           internal scalars (see Misc)
+    float4 *hxyzr;          // [kMaxHits]     packed (x, y, z, r) per hit
+    uint4 *hdbl;            // [kMaxHits]     packed (z, r, iphi | det << 16, ysize) per hit
+    uint4 *binrec;          // [kMaxHits]     per histogram position: (z, r, iphi | det << 16,
+                            //                ysize | hit << 16) of the hit stored there
+    float2 *binxy;          // [kMaxHits]     per histogram position: (x, y)
 };
 
+constexpr uint8_t kUsed = 1, kKilled = 2;
+
 enum Misc : int {
-    kMiscTupleOverflowHits = 0,
+    kMiscStart = 0,       // entries of start_list
+    kMiscExt = 1,         // entries of ext_list
+    kMiscOhcOvf = 3,      // entries of ohc_ovf
+    kMiscTrkOvf = 4,      // entries of trk_ovf
+    kMiscPool = 5,        // neighbour pool entries used
+    kMiscNbrLists = 6,    // cells with at least one neighbour
+    kMiscTrkLists = 7,    // cells with at least one tuple
+    kMiscTkCount = 8,     // next tile (ticket) of the doublet count
+    kMiscTkFill = 9,      // ... of the doublet fill
+    kMiscTkConn = 10,     // ... of connect
+    kMiscTkFish = 11,     // ... of the fishbone
 };
 
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,
@@ -145,9 +176,33 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
     return r;
 }
 
-// Grid-stride helpers for the batched launch shape (x: items, y: event).
-#define PT_EVENT_LOOP(e_, i_, n_)                                                               \
-    const EvDesc &e_ = evs[blockIdx.y];                                                         \
-    for (uint32_t i_ = blockIdx.x * blockDim.x + threadIdx.x; i_ < (n_); i_ += gridDim.x * blockDim.x)
+// Batched launch shape.  A kernel over a batch runs a 1-D grid of
+// 8 * per_event * ceil(n_events / 8) blocks.  Workgroups are dispatched
+// round-robin over the 8 XCDs in linear order, so block L runs on XCD L % 8;
+// event e gets the blocks L = e % 8 + 8 * (per_event * (e / 8) + b),
+// b < per_event, all on one XCD, and its working set (hit records, cells,
+// lists) stays in that XCD's L2 instead of being pulled into all eight.
+struct EvGrid {
+    uint32_t n_events, per_event;
+};
+
+__device__ __forceinline__ bool ev_block(const EvGrid &g, uint32_t &ev, uint32_t &b) {
+    const uint32_t L = blockIdx.x;
+    const uint32_t s = L >> 3;
+    const uint32_t q = s / g.per_event;
+    ev = (L & 7u) + 8u * q;
+    b = s - q * g.per_event;
+    return ev < g.n_events;
+}
+
+inline dim3 ev_grid_dim(const EvGrid &g) { return dim3(8u * g.per_event * ((g.n_events + 7u) / 8u)); }
+
+// Grid-stride loop over the items of this block's event; blocks beyond the
+// last event return.
+#define PT_EVENT_LOOP(e_, i_, n_)                                                                   \
+    uint32_t pt_ev_, pt_b_;                                                                         \
+    if (!ev_block(grid, pt_ev_, pt_b_)) return;                                                     \
+    const EvDesc &e_ = evs[pt_ev_];                                                                 \
+    for (uint32_t i_ = pt_b_ * blockDim.x + threadIdx.x; i_ < (n_); i_ += grid.per_event * blockDim.x)
 
 }  // namespace pt

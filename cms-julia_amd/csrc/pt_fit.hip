@@ -34,13 +34,16 @@ __device__ __forceinline__ uint32_t fit_flag_bits(uint32_t f) {
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ evs, Config cfg, int bucket) {
-    const EvDesc &e = evs[blockIdx.y];
+__global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg,
+                                                    int bucket) {
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
     const uint32_t b0 = e.s.tm_off[bucket - 1], b1 = e.s.tm_off[bucket];
     uint32_t n = b1 - b0;
     if (n > PT_MAX_FITS) n = PT_MAX_FITS;
     uint32_t ref = 0, nfit = 0;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+    for (uint32_t idx = b * blockDim.x + threadIdx.x; idx < n; idx += grid.per_event * blockDim.x) {
         const uint32_t tkid = e.s.tm_bins[b0 + idx];  // 1-based
         const uint32_t start = e.hit_off[tkid - 1] - 1u;
         double hx[N], hy[N], hz[N];
@@ -131,12 +134,13 @@ void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg
     const uint32_t target = (1024u + n_events - 1) / n_events;
     if (per > target) per = target;
     if (per < 1) per = 1;
-    const dim3 g(per, n_events);
-    k_fit_tuples<3><<<g, 256, 0, st>>>(d_evs, cfg, 3);
+    const EvGrid grid{n_events, per};
+    const dim3 g = ev_grid_dim(grid);
+    k_fit_tuples<3><<<g, 256, 0, st>>>(d_evs, grid, cfg, 3);
     if (tm) tm->record("k_fit_tuples<3>", st);
-    k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, cfg, 4);
+    k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, grid, cfg, 4);
     if (tm) tm->record("k_fit_tuples<4>", st);
-    if (cfg.p.fit_5_as_4) k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, cfg, 5);
+    if (cfg.p.fit_5_as_4) k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, grid, cfg, 5);
     if (tm) tm->record("k_fit_tuples<4>(5 hits)", st);
 }
 

@@ -10,7 +10,9 @@ from pathlib import Path
 PKG_ROOT = Path(__file__).resolve().parent.parent          # .../cms-julia_amd
 REPO_ROOT = PKG_ROOT.parent
 LIB_DIR = PKG_ROOT / "lib"
-HIP_LIB = LIB_DIR / "libpixeltrack_hip.so"
+# PIXELTRACK_HIP_LIB selects another build of the same C ABI (e.g. an
+# instrumented one); the default is the in-tree library.
+HIP_LIB = Path(os.environ.get("PIXELTRACK_HIP_LIB", LIB_DIR / "libpixeltrack_hip.so"))
 GEN_LIB = LIB_DIR / "libpixeltrack_gen.so"
 
 # capacities (ca_constants.jl:10-29, PixelTrackHeterogeneous.jl:55-60)
@@ -115,7 +117,7 @@ class PtCounters(C.Structure):
         ("n_cells_found", C.c_uint32), ("n_connections", C.c_uint32), ("n_connect_pairs", C.c_uint32),
         ("n_killed", C.c_uint32), ("n_tuples", C.c_uint32), ("n_tuple_hits", C.c_uint32),
         ("n_by_len", C.c_uint32 * 8), ("n_fitted", C.c_uint32), ("n_loose", C.c_uint32),
-        ("overflow", C.c_uint32), ("ref_assert", C.c_uint32),
+        ("overflow", C.c_uint32), ("ref_assert", C.c_uint32), ("internal_error", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -170,7 +172,7 @@ class PtgFitConfig(C.Structure):
 HIP_SYMBOLS = [
     "pt_params_default", "pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_device_count",
     "pt_ctx_set_frames", "pt_make_tuples", "pt_make_tuples_device_async", "pt_ctx_sync", "pt_get_counters",
-    "pt_get_ca_debug", "pt_ctx_enable_timing", "pt_get_stage_times", "pt_fit_bl_device_async",
+    "pt_get_ca_debug", "pt_ctx_set_streams", "pt_ctx_enable_timing", "pt_get_stage_times", "pt_fit_bl_device_async",
     "pt_malloc", "pt_free", "pt_memcpy_h2d", "pt_memcpy_d2h", "pt_memset",
 ]
 GEN_SYMBOLS = [
@@ -217,6 +219,7 @@ def hip_lib():
         lib.pt_get_counters.argtypes = [vp, C.c_uint32, C.POINTER(PtCounters)]
         lib.pt_get_ca_debug.argtypes = [vp, C.c_uint32, C.POINTER(PtCaDebug)]
         lib.pt_ctx_enable_timing.argtypes = [vp, C.c_int]
+        lib.pt_ctx_set_streams.argtypes = [vp, C.c_int]
         lib.pt_get_stage_times.argtypes = [vp, c_f32p, C.POINTER(C.c_char_p), C.POINTER(C.c_int)]
         lib.pt_fit_bl_device_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
         lib.pt_malloc.argtypes = [vp, C.POINTER(vp), C.c_size_t]

@@ -88,6 +88,9 @@ class Context:
         A.check(self.lib.pt_get_counters(self.h, event, C.byref(c)), "pt_get_counters")
         return c.as_dict()
 
+    def set_streams(self, n):
+        A.check(self.lib.pt_ctx_set_streams(self.h, n), "pt_ctx_set_streams")
+
     def enable_timing(self, on=True):
         A.check(self.lib.pt_ctx_enable_timing(self.h, 1 if on else 0), "pt_ctx_enable_timing")
 

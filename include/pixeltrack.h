@@ -74,6 +74,12 @@ extern "C" {
 #define PT_REF_BAD_MODULE      0x100u /* B1 hit with module >= 96 (gpu_pixel_doublets.jl:231) */
 #define PT_REF_MULT            0x200u /* tuple with >= 8 hits (ca_hit_ntuplet_generator_kernels_impl.jl:189) */
 
+/* internal_error bits: a limit of this library, not of the reference, was hit
+ * and the event's results are not valid.  No counterpart in the reference. */
+#define PT_INT_LOOKBACK        0x001u /* a prefix-scan wait exceeded its spin bound */
+#define PT_INT_NBR_POOL        0x002u /* neighbour pool (lists longer than 4) full   */
+#define PT_INT_OVF_LIST        0x004u /* internal overflow list full                 */
+
 /* Quality enum (track_quality.jl:129-136) */
 #define PT_QUALITY_BAD         0
 #define PT_QUALITY_DUP         1
@@ -158,6 +164,7 @@ typedef struct pt_counters {
     uint32_t n_loose;           /* tracks with quality loose after classification  */
     uint32_t overflow;          /* PT_OVF_* bits                                   */
     uint32_t ref_assert;        /* PT_REF_* bits                                   */
+    uint32_t internal_error;    /* PT_INT_* bits: this library failed, results invalid */
 } pt_counters;
 
 /* Internal CA state of one event, for the stage-level parity checks.  Arrays
@@ -215,9 +222,15 @@ int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
 /* Internal CA state of event `event` of the last batch (device -> host copy). */
 int pt_get_ca_debug(pt_ctx *ctx, uint32_t event, pt_ca_debug *dbg);
 
+/* Events of a batch are split into n_streams groups launched on separate
+ * streams so their kernels overlap (default 4; 1..8).  No reference counterpart:
+ * the reference runs events on host threads (EventProcessor.jl:43). */
+int pt_ctx_set_streams(pt_ctx *ctx, int n_streams);
+
 /* Per-kernel timing of the last batch, milliseconds, from HIP events recorded
  * on the context stream after every kernel when enabled (enable=1 before the
- * batch).  *n_stages: capacity of ms/names on input, entries on output. */
+ * batch; a timed batch runs on one stream).  *n_stages: capacity of ms/names
+ * on input, entries on output. */
 #define PT_N_STAGES 48
 int pt_ctx_enable_timing(pt_ctx *ctx, int enable);
 int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages);
