@@ -85,6 +85,37 @@ __device__ __forceinline__ int owner_of(uint32_t incl, uint32_t g) {
     return lo;
 }
 
+// owner_of for N items at once, the searches in lockstep so that their
+// cross-lane reads are in flight together
+template <int N>
+__device__ __forceinline__ void owners_of(uint32_t incl, const uint32_t (&g)[N], int (&o)[N]) {
+    int lo[N], hi[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        lo[u] = 0;
+        hi[u] = 63;
+    }
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+        int mid[N];
+        uint32_t v[N];
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+            mid[u] = (lo[u] + hi[u]) >> 1;
+            v[u] = (uint32_t)__shfl(incl, mid[u], 64);
+        }
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+            if (v[u] > g[u])
+                hi[u] = mid[u];
+            else
+                lo[u] = mid[u] + 1;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) o[u] = lo[u];
+}
+
 // position of the r-th set bit (r < popcount(m))
 __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
     uint32_t pos = 0;
@@ -384,10 +415,13 @@ __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict_
             SlotCtx c[kEvalU];
             uint4 q[kEvalU];
             uint32_t g[kEvalU];
+            int ow[kEvalU];
+#pragma unroll
+            for (int u = 0; u < kEvalU; ++u) g[u] = base + 64u * u + lane;
+            owners_of(incl, g, ow);
 #pragma unroll
             for (int u = 0; u < kEvalU; ++u) {
-                g[u] = base + 64u * u + lane;
-                const int o = owner_of(incl, g[u]);
+                const int o = ow[u];
                 const uint32_t x = (uint32_t)__shfl(excl, o, 64);
                 q[u] = make_uint4(0u, 0u, 0u, 0u);
                 c[u] = ctx[w][o];
@@ -464,45 +498,77 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         lmsk[w][lane] = msk;
         lpos[w][lane] = sp;
         wave_sync();
-        auto emit = [&](uint32_t js, int p, uint32_t pos, uint32_t cid) {
-            const uint4 q = e.s.binrec[pos];
-            const float2 xy = e.s.binxy[pos];
-            const uint32_t i1 = ((p == 0) ? js : js - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]] + 1u;
-            const uint32_t oi = q.w >> 16;
-            e.s.cell_io[cid] = i1 | (oi << 16);
+        // writes cell cid from its outer-hit record q, xy at histogram position; returns
+        // the outer hit (0-based) and inner hit (1-based) for the list entry
+        auto write_cell = [&](uint32_t js, int p, uint4 q, float2 xy, uint32_t cid, uint32_t &oh, uint32_t &i1) {
+            i1 = ((p == 0) ? js : js - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]] + 1u;
+            oh = (q.w >> 16) - 1u;
+            e.s.cell_io[cid] = i1 | ((oh + 1u) << 16);
             e.s.cell_flags[cid] = 0;
             e.s.cell_pair[cid] = (uint8_t)p;
             e.s.cell_nn[cid] = 0;
             e.s.cell_oxyzr[cid] = make_float4(xy.x, xy.y, __uint_as_float(q.x), __uint_as_float(q.y));
-            const uint32_t r = atomicAdd(&e.s.ohc_cursor[oi - 1u], 1u);
+        };
+        auto list_add = [&](uint32_t oh, uint32_t i1, uint32_t cid, uint32_t r) {
             if (r < kOhcStride) {
-                e.s.ohc_list[(size_t)(oi - 1u) * kOhcStride + r] = make_uint2(cid, i1 - 1u);
+                e.s.ohc_list[(size_t)oh * kOhcStride + r] = make_uint2(cid, i1 - 1u);
             } else if (r == kOhcStride) {
                 ovf |= PT_OVF_OUTER_HIT_CELLS;
                 const uint32_t k = atomicAdd(&e.s.misc[kMiscOhcOvf], 1u);
                 if (k < kOhcOvfCap)
-                    e.s.ohc_ovf[k] = oi - 1u;
+                    e.s.ohc_ovf[k] = oh;
                 else
                     err |= PT_INT_OVF_LIST;
             }
+        };
+        auto emit = [&](uint32_t js, int p, uint32_t pos, uint32_t cid) {
+            uint32_t oh, i1;
+            write_cell(js, p, e.s.binrec[pos], e.s.binxy[pos], cid, oh, i1);
+            list_add(oh, i1, cid, atomicAdd(&e.s.ohc_cursor[oh], 1u));
         };
         const uint32_t nm = (uint32_t)__popcll(msk);
         const uint32_t fincl = wave_incl_scan(nm);
         const uint32_t fexcl = fincl - nm;
         const uint32_t ftotal = __shfl(fincl, 63, 64);
         const uint32_t jw = T * kAggTile + w * 64u;
-        for (uint32_t base = 0; base < ftotal; base += 64u) {
-            const uint32_t g = base + lane;
-            const int o = owner_of(fincl, g);
-            const uint32_t r = g - (uint32_t)__shfl(fexcl, o, 64);
-            if (g < ftotal) {
-                const uint32_t cid = lid[w][o] + r;
-                if (cid < cap) {
-                    const uint4 so = lpos[w][o];
-                    const uint32_t k = nth_set_bit(lmsk[w][o], r);
-                    emit(jw + (uint32_t)o, (int)(so.w >> 16), cand_pos(so.x, so.y, so.z, k), cid);
+        constexpr int kFillU = 3;
+        for (uint32_t base = 0; base < ftotal; base += 64u * kFillU) {
+            // kFillU rounds at a time: loads, then cell writes and list cursors, then list entries
+            bool act[kFillU];
+            uint32_t cid[kFillU], pos[kFillU], js[kFillU], oh[kFillU], i1[kFillU], r[kFillU];
+            int pp[kFillU];
+            uint4 q[kFillU];
+            float2 xy[kFillU];
+            uint32_t gg[kFillU];
+            int ow[kFillU];
+#pragma unroll
+            for (int u = 0; u < kFillU; ++u) gg[u] = base + 64u * u + lane;
+            owners_of(fincl, gg, ow);
+#pragma unroll
+            for (int u = 0; u < kFillU; ++u) {
+                const uint32_t g = gg[u];
+                const int o = ow[u];
+                const uint32_t rk = g - (uint32_t)__shfl(fexcl, o, 64);
+                cid[u] = lid[w][o] + rk;
+                act[u] = g < ftotal && cid[u] < cap;
+                const uint4 so = lpos[w][o];
+                pp[u] = (int)(so.w >> 16);
+                js[u] = jw + (uint32_t)o;
+                pos[u] = act[u] ? cand_pos(so.x, so.y, so.z, nth_set_bit(lmsk[w][o], rk)) : 0u;
+                if (act[u]) {
+                    q[u] = e.s.binrec[pos[u]];
+                    xy[u] = e.s.binxy[pos[u]];
                 }
             }
+#pragma unroll
+            for (int u = 0; u < kFillU; ++u)
+                if (act[u]) {
+                    write_cell(js[u], pp[u], q[u], xy[u], cid[u], oh[u], i1[u]);
+                    r[u] = atomicAdd(&e.s.ohc_cursor[oh[u]], 1u);
+                }
+#pragma unroll
+            for (int u = 0; u < kFillU; ++u)
+                if (act[u]) list_add(oh[u], i1[u], cid[u], r[u]);
         }
         if (j < n_slots && (sp.w & 0xFFFFu) > 64u) {  // candidates beyond a slot's first 64: serial, rare
             SlotCtx s;
@@ -639,6 +705,8 @@ struct ConnLds {
     // per row of the group
     uint32_t rcell[kRowsGrp], rcnt[kRowsGrp];
     float rx[kRowsGrp], ry[kRowsGrp], rz[kRowsGrp], rr[kRowsGrp], rreg[kRowsGrp];
+    // pairs that passed the alignment cut, waiting for the dca cut
+    float qx1[64], qy1[64], qx2[64], qy2[64], qx3[64], qy3[64], qreg[64];
 };
 
 __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
@@ -721,15 +789,17 @@ __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs,
                 const uint32_t iexcl = iincl - items;
                 const uint32_t total = __shfl(iincl, 63, 64);
                 for (uint32_t base = 0; base < total; base += 64u * kEvalU) {
-                    uint32_t t[kEvalU], rs[kEvalU], rst[kEvalU], cl[kEvalU];
+                    uint32_t t[kEvalU], rs[kEvalU], rst[kEvalU], cl[kEvalU], gg[kEvalU];
                     int ow[kEvalU];
                     float4 o3[kEvalU];
 #pragma unroll
+                    for (int u = 0; u < kEvalU; ++u) gg[u] = base + 64u * u + lane;
+                    owners_of(iincl, gg, ow);
+#pragma unroll
                     for (int u = 0; u < kEvalU; ++u) {
-                        const uint32_t g = base + 64u * u + lane;
-                        const int o = owner_of(iincl, g);
+                        const uint32_t g = gg[u];
+                        const int o = ow[u];
                         const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), orb = (uint32_t)__shfl(rexcl, o, 64);
-                        ow[u] = o;
                         t[u] = 0;
                         rs[u] = 0;
                         rst[u] = 0;
@@ -753,24 +823,72 @@ __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs,
                             o3[u] = e.s.cell_oxyzr[t[u]];
                         }
                     }
+                    // alignment cut first; the dca cut (about 2% of the pairs get there) on the
+                    // survivors of all kEvalU rounds, compacted into the low lanes via LDS
+                    bool al[kEvalU];
+                    uint64_t am[kEvalU];
+                    uint32_t qi[kEvalU], nq = 0;
+#pragma unroll
+                    for (int u = 0; u < kEvalU; ++u) {
+                        const bool act = base + 64u * u + lane < total;
+                        al[u] = false;
+                        if (act) {
+                            const int o = ow[u];
+                            const uint32_t r = rs[u];
+                            al[u] = are_aligned(S.rr[r], S.rz[r], S.hr[o], S.hz[o], o3[u].w, o3[u].z, cfg.p.pt_min,
+                                                S.theta[o]);
+                        }
+                        am[u] = __ballot(al[u]);
+                        qi[u] = nq + (uint32_t)__popcll(am[u] & lanes_below());
+                        nq += (uint32_t)__popcll(am[u]);
+                    }
+                    bool pass[kEvalU];
+                    if (nq <= 64u) {
+                        if (nq) {
+#pragma unroll
+                            for (int u = 0; u < kEvalU; ++u)
+                                if (al[u]) {
+                                    const int o = ow[u];
+                                    const uint32_t r = rs[u];
+                                    S.qx1[qi[u]] = S.rx[r];
+                                    S.qy1[qi[u]] = S.ry[r];
+                                    S.qreg[qi[u]] = S.rreg[r];
+                                    S.qx2[qi[u]] = S.hx[o];
+                                    S.qy2[qi[u]] = S.hy[o];
+                                    S.qx3[qi[u]] = o3[u].x;
+                                    S.qy3[qi[u]] = o3[u].y;
+                                }
+                            wave_sync();
+                            const bool ok = lane < nq && dca_cut(S.qx1[lane], S.qy1[lane], S.qx2[lane], S.qy2[lane],
+                                                                 S.qx3[lane], S.qy3[lane], S.qreg[lane],
+                                                                 cfg.p.hard_curv_cut);
+                            const uint64_t okm = __ballot(ok);
+#pragma unroll
+                            for (int u = 0; u < kEvalU; ++u) pass[u] = al[u] && ((okm >> qi[u]) & 1ull);
+                            wave_sync();
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < kEvalU; ++u) pass[u] = false;
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < kEvalU; ++u) {
+                            const int o = ow[u];
+                            const uint32_t r = rs[u];
+                            pass[u] = al[u] && dca_cut(S.rx[r], S.ry[r], S.hx[o], S.hy[o], o3[u].x, o3[u].y, S.rreg[r],
+                                                       cfg.p.hard_curv_cut);
+                        }
+                    }
 #pragma unroll
                     for (int u = 0; u < kEvalU; ++u) {
                         const uint32_t rbase = base + 64u * u;
                         const uint32_t g = rbase + lane;
                         const bool act = g < total;
-                        bool pass = false;
-                        if (act) {
-                            const int o = ow[u];
-                            const uint32_t r = rs[u];
-                            const ConnCtx c{S.rx[r], S.ry[r], S.rz[r], S.rr[r], S.hx[o], S.hy[o], S.hz[o], S.hr[o],
-                                            S.rreg[r], S.theta[o]};
-                            pass = conn_pass(c, cfg, o3[u]);
-                        }
-                        const uint64_t pm = __ballot(pass);
+                        const uint64_t pm = __ballot(pass[u]);
                         const uint32_t lo = (act && rst[u] > rbase) ? rst[u] - rbase : 0u;
                         const uint32_t prev = act ? S.rcnt[rs[u]] : 0u;
                         wave_sync();
-                        if (pass) {
+                        if (pass[u]) {
                             const uint32_t cell = S.rcell[rs[u]];
                             const uint32_t rk = prev + (uint32_t)__popcll(pm & lanes_below() & ~range_mask(0u, lo));
                             if (rk < 4u) nbr4u[(size_t)cell * 4u + rk] = t[u];
@@ -824,10 +942,11 @@ __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs,
 }
 
 // Cells with more than 4 neighbours (rare): walk the candidates again in id
-// order and put neighbours 5..64 in the per-event pool.
-__global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
-    uint32_t err = 0;
-    PT_EVENT_LOOP(e, i, e.s.misc[kMiscExt]) {
+// order and put neighbours 5..64 in the per-event pool.  Items i0, i0+stride, ...
+__device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config &cfg, uint32_t i0, uint32_t stride,
+                                                  uint32_t &err) {
+    const uint32_t n_ext = e.s.misc[kMiscExt];
+    for (uint32_t i = i0; i < n_ext; i += stride) {
         const uint32_t o = e.s.ext_list[i];
         const uint32_t nkeep = e.s.cell_nn[o];
         const uint32_t base = atomicAdd(&e.s.misc[kMiscPool], nkeep - 4u);
@@ -857,6 +976,15 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
             }
         }
     }
+}
+
+// Used when the fishbone (which otherwise does this work first) is off.
+__global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    uint32_t err = 0;
+    connect_ext_items(e, cfg, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
     wave_or_atomic(&e.cnt->internal_error, err);
 }
 
@@ -875,22 +1003,28 @@ constexpr uint32_t kFishRows = 128;
 struct FishLds {
     uint32_t id[kFishRows], ud[kFishRows], first[kFishRows];  // ud: module (16 b) | used << 16
     float dx[kFishRows], dy[kFishRows], dz[kFishRows], n[kFishRows];
-    unsigned long long q[2][kFishRows];  // rows (index in the hit) qualifying with this one as ic
+    unsigned long long qt[2][kFishRows];  // rows (index in the hit) that qualify with this one as jc
 };
 
-// pair k of the b-major enumeration (0,1), (0,2), (1,2), (0,3), ...: (k - b(b-1)/2, b)
+// pair k of the b-major enumeration (0,1), (0,2), (1,2), (0,3), ...: (k - b(b-1)/2, b);
+// the hardware square root only seeds the integer result, which is then corrected
 __device__ __forceinline__ uint32_t tri_b(uint32_t k) {
-    uint32_t bb = (uint32_t)((1.0f + sqrtf(8.0f * (float)k + 1.0f)) * 0.5f);
+    uint32_t bb = (uint32_t)((1.0f + __builtin_amdgcn_sqrtf(8.0f * (float)k + 1.0f)) * 0.5f);
     if (bb * (bb - 1u) / 2u > k) --bb;
     if ((bb + 1u) * bb / 2u <= k) ++bb;
     return bb;
 }
 
-__global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs, EvGrid grid) {
+__global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ FishLds lds_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
+    {  // the rare neighbour lists longer than 4 first (independent of the fishbone)
+        uint32_t err = 0;
+        connect_ext_items(e, cfg, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
+        wave_or_atomic(&e.cnt->internal_error, err);
+    }
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     FishLds &S = lds_all[w];
     const uint32_t n_hits = e.n_hits;
@@ -928,8 +1062,8 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                         S.dz[rr] = dz;
                         S.n[rr] = dx * dx + dy * dy + dz * dz;
                         S.first[rr] = ~0u;
-                        S.q[0][rr] = 0ull;
-                        S.q[1][rr] = 0ull;
+                        S.qt[0][rr] = 0ull;
+                        S.qt[1][rr] = 0ull;
                     }
                 }
                 wave_sync();
@@ -954,8 +1088,8 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                             const float ni = S.n[ri], nj = S.n[rj];
                             const float cos12 = S.dx[ri] * S.dx[rj] + S.dy[ri] * S.dy[rj] + S.dz[ri] * S.dz[rj];
                             if (cos12 * cos12 >= 0.99999f * ni * nj) {
-                                const uint32_t jl = rj - s0;
-                                atomicOr(&S.q[jl >> 6][ri], 1ull << (jl & 63u));
+                                const uint32_t il = ri - s0;
+                                atomicOr(&S.qt[il >> 6][rj], 1ull << (il & 63u));
                                 if (ni > nj) atomicMin(&S.first[ri], a_ic ? idb : ida);
                             }
                         }
@@ -966,14 +1100,20 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                 for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
                     const uint32_t rr = r0 + lane;
                     const int o = owner_of(rincl, rr);
-                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64), n_o = (uint32_t)__shfl(v, o, 64);
+                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
                     if (rr < rows) {
                         const uint32_t myid = S.id[rr];
                         bool kill = S.first[rr] != ~0u;
-                        const uint32_t ml = rr - s0;
-                        const unsigned long long bit = 1ull << (ml & 63u);
-                        for (uint32_t x = s0; x < s0 + n_o && !kill; ++x)
-                            if ((S.q[ml >> 6][x] & bit) && myid < S.first[x]) kill = true;
+                        // the rows ic that qualify with this row as jc (rare): killed if jc < first(ic)
+#pragma unroll
+                        for (int hw = 0; hw < 2; ++hw) {
+                            unsigned long long m = S.qt[hw][rr];
+                            while (m && !kill) {
+                                const uint32_t x = s0 + 64u * hw + (uint32_t)__builtin_ctzll(m);
+                                m &= m - 1ull;
+                                if (myid < S.first[x]) kill = true;
+                            }
+                        }
                         if (kill) {
                             e.s.cell_flags[myid] = kUsed | kKilled;
                             ++killed;
@@ -1028,7 +1168,7 @@ __device__ __forceinline__ void dfs(const EvDesc &e, uint32_t s, uint32_t min_do
 
 // Start candidates (cells with neighbours on a start pair) that the fishbone
 // left alive: count their tuples and tuple hits, one lane per candidate, and
-// sum them per tile of kAggTile cells.
+// sum them per tile of kDfsTile cells.
 __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     uint32_t ref = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
@@ -1044,30 +1184,41 @@ __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ ev
             v = ((uint64_t)nt << 32) | nh;
         }
         e.s.tup[c] = v;
-        if (v) atomicAdd(reinterpret_cast<unsigned long long *>(&e.s.dfs_agg[c / kAggTile]), (unsigned long long)v);
+        if (v) atomicAdd(reinterpret_cast<unsigned long long *>(&e.s.dfs_agg[c / kDfsTile]), (unsigned long long)v);
     }
     wave_or_atomic(&e.cnt->ref_assert, ref);
 }
 
 // Exclusive scan, in cell order, of the start cells' (tuples, hits): block per
-// tile of kAggTile cells, the tile's prefix summed from the preceding tiles.
+// tile of kDfsTile cells (4 per thread), the tile's prefix summed from the
+// preceding tiles' totals.
 __global__ void __launch_bounds__(256) k_dfs_scan(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint64_t lsum[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     const uint32_t n_cells = e.cnt->n_cells;
-    const uint32_t n_tiles = (n_cells + kAggTile - 1) / kAggTile;
+    const uint32_t n_tiles = (n_cells + kDfsTile - 1) / kDfsTile;
     for (uint32_t T = b; T < n_tiles; T += grid.per_event) {
         uint64_t part = 0;
         for (uint32_t k = threadIdx.x; k < T; k += 256u) part += e.s.dfs_agg[k];
+        const uint32_t c0 = T * kDfsTile + threadIdx.x * 4u;
+        const uint64_t bits = (c0 < n_cells) ? (e.s.start_bits[c0 >> 6] >> (c0 & 63u)) : 0ull;
+        uint64_t v[4];
+        uint64_t mine = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            v[u] = (((bits >> u) & 1ull) && c0 + u < n_cells) ? e.s.tup[c0 + u] : 0ull;
+            mine += v[u];
+        }
         const uint64_t pre = block_sum(part, lsum);
-        const uint32_t c = T * kAggTile + threadIdx.x;
-        const bool st = c < n_cells && ((e.s.start_bits[c >> 6] >> (c & 63u)) & 1ull);
-        const uint64_t v = st ? e.s.tup[c] : 0ull;
         uint64_t tot;
-        const uint64_t ex = block_excl_scan4(v, lsum, tot);
-        if (v) e.s.tup[c] = pre + ex;
+        uint64_t run = pre + block_excl_scan4(mine, lsum, tot);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            if (v[u]) e.s.tup[c0 + u] = run;
+            run += v[u];
+        }
         if (T == n_tiles - 1 && threadIdx.x == 0) {
             const uint64_t all = pre + tot;
             e.cnt->n_tuples = (uint32_t)(all >> 32);
@@ -1362,18 +1513,18 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     const EvGrid g_conn = ev_grid_resident(k_connect, n_events);
     k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
     mark("k_connect");
-    k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
-    mark("k_connect_ext");
     if (cfg.p.early_fish_bone) {
         const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events);
-        k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish);
+        k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish, cfg);
+    } else {
+        k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
     }
     mark("k_fishbone");
     const EvGrid g_start = ev_grid(est_cells / 8u, n_events);
     k_dfs_count<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_count");
-    const EvGrid g_cells = ev_grid(est_cells, n_events);
-    k_dfs_scan<<<ev_grid_dim(g_cells), 256, 0, st>>>(d_evs, g_cells);
+    const EvGrid g_scan = ev_grid(est_cells / 4u, n_events);
+    k_dfs_scan<<<ev_grid_dim(g_scan), 256, 0, st>>>(d_evs, g_scan);
     mark("k_dfs_scan");
     k_dfs_emit<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_emit");

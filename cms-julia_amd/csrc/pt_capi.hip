@@ -1,5 +1,7 @@
-// pt_capi.cpp - C ABI of the pixel-track library (include/pixeltrack.h):
+// pt_capi.hip - C ABI of the pixel-track library (include/pixeltrack.h):
 // contexts, the event-slot arena, descriptor upload and kernel sequencing.
+// Host code only (compiled without a device pass); the kernels and their
+// launchers live in pt_ca.hip and pt_fit.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,7 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
 void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm);
 void launch_fit_flat(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double B, int precision,
                      float *state, float *cov, float *pt, float *eta, float *chi2, hipStream_t st);
+void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
 }  // namespace pt
 
 namespace {
@@ -656,14 +659,10 @@ int pt_memcpy_d2h(pt_ctx *c, void *dst, const void *src, size_t bytes) {
     HIP_OK(hipStreamSynchronize(c->stream));
     return PT_OK;
 }
-__global__ void k_fill_bytes(uint8_t *p, uint8_t v, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
-}
 int pt_memset(pt_ctx *c, void *dst, int value, size_t bytes) {
     if (!c) return fail(PT_ERR_ARG, "null context");
     if (!bytes) return PT_OK;
-    size_t blocks = std::min<size_t>((bytes + 255) / 256, 65535);
-    k_fill_bytes<<<(unsigned)blocks, 256, 0, c->stream>>>((uint8_t *)dst, (uint8_t)value, bytes);
+    pt::launch_fill_bytes(dst, (uint8_t)value, bytes, c->stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
     return PT_OK;

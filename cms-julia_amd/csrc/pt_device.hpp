@@ -12,6 +12,16 @@
 
 namespace pt {
 
+// Device pointers in the descriptors are typed as global-memory pointers in
+// the device pass, so every access compiles to global_* instructions (a flat
+// access would also count against the LDS wait counter and stall LDS waits
+// behind memory loads); the host pass sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PT_G __attribute__((address_space(1)))
+#else
+#define PT_G
+#endif
+
 constexpr uint32_t kMaxHits = PT_MAX_HITS;
 constexpr uint32_t kMaxCells = PT_MAX_DOUBLETS;
 constexpr uint32_t kMaxSlots = 6u * PT_MAX_HITS;   // layer 0 is the inner layer of 6 pairs
@@ -22,7 +32,8 @@ constexpr uint32_t kOhcOvfCap = kMaxCells / (PT_MAX_CELLS_PER_HIT + 1) + 1;  // 
 constexpr uint32_t kTrkOvfCap = 4u * PT_MAX_TUPLES / (PT_MAX_CELL_TRACKS + 1) + 8u;
 constexpr uint32_t kAggTile = 256;                // items per prefix-sum tile (one block)
 constexpr uint32_t kDblAgg = (kMaxSlots + kAggTile - 1) / kAggTile;
-constexpr uint32_t kDfsAgg = (kMaxCells + kAggTile - 1) / kAggTile;
+constexpr uint32_t kDfsTile = 1024;               // cells per prefix-sum tile of the ntuplet scan
+constexpr uint32_t kDfsAgg = (kMaxCells + kDfsTile - 1) / kDfsTile;
 
 // Layer pairs (gpu_pixel_doublets.jl:20-28) and per-pair cuts (:29-56)
 struct PairTables {
@@ -36,42 +47,42 @@ struct PairTables {
 
 // Scratch of one event slot (device pointers into the context arena).
 struct Scratch {
-    uint32_t *slot_cnt;     // [kMaxSlots+1]  passing candidates per inner slot, then the
+    PT_G uint32_t *slot_cnt;     // [kMaxSlots+1]  passing candidates per inner slot, then the
                             //                slot's first cell id (total at [n_slots])
-    uint64_t *slot_mask;    // [kMaxSlots]    passing candidates among the slot's first 64
-    uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
-    uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kAggTile slots
-    uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
-    uint8_t *cell_flags;    // [kMaxCells]    kUsed (connect), kKilled (fishbone)
-    uint8_t *cell_pair;     // [kMaxCells]    layer pair
-    uint8_t *cell_nn;       // [kMaxCells]    stored outer neighbours (<= 64)
-    float4 *cell_oxyzr;     // [kMaxCells]    (x, y, z, r) of the outer hit
-    uint4 *nbr4;            // [kMaxCells]    first 4 neighbours (0-based cell ids, ascending)
-    uint32_t *nbr_ext;      // [kMaxCells]    pool offset of neighbours 5..64
-    uint32_t *nbr_pool;     // [kNbrPool]
-    uint32_t *ext_list;     // [kMaxCells]    cells with more than 4 neighbours
-    uint2 *trk;             // [kMaxCells]    (tuples through the cell, largest hit count among them)
-    uint32_t *start_list;   // [kMaxCells]    ntuplet start candidates (any order)
-    uint64_t *start_bits;   // [kMaxCells/64] the same as a bit set
-    uint64_t *tup;          // [kMaxCells]    start cell: (tuples << 32 | hits), then the prefix
-    uint64_t *dfs_agg;      // [kDfsAgg]      the same summed per tile of kAggTile cells
-    uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
-    uint2 *ohc_list;        // [kMaxHits*kOhcStride] (cell, inner hit) 0-based, unordered unless
+    PT_G uint64_t *slot_mask;    // [kMaxSlots]    passing candidates among the slot's first 64
+    PT_G uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
+    PT_G uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kAggTile slots
+    PT_G uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
+    PT_G uint8_t *cell_flags;    // [kMaxCells]    kUsed (connect), kKilled (fishbone)
+    PT_G uint8_t *cell_pair;     // [kMaxCells]    layer pair
+    PT_G uint8_t *cell_nn;       // [kMaxCells]    stored outer neighbours (<= 64)
+    PT_G float4 *cell_oxyzr;     // [kMaxCells]    (x, y, z, r) of the outer hit
+    PT_G uint4 *nbr4;            // [kMaxCells]    first 4 neighbours (0-based cell ids, ascending)
+    PT_G uint32_t *nbr_ext;      // [kMaxCells]    pool offset of neighbours 5..64
+    PT_G uint32_t *nbr_pool;     // [kNbrPool]
+    PT_G uint32_t *ext_list;     // [kMaxCells]    cells with more than 4 neighbours
+    PT_G uint2 *trk;             // [kMaxCells]    (tuples through the cell, largest hit count among them)
+    PT_G uint32_t *start_list;   // [kMaxCells]    ntuplet start candidates (any order)
+    PT_G uint64_t *start_bits;   // [kMaxCells/64] the same as a bit set
+    PT_G uint64_t *tup;          // [kMaxCells]    start cell: (tuples << 32 | hits), then the prefix
+    PT_G uint64_t *dfs_agg;      // [kDfsAgg]      the same summed per tile of kDfsTile cells
+    PT_G uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
+    PT_G uint2 *ohc_list;        // [kMaxHits*kOhcStride] (cell, inner hit) 0-based, unordered unless
                             //                the hit overflowed (then the first 128 cells, ascending)
-    uint32_t *ohc_ovf;      // [kOhcOvfCap]   hits with more than kOhcStride cells
-    uint32_t *trk_ovf;      // [kTrkOvfCap]   cells whose tuple list needs the exact path
-    uint32_t *tuple_cells;  // [kMaxTuples*4] cells of each stored tuple
-    uint32_t *tm_off;       // [16]           tuple multiplicity (9 used)
-    uint16_t *tm_bins;      // [kMaxTuples]
-    uint32_t *h2t_off;      // [kMaxHits+2]   hit -> loose tuples (classification)
-    uint16_t *h2t_bins;     // [4*PT_MAX_TUPLES... 196608]
-    uint32_t *h2t_cursor;   // [kMaxHits]
-    uint32_t *misc;         // [16]           internal scalars (see Misc)
-    float4 *hxyzr;          // [kMaxHits]     packed (x, y, z, r) per hit
-    uint4 *hdbl;            // [kMaxHits]     packed (z, r, iphi | det << 16, ysize) per hit
-    uint4 *binrec;          // [kMaxHits]     per histogram position: (z, r, iphi | det << 16,
+    PT_G uint32_t *ohc_ovf;      // [kOhcOvfCap]   hits with more than kOhcStride cells
+    PT_G uint32_t *trk_ovf;      // [kTrkOvfCap]   cells whose tuple list needs the exact path
+    PT_G uint32_t *tuple_cells;  // [kMaxTuples*4] cells of each stored tuple
+    PT_G uint32_t *tm_off;       // [16]           tuple multiplicity (9 used)
+    PT_G uint16_t *tm_bins;      // [kMaxTuples]
+    PT_G uint32_t *h2t_off;      // [kMaxHits+2]   hit -> loose tuples (classification)
+    PT_G uint16_t *h2t_bins;     // [4*PT_MAX_TUPLES... 196608]
+    PT_G uint32_t *h2t_cursor;   // [kMaxHits]
+    PT_G uint32_t *misc;         // [16]           internal scalars (see Misc)
+    PT_G float4 *hxyzr;          // [kMaxHits]     packed (x, y, z, r) per hit
+    PT_G uint4 *hdbl;            // [kMaxHits]     packed (z, r, iphi | det << 16, ysize) per hit
+    PT_G uint4 *binrec;          // [kMaxHits]     per histogram position: (z, r, iphi | det << 16,
                             //                ysize | hit << 16) of the hit stored there
-    float2 *binxy;          // [kMaxHits]     per histogram position: (x, y)
+    PT_G float2 *binxy;          // [kMaxHits]     per histogram position: (x, y)
 };
 
 constexpr uint8_t kUsed = 1, kKilled = 2;
@@ -93,30 +104,30 @@ enum Misc : int {
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,
 // copied to the device once per batch.
 struct EvDesc {
-    const float *xg, *yg, *zg, *rg, *xerr, *yerr;
-    const int16_t *iphi, *ysize, *det;
-    const uint32_t *hist_off;
-    const uint16_t *hist_bins;
+    PT_G const float *xg, *yg, *zg, *rg, *xerr, *yerr;
+    PT_G const int16_t *iphi, *ysize, *det;
+    PT_G const uint32_t *hist_off;
+    PT_G const uint16_t *hist_bins;
     uint32_t n_hits;
     uint32_t n_pairs;         // layer pairs in use (19, 15 or 13)
     uint32_t n_slots;         // sum of inner-layer sizes over the pairs
     uint32_t layer_start[11];
     uint32_t slot_cum[20];    // inclusive cumulative inner-layer sizes per pair
     // outputs (TrackSOA)
-    int32_t *quality;
-    float *chi2, *eta, *pt, *state, *cov;
-    uint32_t *hit_off;
-    uint16_t *hit_bins;
-    uint32_t *det_off;
-    uint16_t *det_bins;
-    pt_counters *cnt;         // device counters of this event
+    PT_G int32_t *quality;
+    PT_G float *chi2, *eta, *pt, *state, *cov;
+    PT_G uint32_t *hit_off;
+    PT_G uint16_t *hit_bins;
+    PT_G uint32_t *det_off;
+    PT_G uint16_t *det_bins;
+    PT_G pt_counters *cnt;         // device counters of this event
     Scratch s;
 };
 
 struct Config {
     pt_params p;
     double B;                 // Float64(Float32(b_field))
-    const float *frames;      // [1856*9]
+    PT_G const float *frames;      // [1856*9]
     PairTables pairs;
 };
 

@@ -9,6 +9,8 @@
 // k_fit_flat: fit of pre-gathered tracks (fit-only workloads).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pt_bl_fit.hpp"
 #include "pt_device.hpp"
 
@@ -142,6 +144,14 @@ void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg
     if (tm) tm->record("k_fit_tuples<4>", st);
     if (cfg.p.fit_5_as_4) k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, grid, cfg, 5);
     if (tm) tm->record("k_fit_tuples<4>(5 hits)", st);
+}
+
+__global__ void k_fill_bytes(uint8_t *p, uint8_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((bytes + 255) / 256, 65535);
+    k_fill_bytes<<<(unsigned)blocks, 256, 0, st>>>((uint8_t *)dst, value, bytes);
 }
 
 void launch_fit_flat(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double B, int precision,
