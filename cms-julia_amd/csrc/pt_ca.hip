@@ -324,7 +324,7 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
 __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, EvGrid grid, int zero_state) {
     constexpr uint32_t kHead = 32u;
     constexpr uint32_t kBits = (kMaxCells + 63u) / 64u;
-    PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kAggTile - 1) / kAggTile + kDfsAgg + kBits +
+    PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kSlotTile - 1) / kSlotTile + kDfsAgg + kBits +
                             (zero_state ? 20u * kMaxTuples : 0u)) {
         uint32_t k = i;
         if (k < kHead) {
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
             continue;
         }
         k -= e.n_hits;
-        const uint32_t n_dbl_agg = (e.n_slots + kAggTile - 1) / kAggTile;
+        const uint32_t n_dbl_agg = (e.n_slots + kSlotTile - 1) / kSlotTile;
         if (k < n_dbl_agg) {
             e.s.dbl_agg[k] = 0u;
             continue;
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
 // (about 16 per slot) are spread over all 64 lanes, kEvalU rounds of loads in
 // flight at a time; the pass bits come back to the slot's lane through the
 // ballot: the slot's count and, for its first 64 candidates, a bit mask.  Per
-// tile of kAggTile slots the counts are also summed.
+// tile of kSlotTile slots the counts are also summed.
 constexpr int kEvalU = 4;
 
 __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
@@ -439,22 +439,59 @@ __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict_
             e.s.slot_pos[j] = make_uint4(s.a0, s.na, s.b0, n | (s.pk & 0xFFFF0000u));
         }
         const uint32_t sum = wave_sum(cnt);
-        if (lane == 0 && sum) atomicAdd(&e.s.dbl_agg[j0 / kAggTile], sum);
+        if (lane == 0 && sum) atomicAdd(&e.s.dbl_agg[j0 / kSlotTile], sum);
     }
     wave_or_atomic(&e.cnt->ref_assert, ref);
 }
 
-// ------------------------------------------------------------------ K2: doublet fill
-// A block takes a tile of kAggTile slots: its first cell id is the sum of the
-// preceding tiles' counts, a block scan gives each slot's first id (stored in
-// place of its count).  Then, per wavefront, the passing candidates are spread
-// over the lanes and each lane writes one cell: the (inner, outer) record, the
-// pair, the outer hit's coordinates, cleared flags and neighbour count, and an
-// entry in the outer hit's list.  Doublets beyond max_num_of_doublets are
-// counted but not stored, like the serial loop.
-__global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+// ------------------------------------------------------------------ K2: doublet ids
+// Exclusive scan of the slot counts in slot order, in place: block per tile of
+// kSlotTile slots (4 per thread), the tile's prefix summed from the preceding
+// tiles' totals.  Slot j's first cell id lands in slot_cnt[j], the total in
+// slot_cnt[n_slots].
+__global__ void __launch_bounds__(256) k_doublet_scan(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ uint32_t lsum[4];
-    __shared__ uint32_t ltk;
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t n_slots = e.n_slots;
+    const uint32_t n_tiles = (n_slots + kSlotTile - 1) / kSlotTile;
+    const uint32_t cap = cfg.p.max_num_of_doublets;
+    for (uint32_t T = b; T < n_tiles; T += grid.per_event) {
+        uint32_t part = 0;
+        for (uint32_t k = threadIdx.x; k < T; k += 256u) part += e.s.dbl_agg[k];
+        const uint32_t j0 = T * kSlotTile + threadIdx.x * 4u;
+        uint32_t v[4], mine = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            v[u] = (j0 + u < n_slots) ? e.s.slot_cnt[j0 + u] : 0u;
+            mine += v[u];
+        }
+        const uint32_t pre = block_sum(part, lsum);
+        uint32_t tot;
+        uint32_t run = pre + block_excl_scan4(mine, lsum, tot);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            if (j0 + u < n_slots) e.s.slot_cnt[j0 + u] = run;
+            run += v[u];
+        }
+        if (T == n_tiles - 1 && threadIdx.x == 0) {
+            const uint32_t all = pre + tot;
+            e.s.slot_cnt[n_slots] = all;
+            e.cnt->n_cells_found = all;
+            e.cnt->n_cells = all < cap ? all : cap;
+            if (all > cap) atomicOr(&e.cnt->overflow, PT_OVF_DOUBLETS);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ K3: doublet fill
+// A wavefront takes a tile of 64 slots by ticket; the passing candidates of the
+// tile are spread over the lanes and each lane writes one cell: the (inner,
+// outer) record, the pair, the outer hit's coordinates, cleared flags and
+// neighbour count, and an entry in the outer hit's list.  Doublets beyond
+// max_num_of_doublets are counted but not stored, like the serial loop.
+__global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ uint32_t lid[4][64];
     __shared__ unsigned long long lmsk[4][64];
     __shared__ uint4 lpos[4][64];
@@ -463,37 +500,23 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
     const EvDesc &e = evs[ev];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t n_slots = e.n_slots;
-    const uint32_t n_agg = (n_slots + kAggTile - 1) / kAggTile;
+    const uint32_t n_tiles = (n_slots + 63u) / 64u;
     const uint32_t cap = cfg.p.max_num_of_doublets;
     uint32_t ref = 0, ovf = 0, err = 0;
     for (;;) {
-        if (threadIdx.x == 0) ltk = atomicAdd(&e.s.misc[kMiscTkFill], 1u);
-        __syncthreads();
-        const uint32_t T = ltk;
-        __syncthreads();
-        if (T >= n_agg) break;
-        uint32_t part = 0;
-        for (uint32_t k = threadIdx.x; k < T; k += 256u) part += e.s.dbl_agg[k];
-        const uint32_t pre = block_sum(part, lsum);
-        const uint32_t j = T * kAggTile + threadIdx.x;
-        const uint32_t cnt = (j < n_slots) ? e.s.slot_cnt[j] : 0u;
+        const uint32_t t = take_ticket(&e.s.misc[kMiscTkFill]);
+        if (t >= n_tiles) break;
+        const uint32_t j = t * 64u + lane;
+        uint32_t id = 0;
         uint64_t msk = 0;
         uint4 sp = make_uint4(0u, 0u, 0u, 0u);
         if (j < n_slots) {
+            id = e.s.slot_cnt[j];
             msk = e.s.slot_mask[j];
             sp = e.s.slot_pos[j];
         }
-        uint32_t tot;
-        const uint32_t id = pre + block_excl_scan4(cnt, lsum, tot);
-        if (j < n_slots) e.s.slot_cnt[j] = id;
-        if (T == n_agg - 1 && threadIdx.x == 0) {
-            const uint32_t all = pre + tot;
-            e.s.slot_cnt[n_slots] = all;
-            e.cnt->n_cells_found = all;
-            e.cnt->n_cells = all < cap ? all : cap;
-            if (all > cap) ovf |= PT_OVF_DOUBLETS;
-        }
-        if (pre >= cap) continue;  // nothing of this tile is stored (uniform in the block)
+        wave_sync();  // the previous tile's reads of the LDS arrays are done
+        if (__shfl(id, 0, 64) >= cap) continue;  // nothing of this tile is stored
         lid[w][lane] = id;
         lmsk[w][lane] = msk;
         lpos[w][lane] = sp;
@@ -530,7 +553,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         const uint32_t fincl = wave_incl_scan(nm);
         const uint32_t fexcl = fincl - nm;
         const uint32_t ftotal = __shfl(fincl, 63, 64);
-        const uint32_t jw = T * kAggTile + w * 64u;
+        const uint32_t jw = t * 64u;
         constexpr int kFillU = 3;
         for (uint32_t base = 0; base < ftotal; base += 64u * kFillU) {
             // kFillU rounds at a time: loads, then cell writes and list cursors, then list entries
@@ -688,28 +711,24 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 
 // Per hit h, the rows are the cells ending on h that kernel_connect visits (the
 // first 128 of h's list) and the columns the cells starting on h, in id order
-// (one contiguous id range per layer pair).  A wavefront takes kHitTile hits
-// per ticket and packs them into groups of at most kRowsGrp rows; the rows of
-// a group are loaded once into LDS, and every (row, column) pair of the group
-// is one lane's work, kEvalU rounds of column loads in flight.  The passing
-// columns of a row, in order, are the row cell's outer neighbours: the rank of
-// a pass within its row is the passes of earlier rounds (per-row count in LDS)
-// plus those of lower lanes of the same row.
-constexpr uint32_t kHitTile = 16;
-constexpr uint32_t kRowsGrp = 128;
+// (one contiguous id range per layer pair).  A wavefront takes kConnHits hits
+// per ticket, sorts them by column count (so that rows of similar length share
+// a round), and gives every lane one row: the lane walks the row's columns in
+// id order, kConnCols at a time, and keeps the passing ones in order - the
+// serial push! order of the row cell's neighbour list.  The alignment cut
+// runs in every lane; the dca cut (about 2% of the pairs get there) runs on
+// the survivors of a whole round of columns, compacted through LDS.
+constexpr uint32_t kConnHits = 32;
+constexpr int kConnCols = 4;
 
 struct ConnLds {
-    // per hit of the tile
-    float hx[kHitTile], hy[kHitTile], hz[kHitTile], hr[kHitTile], theta[kHitTile], rcpc[kHitTile];
-    uint32_t cols[kHitTile], a[6][kHitTile], cum[6][kHitTile];
-    // per row of the group
-    uint32_t rcell[kRowsGrp], rcnt[kRowsGrp];
-    float rx[kRowsGrp], ry[kRowsGrp], rz[kRowsGrp], rr[kRowsGrp], rreg[kRowsGrp];
+    float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
+    uint32_t cols[kConnHits], a[6][kConnHits], cum[6][kConnHits];
     // pairs that passed the alignment cut, waiting for the dca cut
     float qx1[64], qy1[64], qx2[64], qy2[64], qx3[64], qy3[64], qreg[64];
 };
 
-__global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+__global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ ConnLds lds_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
@@ -717,15 +736,14 @@ __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs,
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     ConnLds &S = lds_all[w];
     const uint32_t n_cells = e.cnt->n_cells, n_hits = e.n_hits;
-    uint32_t *nbr4u = reinterpret_cast<uint32_t *>(e.s.nbr4);
     uint32_t n_eval = 0, n_conn = 0, n_lists = 0, ovf = 0;
     for (;;) {
-        const uint32_t h0 = take_ticket(&e.s.misc[kMiscTkConn]) * kHitTile;
+        const uint32_t h0 = take_ticket(&e.s.misc[kMiscTkConn]) * kConnHits;
         if (h0 >= n_hits) break;
         wave_sync();  // the previous tile's reads of S are done
         const uint32_t h = h0 + lane;
         uint32_t R = 0, C = 0;
-        if (lane < kHitTile && h < n_hits) {
+        if (lane < kConnHits && h < n_hits) {
             R = min(e.s.ohc_cursor[h], kOhcStride);
             if (R) {
                 const int L = layer_of(e, h);
@@ -750,189 +768,138 @@ __global__ void __launch_bounds__(256) k_connect(const EvDesc *__restrict__ evs,
                     S.hz[lane] = p.z;
                     S.hr[lane] = p.w;
                     S.theta[lane] = theta_cut_of(e, cfg, h);
-                    S.rcpc[lane] = 1.0f / (float)C;
                 }
             }
             if (!C) R = 0;
             S.cols[lane] = C;
         }
-        wave_sync();
-        for (uint32_t first = 0; first < kHitTile;) {
-            const uint32_t v = (lane >= first) ? R : 0u;
-            const uint32_t rincl = wave_incl_scan(v);
-            const uint64_t fits = __ballot(lane >= first && lane < kHitTile && rincl <= kRowsGrp);
-            const uint32_t last = 63u - (uint32_t)__clzll(fits);
-            const uint32_t rows = __shfl(rincl, (int)last, 64);
-            if (rows) {
-                const uint32_t rexcl = rincl - v;
-                // stage the rows: cell, inner-hit coordinates, dca region
+        // sort the hits by column count, descending (bitonic over the wavefront)
+        uint32_t key = R ? ((C << 8) | (63u - lane)) : 0u;  // ties: lower lane first
 #pragma unroll
-                for (uint32_t u = 0; u < kRowsGrp / 64u; ++u) {
-                    const uint32_t rr = 64u * u + lane;
-                    const int o = owner_of(rincl, rr);
-                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
-                    if (rr < rows) {
-                        const uint2 en = e.s.ohc_list[(size_t)(h0 + (uint32_t)o) * kOhcStride + (rr - s0)];
-                        const float4 p1 = e.s.hxyzr[en.y];
-                        S.rcell[rr] = en.x;
-                        S.rcnt[rr] = 0u;
-                        S.rx[rr] = p1.x;
-                        S.ry[rr] = p1.y;
-                        S.rz[rr] = p1.z;
-                        S.rr[rr] = p1.w;
-                        S.rreg[rr] = region_of(cfg, e.s.hdbl[en.y].z >> 16);
-                    }
-                }
-                wave_sync();
-                const uint32_t items = (lane >= first && lane <= last) ? R * C : 0u;
-                const uint32_t iincl = wave_incl_scan(items);
-                const uint32_t iexcl = iincl - items;
-                const uint32_t total = __shfl(iincl, 63, 64);
-                for (uint32_t base = 0; base < total; base += 64u * kEvalU) {
-                    uint32_t t[kEvalU], rs[kEvalU], rst[kEvalU], cl[kEvalU], gg[kEvalU];
-                    int ow[kEvalU];
-                    float4 o3[kEvalU];
+        for (uint32_t k = 2; k <= 64; k <<= 1) {
 #pragma unroll
-                    for (int u = 0; u < kEvalU; ++u) gg[u] = base + 64u * u + lane;
-                    owners_of(iincl, gg, ow);
-#pragma unroll
-                    for (int u = 0; u < kEvalU; ++u) {
-                        const uint32_t g = gg[u];
-                        const int o = ow[u];
-                        const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), orb = (uint32_t)__shfl(rexcl, o, 64);
-                        t[u] = 0;
-                        rs[u] = 0;
-                        rst[u] = 0;
-                        cl[u] = 0;
-                        o3[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (g < total) {
-                            const uint32_t k = g - ox, cols = S.cols[o];
-                            uint32_t row = (uint32_t)((float)k * S.rcpc[o]);
-                            if (row * cols > k) --row;
-                            if ((row + 1u) * cols <= k) ++row;
-                            const uint32_t col = k - row * cols;
-                            uint32_t q = 0, q0 = 0;
-                            while (col >= S.cum[q][o]) {
-                                q0 = S.cum[q][o];
-                                ++q;
-                            }
-                            t[u] = S.a[q][o] + col - q0;
-                            rs[u] = orb + row;
-                            rst[u] = ox + row * cols;
-                            cl[u] = cols;
-                            o3[u] = e.s.cell_oxyzr[t[u]];
-                        }
-                    }
-                    // alignment cut first; the dca cut (about 2% of the pairs get there) on the
-                    // survivors of all kEvalU rounds, compacted into the low lanes via LDS
-                    bool al[kEvalU];
-                    uint64_t am[kEvalU];
-                    uint32_t qi[kEvalU], nq = 0;
-#pragma unroll
-                    for (int u = 0; u < kEvalU; ++u) {
-                        const bool act = base + 64u * u + lane < total;
-                        al[u] = false;
-                        if (act) {
-                            const int o = ow[u];
-                            const uint32_t r = rs[u];
-                            al[u] = are_aligned(S.rr[r], S.rz[r], S.hr[o], S.hz[o], o3[u].w, o3[u].z, cfg.p.pt_min,
-                                                S.theta[o]);
-                        }
-                        am[u] = __ballot(al[u]);
-                        qi[u] = nq + (uint32_t)__popcll(am[u] & lanes_below());
-                        nq += (uint32_t)__popcll(am[u]);
-                    }
-                    bool pass[kEvalU];
-                    if (nq <= 64u) {
-                        if (nq) {
-#pragma unroll
-                            for (int u = 0; u < kEvalU; ++u)
-                                if (al[u]) {
-                                    const int o = ow[u];
-                                    const uint32_t r = rs[u];
-                                    S.qx1[qi[u]] = S.rx[r];
-                                    S.qy1[qi[u]] = S.ry[r];
-                                    S.qreg[qi[u]] = S.rreg[r];
-                                    S.qx2[qi[u]] = S.hx[o];
-                                    S.qy2[qi[u]] = S.hy[o];
-                                    S.qx3[qi[u]] = o3[u].x;
-                                    S.qy3[qi[u]] = o3[u].y;
-                                }
-                            wave_sync();
-                            const bool ok = lane < nq && dca_cut(S.qx1[lane], S.qy1[lane], S.qx2[lane], S.qy2[lane],
-                                                                 S.qx3[lane], S.qy3[lane], S.qreg[lane],
-                                                                 cfg.p.hard_curv_cut);
-                            const uint64_t okm = __ballot(ok);
-#pragma unroll
-                            for (int u = 0; u < kEvalU; ++u) pass[u] = al[u] && ((okm >> qi[u]) & 1ull);
-                            wave_sync();
-                        } else {
-#pragma unroll
-                            for (int u = 0; u < kEvalU; ++u) pass[u] = false;
-                        }
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < kEvalU; ++u) {
-                            const int o = ow[u];
-                            const uint32_t r = rs[u];
-                            pass[u] = al[u] && dca_cut(S.rx[r], S.ry[r], S.hx[o], S.hy[o], o3[u].x, o3[u].y, S.rreg[r],
-                                                       cfg.p.hard_curv_cut);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kEvalU; ++u) {
-                        const uint32_t rbase = base + 64u * u;
-                        const uint32_t g = rbase + lane;
-                        const bool act = g < total;
-                        const uint64_t pm = __ballot(pass[u]);
-                        const uint32_t lo = (act && rst[u] > rbase) ? rst[u] - rbase : 0u;
-                        const uint32_t prev = act ? S.rcnt[rs[u]] : 0u;
-                        wave_sync();
-                        if (pass[u]) {
-                            const uint32_t cell = S.rcell[rs[u]];
-                            const uint32_t rk = prev + (uint32_t)__popcll(pm & lanes_below() & ~range_mask(0u, lo));
-                            if (rk < 4u) nbr4u[(size_t)cell * 4u + rk] = t[u];
-                            e.s.cell_flags[t[u]] = kUsed;
-                            e.s.cell_flags[cell] = kUsed;
-                            e.s.trk[t[u]] = make_uint2(0u, 0u);
-                            e.s.trk[cell] = make_uint2(0u, 0u);
-                        }
-                        if (act && (g + 1u == rst[u] + cl[u] || lane == 63u))
-                            S.rcnt[rs[u]] = prev + (uint32_t)__popcll(pm & range_mask(lo, lane + 1u));
-                        wave_sync();
-                    }
-                }
-                // the group's rows: store the neighbour count, queue long lists and start cells
-                for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
-                    const uint32_t rr = r0 + lane;
-                    const uint32_t c = (rr < rows) ? S.rcnt[rr] : 0u;
-                    const uint32_t cell = c ? S.rcell[rr] : 0u;
-                    bool st = false;
-                    if (c) {
-                        const uint32_t nn = c < PT_MAX_NEIGHBORS ? c : PT_MAX_NEIGHBORS;
-                        e.s.cell_nn[cell] = (uint8_t)nn;
-                        n_conn += nn;
-                        ++n_lists;
-                        if (c > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
-                        if (nn > 4u) e.s.ext_list[atomicAdd(&e.s.misc[kMiscExt], 1u)] = cell;
-                        st = start_pair(cfg, e.s.cell_pair[cell]);
-                    }
-                    const uint64_t sb = __ballot(st);
-                    if (sb) {
-                        uint32_t sbase = 0;
-                        if (lane == 0) sbase = atomicAdd(&e.s.misc[kMiscStart], (uint32_t)__popcll(sb));
-                        sbase = __shfl(sbase, 0, 64);
-                        if (st) {
-                            e.s.start_list[sbase + (uint32_t)__popcll(sb & lanes_below())] = cell;
-                            atomicOr(reinterpret_cast<unsigned long long *>(&e.s.start_bits[cell >> 6]),
-                                     1ull << (cell & 63u));
-                        }
-                    }
-                }
-                if (lane == 0) n_eval += total;
-                wave_sync();
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                const uint32_t other = (uint32_t)__shfl_xor(key, (int)jj, 64);
+                const bool up = ((lane & k) == 0);  // descending blocks where (lane & k) == 0
+                const bool lower = (lane & jj) == 0;
+                const bool take_max = (up == lower);
+                key = take_max ? max(key, other) : min(key, other);
             }
-            first = last + 1u;
+        }
+        const uint32_t hl = 63u - (key & 0xFFu);           // hit lane of the i-th sorted hit
+        const uint32_t Rsrc = (uint32_t)__shfl(R, (int)hl, 64);
+        const uint32_t Rs = key ? Rsrc : 0u;
+        const uint32_t rincl = wave_incl_scan(Rs);
+        const uint32_t rows = __shfl(rincl, 63, 64);
+        wave_sync();
+        for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
+            const uint32_t rr = r0 + lane;
+            const bool act = rr < rows;
+            const int oi = owner_of(rincl, rr);
+            const uint32_t ohl = (uint32_t)__shfl(hl, oi, 64);
+            const uint32_t rbase = (uint32_t)__shfl(rincl, oi, 64) - (uint32_t)__shfl(Rs, oi, 64);
+            uint32_t cell = 0, cols = 0;
+            float x1 = 0.f, y1 = 0.f, z1 = 0.f, r1 = 0.f, reg = 0.f, x2 = 0.f, y2 = 0.f, zi = 0.f, ri = 0.f, th = 0.f;
+            if (act) {
+                const uint2 en = e.s.ohc_list[(size_t)(h0 + ohl) * kOhcStride + (rr - rbase)];
+                cell = en.x;
+                const float4 p1 = e.s.hxyzr[en.y];
+                x1 = p1.x, y1 = p1.y, z1 = p1.z, r1 = p1.w;
+                reg = region_of(cfg, e.s.hdbl[en.y].z >> 16);
+                x2 = S.hx[ohl], y2 = S.hy[ohl], zi = S.hz[ohl], ri = S.hr[ohl], th = S.theta[ohl];
+                cols = S.cols[ohl];
+                n_eval += cols;
+            }
+            const uint32_t cmax = wave_max(cols);
+            uint32_t cnt = 0, q = 0;
+            uint4 nb = make_uint4(~0u, ~0u, ~0u, ~0u);
+            for (uint32_t c0 = 0; c0 < cmax; c0 += kConnCols) {
+                uint32_t t[kConnCols];
+                float4 o3[kConnCols];
+                bool al[kConnCols];
+#pragma unroll
+                for (int k = 0; k < kConnCols; ++k) {
+                    const uint32_t c = c0 + (uint32_t)k;
+                    t[k] = 0;
+                    o3[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (c < cols) {
+                        while (c >= S.cum[q][ohl]) ++q;
+                        t[k] = S.a[q][ohl] + c - (q ? S.cum[q - 1][ohl] : 0u);
+                        o3[k] = e.s.cell_oxyzr[t[k]];
+                    }
+                }
+                uint32_t nq = 0, qi[kConnCols];
+#pragma unroll
+                for (int k = 0; k < kConnCols; ++k) {
+                    al[k] = c0 + (uint32_t)k < cols && are_aligned(r1, z1, ri, zi, o3[k].w, o3[k].z, cfg.p.pt_min, th);
+                    const uint64_t am = __ballot(al[k]);
+                    qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
+                    nq += (uint32_t)__popcll(am);
+                }
+                uint32_t pm = 0;  // this lane's passing columns of the round, bit k
+                if (nq) {
+                    // survivors into LDS in (k, lane) order, the dca cut 64 at a time
+                    for (uint32_t qb = 0; qb < nq; qb += 64u) {
+#pragma unroll
+                        for (int k = 0; k < kConnCols; ++k)
+                            if (al[k] && qi[k] >= qb && qi[k] < qb + 64u) {
+                                const uint32_t s = qi[k] - qb;
+                                S.qx1[s] = x1;
+                                S.qy1[s] = y1;
+                                S.qreg[s] = reg;
+                                S.qx2[s] = x2;
+                                S.qy2[s] = y2;
+                                S.qx3[s] = o3[k].x;
+                                S.qy3[s] = o3[k].y;
+                            }
+                        wave_sync();
+                        const bool ok = qb + lane < nq && dca_cut(S.qx1[lane], S.qy1[lane], S.qx2[lane], S.qy2[lane],
+                                                                  S.qx3[lane], S.qy3[lane], S.qreg[lane],
+                                                                  cfg.p.hard_curv_cut);
+                        const uint64_t okm = __ballot(ok);
+#pragma unroll
+                        for (int k = 0; k < kConnCols; ++k)
+                            if (al[k] && qi[k] >= qb && qi[k] < qb + 64u && ((okm >> (qi[k] - qb)) & 1ull))
+                                pm |= 1u << k;
+                        wave_sync();
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kConnCols; ++k)
+                    if ((pm >> k) & 1u) {
+                        e.s.cell_flags[t[k]] = kUsed;
+                        e.s.trk[t[k]] = make_uint2(0u, 0u);
+                        if (cnt == 0) nb.x = t[k];
+                        if (cnt == 1) nb.y = t[k];
+                        if (cnt == 2) nb.z = t[k];
+                        if (cnt == 3) nb.w = t[k];
+                        ++cnt;
+                    }
+            }
+            bool st = false;
+            if (cnt) {
+                const uint32_t nn = cnt < PT_MAX_NEIGHBORS ? cnt : PT_MAX_NEIGHBORS;
+                e.s.cell_flags[cell] = kUsed;
+                e.s.trk[cell] = make_uint2(0u, 0u);
+                e.s.cell_nn[cell] = (uint8_t)nn;
+                e.s.nbr4[cell] = nb;
+                n_conn += nn;
+                ++n_lists;
+                if (cnt > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
+                if (nn > 4u) e.s.ext_list[atomicAdd(&e.s.misc[kMiscExt], 1u)] = cell;
+                st = start_pair(cfg, e.s.cell_pair[cell]);
+            }
+            const uint64_t sb = __ballot(st);
+            if (sb) {
+                uint32_t sbase = 0;
+                if (lane == 0) sbase = atomicAdd(&e.s.misc[kMiscStart], (uint32_t)__popcll(sb));
+                sbase = __shfl(sbase, 0, 64);
+                if (st) {
+                    e.s.start_list[sbase + (uint32_t)__popcll(sb & lanes_below())] = cell;
+                    atomicOr(reinterpret_cast<unsigned long long *>(&e.s.start_bits[cell >> 6]),
+                             1ull << (cell & 63u));
+                }
+            }
         }
     }
     wave_add_atomic(&e.cnt->n_connect_pairs, n_eval);
@@ -999,6 +966,7 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
 // most kFishRows list entries (rows, loaded once into LDS) and spreads every
 // unordered row pair of a group over the lanes; a second step decides each row.
 constexpr uint32_t kFishRows = 128;
+constexpr uint32_t kHitTile = 16;  // hits per ticket
 
 struct FishLds {
     uint32_t id[kFishRows], ud[kFishRows], first[kFishRows];  // ud: module (16 b) | used << 16
@@ -1497,7 +1465,7 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     };
     const EvGrid one{n_events, 1u};
     mark("start");
-    const EvGrid g_reset = ev_grid(max_hits + max_slots / kAggTile + kDfsAgg + kMaxCells / 64u + 64u +
+    const EvGrid g_reset = ev_grid(max_hits + max_slots / kSlotTile + kDfsAgg + kMaxCells / 64u + 64u +
                                        (zero_state ? 20u * kMaxTuples : 0u),
                                    n_events);
     k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
@@ -1505,6 +1473,9 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events);
     k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg);
     mark("k_doublet_count");
+    const EvGrid g_dscan = ev_grid(max_slots / 4u, n_events);
+    k_doublet_scan<<<ev_grid_dim(g_dscan), 256, 0, st>>>(d_evs, g_dscan, cfg);
+    mark("k_doublet_scan");
     const EvGrid g_fill = ev_grid_resident(k_doublet_fill, n_events);
     k_doublet_fill<<<ev_grid_dim(g_fill), 256, 0, st>>>(d_evs, g_fill, cfg);
     mark("k_doublet_fill");

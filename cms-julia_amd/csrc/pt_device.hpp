@@ -30,8 +30,8 @@ constexpr uint32_t kNbrPool = 1024u * 1024u;   // neighbour entries beyond the 4
 constexpr uint32_t kOhcStride = PT_MAX_CELLS_PER_HIT;  // outer-hit list capacity per hit
 constexpr uint32_t kOhcOvfCap = kMaxCells / (PT_MAX_CELLS_PER_HIT + 1) + 1;  // hits that can exceed it
 constexpr uint32_t kTrkOvfCap = 4u * PT_MAX_TUPLES / (PT_MAX_CELL_TRACKS + 1) + 8u;
-constexpr uint32_t kAggTile = 256;                // items per prefix-sum tile (one block)
-constexpr uint32_t kDblAgg = (kMaxSlots + kAggTile - 1) / kAggTile;
+constexpr uint32_t kSlotTile = 1024;              // slots per prefix-sum tile of the doublet scan
+constexpr uint32_t kDblAgg = (kMaxSlots + kSlotTile - 1) / kSlotTile;
 constexpr uint32_t kDfsTile = 1024;               // cells per prefix-sum tile of the ntuplet scan
 constexpr uint32_t kDfsAgg = (kMaxCells + kDfsTile - 1) / kDfsTile;
 
@@ -51,7 +51,7 @@ struct Scratch {
                             //                slot's first cell id (total at [n_slots])
     PT_G uint64_t *slot_mask;    // [kMaxSlots]    passing candidates among the slot's first 64
     PT_G uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
-    PT_G uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kAggTile slots
+    PT_G uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kSlotTile slots
     PT_G uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
     PT_G uint8_t *cell_flags;    // [kMaxCells]    kUsed (connect), kKilled (fishbone)
     PT_G uint8_t *cell_pair;     // [kMaxCells]    layer pair

@@ -1,6 +1,6 @@
 // pt_fit.hip - broken-line fit kernels for gfx950.
 //
-// k_fit_tuples<N>: the fit stage of make_tuples (launchBrokenLineKernelsOnCPU,
+// k_fit_tuples: the fit stage of make_tuples (launchBrokenLineKernelsOnCPU,
 //   broken_line_fit_on_gpu.jl:166-203): for each length bucket (3 -> N=3,
 //   4 -> N=4, 5 -> N=4 on the first four hits, fit5as4) fit the first 24,576
 //   entries of the tuple-multiplicity bucket, i.e. the highest tuple ids.
@@ -35,17 +35,14 @@ __device__ __forceinline__ uint32_t fit_flag_bits(uint32_t f) {
     return r;
 }
 
+// The fits of one length bucket, items idx0, idx0 + stride, ...
 template <int N>
-__global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg,
-                                                    int bucket) {
-    uint32_t ev, b;
-    if (!ev_block(grid, ev, b)) return;
-    const EvDesc &e = evs[ev];
+__device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, int bucket, uint32_t idx0,
+                                           uint32_t stride, uint32_t &ref, uint32_t &nfit) {
     const uint32_t b0 = e.s.tm_off[bucket - 1], b1 = e.s.tm_off[bucket];
     uint32_t n = b1 - b0;
     if (n > PT_MAX_FITS) n = PT_MAX_FITS;
-    uint32_t ref = 0, nfit = 0;
-    for (uint32_t idx = b * blockDim.x + threadIdx.x; idx < n; idx += grid.per_event * blockDim.x) {
+    for (uint32_t idx = idx0; idx < n; idx += stride) {
         const uint32_t tkid = e.s.tm_bins[b0 + idx];  // 1-based
         const uint32_t start = e.hit_off[tkid - 1] - 1u;
         double hx[N], hy[N], hz[N];
@@ -79,6 +76,23 @@ __global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ e
         e.chi2[row] = (float)o.chi2;
         ++nfit;
     }
+}
+
+// All three length buckets in one launch (3 -> N=3, 4 -> N=4, 5 -> N=4 on the
+// first four hits when fit_5_as_4): the event's blocks are split in three.
+__global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t per = grid.per_event / 3u;
+    const uint32_t bk = b / per, bb = b - bk * per;
+    uint32_t ref = 0, nfit = 0;
+    if (bk == 0)
+        fit_bucket<3>(e, cfg, 3, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
+    else if (bk == 1)
+        fit_bucket<4>(e, cfg, 4, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
+    else if (bk == 2 && cfg.p.fit_5_as_4)
+        fit_bucket<4>(e, cfg, 5, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
     uint32_t v = nfit;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -133,17 +147,12 @@ __global__ void __launch_bounds__(256) k_fit_flat_fp64(uint32_t n, const float *
 // ------------------------------------------------------------------ launchers
 void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm) {
     uint32_t per = (PT_MAX_FITS + 255) / 256;
-    const uint32_t target = (1024u + n_events - 1) / n_events;
+    const uint32_t target = (2048u + n_events - 1) / n_events / 3u;
     if (per > target) per = target;
     if (per < 1) per = 1;
-    const EvGrid grid{n_events, per};
-    const dim3 g = ev_grid_dim(grid);
-    k_fit_tuples<3><<<g, 256, 0, st>>>(d_evs, grid, cfg, 3);
-    if (tm) tm->record("k_fit_tuples<3>", st);
-    k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, grid, cfg, 4);
-    if (tm) tm->record("k_fit_tuples<4>", st);
-    if (cfg.p.fit_5_as_4) k_fit_tuples<4><<<g, 256, 0, st>>>(d_evs, grid, cfg, 5);
-    if (tm) tm->record("k_fit_tuples<4>(5 hits)", st);
+    const EvGrid grid{n_events, 3u * per};
+    k_fit_tuples<<<ev_grid_dim(grid), 256, 0, st>>>(d_evs, grid, cfg);
+    if (tm) tm->record("k_fit_tuples", st);
 }
 
 __global__ void k_fill_bytes(uint8_t *p, uint8_t v, size_t n) {
