@@ -1098,63 +1098,138 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
 // ------------------------------------------------------------------ ntuplets (gpu_ca_cell.jl:232-270)
 // DFS with the serial visiting order, one template level per depth (the
 // reference asserts at most 4 doublets per chain, gpu_ca_cell.jl:234), so the
-// path lives in registers.  leaf(path, n_cells) is called at every emission
-// point of the recursive reference.
-__device__ __forceinline__ uint32_t nbr_at(const EvDesc &e, uint4 v, uint32_t ext, uint32_t i) {
-    return (i == 0) ? v.x : (i == 1) ? v.y : (i == 2) ? v.z : (i == 3) ? v.w : e.s.nbr_pool[ext + i - 4u];
+// path lives in registers.  At each level the data of all (up to 4 inline)
+// neighbours is loaded at once - id, hits, neighbour count and list, killed
+// flag - so a level costs one memory round trip.  leaf(cells, hits, n) is
+// called at every emission point of the recursive reference.
+struct DNode {
+    uint32_t id, io, nn;
+    uint4 nb;
+    bool killed;
+};
+
+__device__ __forceinline__ DNode load_node(const EvDesc &e, uint32_t c) {
+    DNode n;
+    n.id = c;
+    n.io = e.s.cell_io[c];
+    n.nn = e.s.cell_nn[c];
+    n.killed = (e.s.cell_flags[c] & kKilled) != 0;
+    n.nb = e.s.nbr4[c];
+    return n;
+}
+__device__ __forceinline__ uint32_t comp4(uint4 v, uint32_t i) {
+    return (i == 0) ? v.x : (i == 1) ? v.y : (i == 2) ? v.z : v.w;
+}
+__device__ __forceinline__ DNode pick4(uint32_t i, const DNode &a, const DNode &b, const DNode &c, const DNode &d) {
+    DNode r;
+    r.id = (i == 0) ? a.id : (i == 1) ? b.id : (i == 2) ? c.id : d.id;
+    r.io = (i == 0) ? a.io : (i == 1) ? b.io : (i == 2) ? c.io : d.io;
+    r.nn = (i == 0) ? a.nn : (i == 1) ? b.nn : (i == 2) ? c.nn : d.nn;
+    r.killed = (i == 0) ? a.killed : (i == 1) ? b.killed : (i == 2) ? c.killed : d.killed;
+    r.nb.x = (i == 0) ? a.nb.x : (i == 1) ? b.nb.x : (i == 2) ? c.nb.x : d.nb.x;
+    r.nb.y = (i == 0) ? a.nb.y : (i == 1) ? b.nb.y : (i == 2) ? c.nb.y : d.nb.y;
+    r.nb.z = (i == 0) ? a.nb.z : (i == 1) ? b.nb.z : (i == 2) ? c.nb.z : d.nb.z;
+    r.nb.w = (i == 0) ? a.nb.w : (i == 1) ? b.nb.w : (i == 2) ? c.nb.w : d.nb.w;
+    return r;
 }
 
 template <int D, class F>
-__device__ __forceinline__ void dfs_level(const EvDesc &e, uint32_t (&stk)[4], uint32_t min_doublets,
-                                          uint32_t &ref_bits, F &leaf) {
-    const uint32_t c = stk[D];
-    const uint32_t nn = e.s.cell_nn[c];
+__device__ __forceinline__ void dfs_level(const EvDesc &e, const DNode &node, uint32_t (&cells)[4], uint32_t (&ios)[4],
+                                          uint32_t min_doublets, uint32_t &ref_bits, F &leaf) {
+    cells[D] = node.id;
+    ios[D] = node.io;
+    const uint32_t nn = node.nn;
     bool child = false;
     if (nn) {
-        const uint4 v = e.s.nbr4[c];
-        const uint32_t ext = (nn > 4u) ? e.s.nbr_ext[c] : 0u;
-        for (uint32_t i = 0; i < nn; ++i) {
-            const uint32_t n = nbr_at(e, v, ext, i);
-            if (e.s.cell_flags[n] & kKilled) continue;
-            child = true;
-            if constexpr (D + 1 < 4) {
-                stk[D + 1] = n;
-                dfs_level<D + 1>(e, stk, min_doublets, ref_bits, leaf);
-            } else {
-                ref_bits |= PT_REF_DEPTH;  // a fifth doublet: the reference's @assert fails
+        if constexpr (D + 1 < 4) {
+            DNode c0 = {}, c1 = {}, c2 = {}, c3 = {};
+            c0 = load_node(e, node.nb.x);
+            if (nn > 1) c1 = load_node(e, node.nb.y);
+            if (nn > 2) c2 = load_node(e, node.nb.z);
+            if (nn > 3) c3 = load_node(e, node.nb.w);
+            const uint32_t n4 = nn < 4u ? nn : 4u;
+            for (uint32_t i = 0; i < n4; ++i) {
+                const DNode c = pick4(i, c0, c1, c2, c3);
+                if (c.killed) continue;
+                child = true;
+                dfs_level<D + 1>(e, c, cells, ios, min_doublets, ref_bits, leaf);
+            }
+            if (nn > 4u) {  // neighbours 5..64 from the pool (rare)
+                const uint32_t ext = e.s.nbr_ext[node.id];
+                for (uint32_t i = 4; i < nn; ++i) {
+                    const DNode c = load_node(e, e.s.nbr_pool[ext + i - 4u]);
+                    if (c.killed) continue;
+                    child = true;
+                    dfs_level<D + 1>(e, c, cells, ios, min_doublets, ref_bits, leaf);
+                }
+            }
+        } else {
+            // a live fifth doublet: the reference's @assert fails
+            const uint32_t ext = (nn > 4u) ? e.s.nbr_ext[node.id] : 0u;
+            for (uint32_t i = 0; i < nn; ++i) {
+                const uint32_t n = (i < 4u) ? comp4(node.nb, i) : e.s.nbr_pool[ext + i - 4u];
+                if (e.s.cell_flags[n] & kKilled) continue;
+                child = true;
+                ref_bits |= PT_REF_DEPTH;
             }
         }
     }
-    if (!child && (uint32_t)(D + 1) >= min_doublets) leaf(stk, (uint32_t)(D + 1));
+    if (!child && (uint32_t)(D + 1) >= min_doublets) leaf(cells, ios, (uint32_t)(D + 1));
 }
 
 template <class F>
-__device__ __forceinline__ void dfs(const EvDesc &e, uint32_t s, uint32_t min_doublets, uint32_t &ref_bits, F &&leaf) {
-    uint32_t stk[4] = {s, 0u, 0u, 0u};
-    dfs_level<0>(e, stk, min_doublets, ref_bits, leaf);
+__device__ __forceinline__ void dfs(const EvDesc &e, const DNode &root, uint32_t min_doublets, uint32_t &ref_bits,
+                                    F &&leaf) {
+    uint32_t cells[4] = {0u, 0u, 0u, 0u}, ios[4] = {0u, 0u, 0u, 0u};
+    dfs_level<0>(e, root, cells, ios, min_doublets, ref_bits, leaf);
 }
 
 // Start candidates (cells with neighbours on a start pair) that the fishbone
-// left alive: count their tuples and tuple hits, one lane per candidate, and
-// sum them per tile of kDfsTile cells.
+// left alive: one lane per candidate runs the DFS once, counts the tuples and
+// tuple hits (summed per tile of kDfsTile cells for the scan), and keeps up to
+// kTupKeep tuples' cells in a per-event buffer; a candidate with more tuples
+// runs the DFS again when emitting.
+constexpr uint32_t kTupKeep = 4;
+
 __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
-    uint32_t ref = 0;
+    uint32_t ref = 0, err = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
     PT_EVENT_LOOP(e, i, e.s.misc[kMiscStart]) {
         const uint32_t c = e.s.start_list[i];
+        const DNode root = load_node(e, c);
         uint64_t v = 0;
-        if (!(e.s.cell_flags[c] & kKilled)) {
+        uint2 tb = make_uint2(~0u, 0u);
+        if (!root.killed) {
             uint32_t nt = 0, nh = 0;
-            dfs(e, c, min_d, ref, [&](const uint32_t *, uint32_t len) {
+            uint4 keep[kTupKeep];
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k) keep[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            dfs(e, root, min_d, ref, [&](const uint32_t *cells, const uint32_t *, uint32_t len) {
+                const uint4 t4 = make_uint4(cells[0], cells[1], len > 2 ? cells[2] : ~0u, len > 3 ? cells[3] : ~0u);
+#pragma unroll
+                for (uint32_t k = 0; k < kTupKeep; ++k)
+                    if (nt == k) keep[k] = t4;
                 ++nt;
                 nh += len + 1;
             });
             v = ((uint64_t)nt << 32) | nh;
+            tb.y = nt;
+            if (nt && nt <= kTupKeep) {
+                const uint32_t off = atomicAdd(&e.s.misc[kMiscTupBuf], nt);
+                if (off + nt <= kTupBuf) {
+                    tb = make_uint2(off, nt);
+#pragma unroll
+                    for (uint32_t k = 0; k < kTupKeep; ++k)
+                        if (k < nt) e.s.tup_buf[off + k] = keep[k];
+                }
+            }
         }
         e.s.tup[c] = v;
+        e.s.tup_where[i] = tb;
         if (v) atomicAdd(reinterpret_cast<unsigned long long *>(&e.s.dfs_agg[c / kDfsTile]), (unsigned long long)v);
     }
     wave_or_atomic(&e.cnt->ref_assert, ref);
+    (void)err;
 }
 
 // Exclusive scan, in cell order, of the start cells' (tuples, hits): block per
@@ -1200,16 +1275,18 @@ __global__ void __launch_bounds__(256) k_dfs_scan(const EvDesc *__restrict__ evs
 // container offsets and hit ids (bulk_fill), the tuple's cells, quality bad,
 // and per cell the tuple count and largest tuple size for the early duplicate
 // remover.  The first 32768 tuples are stored, like the capped container.
+// Tuples kept by the count pass are copied; the others come from a second DFS.
 __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     uint32_t ref = 0, err = 0, ovf = 0, lists = 0;
     const uint32_t min_d = cfg.p.min_hits_per_ntuplet - 1;
     PT_EVENT_LOOP(e, i, e.s.misc[kMiscStart]) {
         const uint32_t c = e.s.start_list[i];
-        if (e.s.cell_flags[c] & kKilled) continue;
+        const uint2 tb = e.s.tup_where[i];  // (kept-tuple offset or ~0, tuples)
+        if (!tb.y) continue;               // killed, or no tuple
         const uint64_t pre = e.s.tup[c];
         uint32_t it = (uint32_t)(pre >> 32);  // 0-based id of this cell's first tuple
         uint32_t hp = (uint32_t)pre;
-        dfs(e, c, min_d, ref, [&](const uint32_t *stk, uint32_t len) {
+        auto emit = [&](const uint32_t *cells, const uint32_t *ios, uint32_t len) {
             ++it;  // 1-based tuple id (bulk_fill)
             if (it <= kMaxTuples) {
                 e.hit_off[it - 1] = hp + 1u;
@@ -1217,10 +1294,9 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
 #pragma unroll
                 for (uint32_t q = 0; q < 4; ++q) {
                     if (q < len) {
-                        const uint32_t cq = stk[q];
-                        const uint32_t io = e.s.cell_io[cq];
-                        e.hit_bins[hp + q] = (uint16_t)(io & 0xFFFFu);
-                        if (q + 1 == len) e.hit_bins[hp + len] = (uint16_t)(io >> 16);
+                        const uint32_t cq = cells[q];
+                        e.hit_bins[hp + q] = (uint16_t)(ios[q] & 0xFFFFu);
+                        if (q + 1 == len) e.hit_bins[hp + len] = (uint16_t)(ios[q] >> 16);
                         const uint32_t before = atomicAdd(&e.s.trk[cq].x, 1u);
                         atomicMax(&e.s.trk[cq].y, len + 1u);
                         if (before == 0u) ++lists;
@@ -1238,11 +1314,34 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
                         if (q == 3) tc.w = cq;
                     }
                 }
-                reinterpret_cast<uint4 *>(e.s.tuple_cells)[it - 1] = tc;
+                reinterpret_cast<PT_G uint4 *>(e.s.tuple_cells)[it - 1] = tc;
                 e.quality[it - 1] = PT_QUALITY_BAD;
             }
             hp += len + 1;
-        });
+        };
+        if (tb.x != ~0u) {
+            // the kept tuples: cells, then their hits, all loads in flight together
+            uint4 t4[kTupKeep];
+            uint32_t io[kTupKeep][4];
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k) t4[k] = (k < tb.y) ? e.s.tup_buf[tb.x + k] : make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k) {
+                const uint32_t cc[4] = {t4[k].x, t4[k].y, t4[k].z, t4[k].w};
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) io[k][q] = (cc[q] != ~0u) ? e.s.cell_io[cc[q]] : 0u;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k) {
+                if (k >= tb.y) break;
+                const uint32_t cc[4] = {t4[k].x, t4[k].y, t4[k].z, t4[k].w};
+                const uint32_t len = (cc[2] == ~0u) ? 2u : (cc[3] == ~0u) ? 3u : 4u;
+                emit(cc, io[k], len);
+            }
+        } else {
+            const DNode root = load_node(e, c);
+            dfs(e, root, min_d, ref, emit);
+        }
     }
     wave_or_atomic(&e.cnt->overflow, ovf);
     wave_or_atomic(&e.cnt->internal_error, err);

@@ -84,6 +84,8 @@ size_t carve(char *base, pt::Scratch *s) {
     t.start_bits = (uint64_t *)take((size_t)(kMaxCells / 64 + 1) * 8);
     t.tup = (uint64_t *)take((size_t)kMaxCells * 8);
     t.dfs_agg = (uint64_t *)take((size_t)kDfsAgg * 8);
+    t.tup_where = (uint2 *)take((size_t)kMaxCells * 8);
+    t.tup_buf = (uint4 *)take((size_t)kTupBuf * 16);
     t.ohc_cursor = (uint32_t *)take(kMaxHits * 4);
     t.ohc_list = (uint2 *)take((size_t)kMaxHits * kOhcStride * 8);
     t.ohc_ovf = (uint32_t *)take(kOhcOvfCap * 4);

@@ -34,6 +34,7 @@ constexpr uint32_t kSlotTile = 1024;              // slots per prefix-sum tile o
 constexpr uint32_t kDblAgg = (kMaxSlots + kSlotTile - 1) / kSlotTile;
 constexpr uint32_t kDfsTile = 1024;               // cells per prefix-sum tile of the ntuplet scan
 constexpr uint32_t kDfsAgg = (kMaxCells + kDfsTile - 1) / kDfsTile;
+constexpr uint32_t kTupBuf = 2u * PT_MAX_TUPLES;   // tuples kept between the ntuplet count and emit
 
 // Layer pairs (gpu_pixel_doublets.jl:20-28) and per-pair cuts (:29-56)
 struct PairTables {
@@ -66,6 +67,8 @@ struct Scratch {
     PT_G uint64_t *start_bits;   // [kMaxCells/64] the same as a bit set
     PT_G uint64_t *tup;          // [kMaxCells]    start cell: (tuples << 32 | hits), then the prefix
     PT_G uint64_t *dfs_agg;      // [kDfsAgg]      the same summed per tile of kDfsTile cells
+    PT_G uint2 *tup_where;       // [kMaxCells]    per start-list entry: (buffer offset, tuples) or ~0
+    PT_G uint4 *tup_buf;         // [kTupBuf]      cells of the tuples kept by the count pass
     PT_G uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
     PT_G uint2 *ohc_list;        // [kMaxHits*kOhcStride] (cell, inner hit) 0-based, unordered unless
                             //                the hit overflowed (then the first 128 cells, ascending)
@@ -99,6 +102,7 @@ enum Misc : int {
     kMiscTkFill = 9,      // ... of the doublet fill
     kMiscTkConn = 10,     // ... of connect
     kMiscTkFish = 11,     // ... of the fishbone
+    kMiscTupBuf = 12,     // entries of tup_buf
 };
 
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,
