@@ -211,26 +211,31 @@ __device__ __forceinline__ int pair_of_slot(const EvDesc &e, uint32_t j) {
     return p;
 }
 
-// returns the number of candidates (0 when the inner hit fails its cuts)
-__device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cfg, uint32_t j, uint32_t &ref_bits,
-                                               SlotCtx &s) {
+// Inner-slot setup (gpu_pixel_doublets.jl:262-316) in three steps around its
+// two memory reads, so that the reads of several slots can be in flight
+// together: the inner-hit record gives the cuts and the phi window, the
+// histogram offsets of the window give the candidate ranges.
+struct SlotPrep {
+    uint32_t hoff, kl, kh;  // window bins (kl == kh: none); hoff = 128 * outer layer
+    bool ok;
+};
+__device__ __forceinline__ SlotPrep slot_prep(const EvDesc &e, const Config &cfg, int p, uint4 q, uint32_t &ref_bits,
+                                              SlotCtx &s) {
     const pt_params &P = cfg.p;
-    const int p = pair_of_slot(e, j);
     const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
-    const uint32_t i0 = ((p == 0) ? j : j - e.slot_cum[p - 1]) + e.layer_start[inner];
-    const uint4 q = e.s.hdbl[i0];
+    SlotPrep r{0u, 0u, 0u, false};
     const int16_t mi = (int16_t)(q.z >> 16);
     s.pk = (uint32_t)p << 16;
-    if (mi > 2000) return 0;
+    if (mi > 2000) return r;
     s.me_z = __uint_as_float(q.x);
-    if (s.me_z < cfg.pairs.min_z[p] || s.me_z > cfg.pairs.max_z[p]) return 0;
+    if (s.me_z < cfg.pairs.min_z[p] || s.me_z > cfg.pairs.max_z[p]) return r;
     int me_s = -1;
     if (P.do_cluster_cut) {
         if (inner == 0 && !(mi < 96)) ref_bits |= PT_REF_BAD_MODULE;
         const bool is_outer_ladder = P.ideal_conditions ? true : (((mi / 8) % 2) == 0);
         me_s = (inner > 0 || is_outer_ladder) ? (int)(int16_t)q.w : -1;
-        if (inner == 0 && outer > 3 && me_s > 0 && me_s < 36) return 0;
-        if (inner == 1 && outer > 3 && me_s > 0 && me_s < 28) return 0;
+        if (inner == 0 && outer > 3 && me_s > 0 && me_s < 36) return r;
+        if (inner == 1 && outer > 3 && me_s > 0 && me_s < 28) return r;
     }
     s.me_s = me_s;
     const int16_t me_p = (int16_t)(q.z & 0xFFFFu);
@@ -240,31 +245,62 @@ __device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cf
     const int kl = hist_bin((int16_t)(me_p - cut));
     int kh = hist_bin((int16_t)(me_p + cut)) + 1;
     if (kh == 129) kh = 1;
-    if (kl == kh) return 0;  // while (current_bin != kh) visits no bin
-    const int h_off = 128 * outer;
-    auto off = [&](int k) {  // Julia off[k + h_off], clamped to the hit count
-        uint32_t v = e.hist_off[k + h_off - 1];
+    if (kl == kh) return r;  // while (current_bin != kh) visits no bin
+    r.hoff = 128u * (uint32_t)outer;
+    r.kl = (uint32_t)kl;
+    r.kh = (uint32_t)kh;
+    r.ok = true;
+    return r;
+}
+// histogram offsets of the window (Julia off[k + h_off]): kl..kh-1 or, when it
+// wraps, kl..128 and 1..kh-1
+__device__ __forceinline__ void slot_offsets(const EvDesc &e, const SlotPrep &r, uint32_t (&o)[4]) {
+    const uint32_t ho = r.hoff;
+    o[0] = e.hist_off[r.kl + ho - 1u];
+    if (r.kl < r.kh) {
+        o[1] = e.hist_off[r.kh + ho - 1u];
+        o[2] = o[3] = 0u;
+    } else {
+        o[1] = e.hist_off[128u + ho];
+        o[2] = e.hist_off[ho];
+        o[3] = e.hist_off[r.kh + ho - 1u];
+    }
+}
+// candidate ranges from the offsets, clamped to the hit count; returns the
+// number of candidates
+__device__ __forceinline__ uint32_t slot_window(const EvDesc &e, const SlotPrep &r, const uint32_t (&o)[4],
+                                                uint32_t &ref_bits, SlotCtx &s) {
+    auto clampo = [&](uint32_t v) {
         if (v > e.n_hits) {
             ref_bits |= PT_REF_HIST_LAYER;
             v = e.n_hits;
         }
         return v;
     };
-    uint32_t nb;
-    s.a0 = off(kl);
-    if (kl < kh) {
-        const uint32_t a1 = off(kh);
+    s.a0 = clampo(o[0]);
+    if (r.kl < r.kh) {
+        const uint32_t a1 = clampo(o[1]);
         s.na = a1 > s.a0 ? a1 - s.a0 : 0u;
         s.b0 = 0;
-        nb = 0;
-    } else {  // bins kl..128 then 1..kh-1
-        const uint32_t a1 = off(129);
-        s.na = a1 > s.a0 ? a1 - s.a0 : 0u;
-        s.b0 = off(1);
-        const uint32_t b1 = off(kh);
-        nb = b1 > s.b0 ? b1 - s.b0 : 0u;
+        return s.na;
     }
-    return s.na + nb;
+    const uint32_t a1 = clampo(o[1]);  // off(129)
+    s.na = a1 > s.a0 ? a1 - s.a0 : 0u;
+    s.b0 = clampo(o[2]);
+    const uint32_t b1 = clampo(o[3]);
+    return s.na + (b1 > s.b0 ? b1 - s.b0 : 0u);
+}
+
+// returns the number of candidates (0 when the inner hit fails its cuts)
+__device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cfg, uint32_t j, uint32_t &ref_bits,
+                                               SlotCtx &s) {
+    const int p = pair_of_slot(e, j);
+    const uint32_t i0 = ((p == 0) ? j : j - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]];
+    const SlotPrep r = slot_prep(e, cfg, p, e.s.hdbl[i0], ref_bits, s);
+    if (!r.ok) return 0;
+    uint32_t o[4];
+    slot_offsets(e, r, o);
+    return slot_window(e, r, o, ref_bits, s);
 }
 
 __device__ __forceinline__ uint32_t cand_pos(uint32_t a0, uint32_t na, uint32_t b0, uint32_t k) {
@@ -382,20 +418,34 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
 
 // ------------------------------------------------------------------ K1: doublet count
 // A wavefront takes a tile of 64 consecutive inner slots.  Their candidates
-// (about 16 per slot) are spread over all 64 lanes, kEvalU rounds of loads in
-// flight at a time; the pass bits come back to the slot's lane through the
-// ballot: the slot's count and, for its first 64 candidates, a bit mask.  Per
-// tile of kSlotTile slots the counts are also summed.
+// (about 16 per slot, lane-contiguous in slot order) are spread over all 64
+// lanes, kEvalU rounds of 64 loads in flight at a time.  Each item finds its
+// slot through an owner map in LDS (a slot's lane at its first item, carried
+// forward by a running maximum), and each round's pass ballot is stored as a
+// 64-bit word; afterwards every lane reads its own slot's bits back: the
+// count and, for the first 64 candidates, the bit mask.  Windows of
+// kCountWin items bound the map.  Per tile of kSlotTile slots the counts are
+// also summed.
 constexpr int kEvalU = 4;
+constexpr uint32_t kCountWin = 1024;
+static_assert(kCountWin % (64u * kEvalU) == 0 && kCountWin == 64u * 16u, "window = 16 words per lane");
+
+struct alignas(16) SlotItem {
+    SlotCtx s;
+    uint32_t excl;  // the slot's first item in the tile
+};
 
 __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
-    __shared__ SlotCtx ctx[4][64];
+    __shared__ SlotItem ctx[4][64];
+    __shared__ uint4 omap[4][kCountWin / 4];
+    __shared__ uint64_t pbits[4][kCountWin / 64 + 1];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t n_slots = e.n_slots;
     const uint32_t n_tiles = (n_slots + 63u) / 64u;
+    const uint32_t *own = reinterpret_cast<const uint32_t *>(&omap[w][0]);
     uint32_t ref = 0;
     for (;;) {
         const uint32_t t = take_ticket(&e.s.misc[kMiscTkCount]);
@@ -404,34 +454,79 @@ __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict_
         wave_sync();  // the previous tile's reads of ctx are done
         SlotCtx s = {};
         const uint32_t n = (j < n_slots) ? slot_setup(e, cfg, j, ref, s) : 0u;
-        ctx[w][lane] = s;
-        wave_sync();
         const uint32_t incl = wave_incl_scan(n);
         const uint32_t excl = incl - n;
         const uint32_t total = __shfl(incl, 63, 64);
+        ctx[w][lane] = SlotItem{s, excl};
         uint32_t cnt = 0;
         uint64_t msk = 0;
-        for (uint32_t base = 0; base < total; base += 64u * kEvalU) {
-            SlotCtx c[kEvalU];
-            uint4 q[kEvalU];
-            uint32_t g[kEvalU];
-            int ow[kEvalU];
+        for (uint32_t w0 = 0; w0 < total; w0 += kCountWin) {
+            const uint32_t wn = min(total - w0, kCountWin);
+            // owner map of items [w0, w0 + wn): lane + 1 where a slot's items start
+            // (or continue from the previous window), zero elsewhere; then the
+            // running maximum gives every item its slot's lane + 1
 #pragma unroll
-            for (int u = 0; u < kEvalU; ++u) g[u] = base + 64u * u + lane;
-            owners_of(incl, g, ow);
+            for (int v = 0; v < 4; ++v) omap[w][4u * lane + v] = make_uint4(0u, 0u, 0u, 0u);
+            wave_sync();
+            if (n > 0 && incl > w0 && excl < w0 + wn)
+                reinterpret_cast<uint32_t *>(&omap[w][0])[excl > w0 ? excl - w0 : 0u] = lane + 1u;
+            wave_sync();
+            uint32_t m[16];
 #pragma unroll
-            for (int u = 0; u < kEvalU; ++u) {
-                const int o = ow[u];
-                const uint32_t x = (uint32_t)__shfl(excl, o, 64);
-                q[u] = make_uint4(0u, 0u, 0u, 0u);
-                c[u] = ctx[w][o];
-                if (g[u] < total) q[u] = e.s.binrec[cand_pos(c[u], g[u] - x)];
+            for (int v = 0; v < 4; ++v) {
+                const uint4 x = omap[w][4u * lane + v];
+                m[4 * v] = x.x;
+                m[4 * v + 1] = x.y;
+                m[4 * v + 2] = x.z;
+                m[4 * v + 3] = x.w;
             }
 #pragma unroll
-            for (int u = 0; u < kEvalU; ++u) {
-                const bool ok = g[u] < total && cand_pass(e, cfg, c[u], q[u], ref);
-                fold_round(__ballot(ok), excl, incl, base + 64u * u, cnt, msk);
+            for (int i = 1; i < 16; ++i) m[i] = max(m[i], m[i - 1]);
+            uint32_t run = m[15];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up(run, d, 64);
+                if (lane >= (uint32_t)d) run = max(run, y);
             }
+            uint32_t carry = (uint32_t)__shfl_up(run, 1, 64);
+            if (lane == 0) carry = 0u;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                omap[w][4u * lane + v] = make_uint4(max(m[4 * v], carry), max(m[4 * v + 1], carry),
+                                                    max(m[4 * v + 2], carry), max(m[4 * v + 3], carry));
+            wave_sync();
+            for (uint32_t base = 0; base < wn; base += 64u * kEvalU) {
+                SlotItem c[kEvalU];
+                uint4 q[kEvalU];
+#pragma unroll
+                for (int u = 0; u < kEvalU; ++u) {
+                    const uint32_t g = base + 64u * u + lane;
+                    const bool act = g < wn;
+                    const uint32_t o = act ? own[g] - 1u : 0u;
+                    c[u] = ctx[w][o];
+                    q[u] = make_uint4(0u, 0u, 0u, 0u);
+                    if (act) q[u] = e.s.binrec[cand_pos(c[u].s, w0 + g - c[u].excl)];
+                }
+#pragma unroll
+                for (int u = 0; u < kEvalU; ++u) {
+                    const bool ok = base + 64u * u + lane < wn && cand_pass(e, cfg, c[u].s, q[u], ref);
+                    const uint64_t bal = __ballot(ok);
+                    if (lane == 0) pbits[w][(base >> 6) + u] = bal;
+                }
+            }
+            wave_sync();
+            // this lane's items in the window
+            const uint32_t s0 = max(excl, w0), s1 = min(incl, w0 + wn);
+            for (uint32_t x = s0; x < s1; x += 64u) {
+                const uint32_t r = x - w0, sh = r & 63u, len = min(64u, s1 - x);
+                const uint64_t lo = pbits[w][r >> 6], hi = pbits[w][(r >> 6) + 1u];
+                uint64_t bits = sh ? ((lo >> sh) | (hi << (64u - sh))) : lo;
+                if (len < 64u) bits &= (1ull << len) - 1ull;
+                cnt += (uint32_t)__popcll(bits);
+                const uint32_t k0 = x - excl;
+                if (k0 < 64u) msk |= bits << k0;
+            }
+            wave_sync();  // reads of omap and pbits done before the next window
         }
         if (j < n_slots) {
             e.s.slot_cnt[j] = cnt;
