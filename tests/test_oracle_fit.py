@@ -1,0 +1,346 @@
+"""Pins of the oracle's broken-line fit (CPU only).
+
+The reference's own fit test (testBrokenLine.jl) prints and asserts nothing,
+so its inputs are used as known-answer vectors and every intermediate the
+oracle produces for them is re-derived independently:
+  * fast fit: the circle through the three hits, solved as a 2x2 linear
+    system, and the turning angle from numpy's arctan2;
+  * prepared data: arc lengths as radius x angle from the circle point nearest
+    the origin, (S, Z) a rotation of (s, z) that levels the track;
+  * band systems: the broken-line cost (weighted residuals plus squared slope
+    changes over the scattering variances) written from its definition; its
+    Hessian / 2 must equal the oracle's band matrix, the oracle's inverse and
+    solution are checked against a 50-digit mpmath solve, and the fitted
+    chi2 must equal the cost at the solution;
+  * a noiseless helix generated here from its parameters is fitted back to
+    those parameters (this pins the Karimaki translations, the signs and the
+    output conventions);
+  * smeared high-pT tracks have pulls of unit width and chi2/ndf near 1;
+  * the forward-mode gradient used for the adjoint checks agrees with
+    central finite differences of the oracle's own loss.
+The vectors are committed in tests/golden/bl_kat.json (tests/golden/make_bl_kat.py).
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import mpmath
+import numpy as np
+import pytest
+
+import oracle_layout as L
+import oracle_lib as O
+from pixeltrack import _abi as A
+from pixeltrack import generator as G
+
+KAT = json.loads((Path(__file__).resolve().parent / "golden" / "bl_kat.json").read_text())
+B_FIT = float(np.float32(A.B_FIELD_INV_GEV))
+B_TESLA_MATCHED = A.B_FIELD_INV_GEV * 100.0 / 0.299792458  # solenoid field whose pT/R equals the fitter's B
+
+
+def kat_cases():
+    return sorted(KAT["cases"])
+
+
+def kat_detail(name):
+    c = KAT["cases"][name]
+    return c, L.detail(np.array(c["hits"]), np.array(c["ge"], np.float32), KAT["B"])
+
+
+# ---------------------------------------------------------------- the broken-line cost, from its definition
+def kink_rows(pos):
+    """Slope change at each interior point i as a linear form in u:
+    (u[i+1]-u[i])/(pos[i+1]-pos[i]) - (u[i]-u[i-1])/(pos[i]-pos[i-1])."""
+    N = len(pos)
+    rows = []
+    for i in range(1, N - 1):
+        a = np.zeros(N)
+        d1, d2 = pos[i] - pos[i - 1], pos[i + 1] - pos[i]
+        a[i - 1] = 1.0 / d1
+        a[i] = -(1.0 / d1 + 1.0 / d2)
+        a[i + 1] = 1.0 / d2
+        rows.append((i, a))
+    return rows
+
+
+def line_cost_terms(d):
+    """Quadratic form of the line cost chi2(u) = sum w (Z-u)^2 + sum kink^2 / VB."""
+    N = len(d["S"])
+    H = np.diag(d["line_w"])
+    for i, a in kink_rows(d["S"]):
+        H = H + np.outer(a, a) / d["VB"][i]
+    rhs = d["line_w"] * d["Z"]
+
+    def cost(u):
+        c = float(np.sum(d["line_w"] * (d["Z"] - u) ** 2))
+        for i, a in kink_rows(d["S"]):
+            c += float(a @ u) ** 2 / d["VB"][i]
+        return c
+
+    return H, rhs, cost, N
+
+
+def circle_cost_terms(d):
+    """Circle cost over (u, dkappa): sum w (Z-u)^2 + sum (kink + (s[i+1]-s[i-1]) dkappa / 2)^2 / VB."""
+    s = d["s"]
+    N = len(s)
+    H = np.zeros((N + 1, N + 1))
+    H[:N, :N] = np.diag(d["circ_w"])
+    rows = []
+    for i, a in kink_rows(s):
+        ae = np.append(a, 0.5 * (s[i + 1] - s[i - 1]))
+        rows.append((i, ae))
+        H = H + np.outer(ae, ae) / d["circ_VB"][i]
+    rhs = np.append(d["circ_w"] * d["circ_Z"], 0.0)
+
+    def cost(u):
+        c = float(np.sum(d["circ_w"] * (d["circ_Z"] - u[:N]) ** 2))
+        for i, ae in rows:
+            c += float(ae @ u) ** 2 / d["circ_VB"][i]
+        return c
+
+    return H, rhs, cost, N
+
+
+def mp_solve(H, rhs):
+    mpmath.mp.dps = 50
+    M = mpmath.matrix(H.tolist())
+    u = mpmath.lu_solve(M, mpmath.matrix(rhs.tolist()))
+    Minv = M ** -1
+    return (np.array([float(x) for x in u]),
+            np.array([[float(Minv[i, j]) for j in range(H.shape[1])] for i in range(H.shape[0])]))
+
+
+def assert_close_scaled(a, b, rtol, what):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    err = np.max(np.abs(a - b)) / scale
+    assert err <= rtol, f"{what}: max |a-b| / max|b| = {err:.3e} > {rtol:.0e}"
+
+
+# ---------------------------------------------------------------- known-answer vectors
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_fixture_reproduced(name):
+    """The oracle still produces the committed vectors (regression)."""
+    c, d = kat_detail(name)
+    for k, v in c["expected"].items():
+        np.testing.assert_allclose(np.asarray(d[k], np.float64), np.asarray(v, np.float64), rtol=1e-13, atol=0,
+                                   err_msg=k)
+
+
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_fast_fit_is_the_circle_through_three_hits(name):
+    c, d = kat_detail(name)
+    h = np.array(c["hits"])
+    N = h.shape[0]
+    p0, pm, pn = h[0, :2], h[N // 2, :2], h[N - 1, :2]
+    A2 = 2.0 * np.array([pm - p0, pn - p0])
+    rhs = np.array([pm @ pm - p0 @ p0, pn @ pn - p0 @ p0])
+    ctr = np.linalg.solve(A2, rhs)
+    R = np.linalg.norm(p0 - ctr)
+    for p in (pm, pn):
+        assert abs(np.linalg.norm(p - ctr) - R) < 1e-9 * R
+    ff = d["fast"]
+    np.testing.assert_allclose(ff[:2], ctr, rtol=1e-10)
+    np.testing.assert_allclose(ff[2], R, rtol=1e-10)
+    u, v = p0 - ctr, pn - ctr
+    turn = np.arctan2(u[0] * v[1] - u[1] * v[0], u @ v)
+    np.testing.assert_allclose(ff[3], R * turn / (h[N - 1, 2] - h[0, 2]), rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_prepared_data_geometry(name):
+    c, d = kat_detail(name)
+    h = np.array(c["hits"])
+    N = h.shape[0]
+    ctr, R = d["fast"][:2], d["fast"][2]
+    rad = np.stack([d["radx"], d["rady"]], 1)
+    np.testing.assert_allclose(rad, h[:, :2] - ctr, rtol=0, atol=1e-12)
+    # charge: +1 turns clockwise (seen from +z), -1 counter-clockwise
+    t0, t1 = h[1, :2] - h[0, :2], h[N - 1, :2] - h[N - 2, :2]
+    assert d["q"] == (-1 if t0[0] * t1[1] - t0[1] * t1[0] > 0 else 1)
+    # arc length from the circle point nearest the origin: |s| = R * angle
+    e = -ctr / np.linalg.norm(ctr) * R
+    ang = np.arccos(np.clip(rad @ e / (np.linalg.norm(rad, axis=1) * R), -1, 1))
+    np.testing.assert_allclose(np.abs(d["s"]), R * ang, rtol=1e-9, atol=1e-12)
+    assert np.all(np.diff(d["s"]) > 0)
+    # (S, Z) is a rotation of (s, z) that makes the track nearly horizontal
+    np.testing.assert_allclose(d["S"] ** 2 + d["Z"] ** 2, d["s"] ** 2 + h[:, 2] ** 2, rtol=1e-12)
+    assert np.ptp(d["Z"]) < 1e-2 * np.ptp(d["S"])
+    assert d["VB"][0] == 0 and d["VB"][-1] == 0 and np.all(d["VB"][1:-1] > 0)
+
+
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_line_system_from_the_cost(name):
+    _, d = kat_detail(name)
+    H, rhs, cost, N = line_cost_terms(d)
+    assert_close_scaled(d["line_C"], H, 1e-12, "line band matrix vs Hessian/2 of the cost")
+    u_mp, inv_mp = mp_solve(H, rhs)
+    assert_close_scaled(d["line_I"], inv_mp, 1e-9, "line inverse vs mpmath")
+    assert_close_scaled(d["line_u"], u_mp, 1e-9, "line solution vs mpmath")
+    np.testing.assert_allclose(d["line_chi2"], cost(u_mp), rtol=1e-8)
+
+
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_circle_system_from_the_cost(name):
+    _, d = kat_detail(name)
+    H, rhs, cost, N = circle_cost_terms(d)
+    np.testing.assert_allclose(d["circ_VB"], d["VB"] * (1.0 + (d["q"] / d["fast"][3]) ** 2), rtol=1e-14)
+    np.testing.assert_allclose(d["circ_Z"], np.hypot(d["radx"], d["rady"]) - d["fast"][2], rtol=0,
+                               atol=1e-14 * d["fast"][2])
+    assert_close_scaled(d["circ_C"], H, 1e-12, "circle band matrix vs Hessian/2 of the cost")
+    u_mp, inv_mp = mp_solve(H, rhs)
+    assert_close_scaled(d["circ_I"], inv_mp, 1e-9, "circle inverse vs mpmath")
+    assert_close_scaled(d["circ_u"], u_mp, 1e-9, "circle solution vs mpmath")
+    np.testing.assert_allclose(d["circle_chi2"], cost(u_mp), rtol=1e-8, atol=1e-20)  # 3 hits: 0 dof, chi2 ~ 0
+
+
+@pytest.mark.parametrize("name", kat_cases())
+def test_kat_outputs_consistent(name):
+    c, d = kat_detail(name)
+    N = c["N"]
+    st = d["state"]
+    np.testing.assert_allclose(d["pt"], KAT["B"] / abs(st[2] * KAT["B"]), rtol=1e-14)
+    np.testing.assert_allclose(d["eta"], np.arcsinh(st[3]), rtol=1e-14)
+    np.testing.assert_allclose(d["chi2"], (d["line_chi2"] + d["circle_chi2"]) / (2 * N - 5), rtol=1e-14)
+    cov = d["cov"]
+    assert cov[0] > 0 and cov[5] > 0 and cov[9] > 0 and cov[12] > 0 and cov[14] > 0
+    assert d["flags"] == 0
+
+
+def test_spd_inverse_matches_numpy():
+    """invert33/44/55 (cholesky_inversion.jl) through the band systems of many tracks."""
+    rng = np.random.default_rng(5)
+    tr = G.fit_tracks(64, seed=3, b_tesla=B_TESLA_MATCHED)
+    for t in range(64):
+        N = 3 if tr["nhit"][t] == 3 else 4
+        h = tr["hits"][:, :N, t].T.astype(np.float64)
+        g = tr["ge"][:, :N, t].T
+        d = L.detail(h, g, B_FIT)
+        for M, I in ((d["line_C"], d["line_I"]), (d["circ_C"], d["circ_I"])):
+            assert_close_scaled(I, np.linalg.inv(M), 1e-8, "inverse")
+    del rng
+
+
+# ---------------------------------------------------------------- closed-form helix
+def helix_hits(pt, phi0, q, cot, z0, B, radii):
+    """Hits of a prompt helix through the origin at transverse radii `radii`.
+    R = pT / B; charge +1 turns clockwise seen from +z."""
+    R = pt / B
+    ctr = q * R * np.array([np.sin(phi0), -np.cos(phi0)])
+    u0 = -ctr / R
+    out = []
+    for r in radii:
+        alpha = 2.0 * np.arcsin(r / (2.0 * R))
+        a = -q * alpha
+        rot = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]])
+        p = ctr + R * (rot @ u0)
+        out.append([p[0], p[1], z0 + R * alpha * cot])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("pt,phi0,q,cot,z0", [(1.0, 0.3, 1, 0.4, 1.5), (3.0, -2.0, -1, -1.1, -4.0),
+                                              (12.0, 2.9, 1, 0.05, 0.2), (0.8, 1.2, -1, 2.0, 7.0)])
+@pytest.mark.parametrize("N", [3, 4])
+def test_noiseless_helix_is_recovered(pt, phi0, q, cot, z0, N):
+    radii = [2.9, 6.8, 10.9, 16.0][:N]
+    h = helix_hits(pt, phi0, q, cot, z0, B_FIT, radii)
+    ge = np.tile(np.array([1e-6, 0.0, 1e-6, 0.0, 0.0, 4e-6], np.float32), (N, 1))
+    d = L.detail(h, ge, B_FIT)
+    st = d["state"]
+    dphi = (st[0] - phi0 + np.pi) % (2 * np.pi) - np.pi
+    assert abs(dphi) < 1e-9
+    assert abs(st[1]) < 1e-9
+    np.testing.assert_allclose(st[2], q / pt, rtol=1e-9)
+    np.testing.assert_allclose(st[3], cot, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(st[4], z0, rtol=1e-9, atol=1e-9)
+    assert d["line_chi2"] < 1e-12 and d["circle_chi2"] < 1e-12
+    np.testing.assert_allclose(d["pt"], pt, rtol=1e-9)
+
+
+# ---------------------------------------------------------------- statistics of smeared tracks
+def test_pulls_and_chi2_of_smeared_high_pt_tracks():
+    """The generator smears in the module frames with the covariance it
+    reports; above 10 GeV the scattering term the fit adds is small, so the
+    pulls of the five parameters have about unit width and chi2/ndf averages
+    about 1.  Robust widths (1.4826 x median absolute deviation) over the
+    tracks whose fitted charge is right: the fit takes the charge from the
+    turning sense of the first and last segments, which noise can flip for a
+    stiff track (the method's own behaviour, 0.04% here).  The reference
+    method's phi error comes out about 13% small, so the band is 0.85-1.2."""
+    n = 20000
+    tr = G.fit_tracks(n, seed=17, len_mix=(0.0, 1.0, 0.0), pt_min=10.0, pt_max=60.0, eta_max=0.8,
+                      b_tesla=B_TESLA_MATCHED)
+    out, flags = O.fit_bl(tr["hits"], tr["ge"], tr["nhit"], n_threads=8)
+    truth = tr["truth"].astype(np.float64)
+    ok = (flags == 0) & (np.sign(out[:, 2]) == np.sign(truth[2]))
+    assert ok.mean() > 0.998
+    diag = {0: 0, 1: 5, 2: 9, 3: 12, 4: 14}
+    for k, ci in diag.items():
+        res = out[ok, k] - truth[k, ok]
+        if k == 0:
+            res = (res + np.pi) % (2 * np.pi) - np.pi
+        pull = res / np.sqrt(out[ok, 5 + ci])
+        width = 1.4826 * np.median(np.abs(pull - np.median(pull)))
+        assert 0.85 < width < 1.2, (k, width)
+        assert abs(np.median(pull)) < 0.05, (k, np.median(pull))
+    assert 0.85 < np.mean(out[ok, 22]) < 1.1, np.mean(out[ok, 22])
+
+
+# ---------------------------------------------------------------- gradient used by the adjoint checks
+def loss_and_grad(N, xyz, sig, rot, truth=None):
+    val = np.zeros(1)
+    grad = np.zeros(5 * N)
+    tr = None if truth is None else np.ascontiguousarray(truth, np.float64)
+    rc = O.lib().oracle_fit_grad(N, np.ascontiguousarray(xyz, np.float64).ctypes.data,
+                                 np.ascontiguousarray(sig, np.float64).ctypes.data,
+                                 np.ascontiguousarray(rot, np.float32).ctypes.data, C.c_double(B_FIT),
+                                 tr.ctypes.data if tr is not None else None, val.ctypes.data, grad.ctypes.data)
+    assert rc == 0
+    return val[0], grad
+
+
+@pytest.mark.parametrize("with_truth", [False, True])
+def test_forward_mode_gradient_matches_finite_differences(frames, with_truth):
+    tr = G.fit_tracks(12, seed=11, len_mix=(0.3, 0.7, 0.0), pt_min=0.5, pt_max=50.0, b_tesla=B_TESLA_MATCHED)
+    for t in range(12):
+        N = int(tr["nhit"][t])
+        xyz = tr["hits"][:, :N, t].T.astype(np.float64).copy()
+        sig = tr["sig"][:, :N, t].T.astype(np.float64).copy()
+        rot = frames[tr["det"][:N, t]]
+        truth = tr["truth"][:, t].astype(np.float64) if with_truth else None
+        L0, g = loss_and_grad(N, xyz, sig, rot, truth)
+        fd = np.zeros(5 * N)
+        for k in range(N):
+            for c in range(5):
+                if c < 3:
+                    h = 1e-6
+                    xp, xm = xyz.copy(), xyz.copy()
+                    xp[k, c] += h
+                    xm[k, c] -= h
+                    fd[5 * k + c] = (loss_and_grad(N, xp, sig, rot, truth)[0] -
+                                     loss_and_grad(N, xm, sig, rot, truth)[0]) / (2 * h)
+                else:
+                    h = 1e-5 * sig[k, c - 3]
+                    sp, sm = sig.copy(), sig.copy()
+                    sp[k, c - 3] += h
+                    sm[k, c - 3] -= h
+                    fd[5 * k + c] = (loss_and_grad(N, xyz, sp, rot, truth)[0] -
+                                     loss_and_grad(N, xyz, sm, rot, truth)[0]) / (2 * h)
+        scale = np.max(np.abs(fd))
+        assert np.max(np.abs(g - fd)) <= 2e-5 * scale, (t, N, np.max(np.abs(g - fd)) / scale)
+
+
+def test_op_counts_are_stable():
+    """The flop model of the fit roofline: counted with an op-counting scalar."""
+    counts = {}
+    for N in (3, 4):
+        c = KAT["cases"]["testFit4" if N == 4 else "testFit3"]
+        h = np.ascontiguousarray(np.array(c["hits"]), np.float64)
+        g = np.ascontiguousarray(np.array(c["ge"], np.float32))
+        cnt = np.zeros(6, np.uint64)
+        assert O.lib().oracle_fit_opcount(N, h.ctypes.data, g.ctypes.data, C.c_double(KAT["B"]), cnt.ctypes.data) == 0
+        counts[N] = cnt.tolist()
+    # (add/sub, mul, div, sqrt, transcendental, compare)
+    assert sum(counts[4][:4]) > sum(counts[3][:4]) > 300
+    assert counts[3][4] > 0 and counts[4][4] > counts[3][4] - 1
