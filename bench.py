@@ -108,6 +108,91 @@ def run_cpu_baseline(events, frames, budget_s=12.0, max_threads=16):
                        f"{per_event * 1e3:.1f} ms/event single-thread")
 
 
+def fit_only(ctx, n_tracks=1 << 20, steps=20):
+    """Fit-only workload: 1,048,576 pre-gathered tracks (seed 7, lengths
+    3/4/5 in a 30/55/15% mix in random order, pT log-uniform 0.5-10 GeV,
+    per-layer sigma_rphi 8-15 um and sigma_z 15-40 um), resident in HBM.
+    tracks/s of the FP64 and mixed fits under each schedule (persistent
+    after the autotuner has chosen its launch shape)."""
+    from pixeltrack import _abi as A, ca, generator as G
+    tr = G.fit_tracks(n_tracks, seed=7)
+    fb = ca.FitBatch(ctx, tr["hits"], tr["ge"], tr["nhit"])
+    out = {"workload": f"{n_tracks} tracks, lengths 3/4/5 = 30/55/15% (seed 7), pT 0.5-10 GeV",
+           "tracks": n_tracks}
+    try:
+        for prec, pname in ((A.FIT_FP64, "fp64"), (A.FIT_MIXED, "mixed")):
+            tuned, tuned_ms = fb.autotune(prec)
+            out[f"{pname}_tuning"] = dict(tuned, ms=round(tuned_ms, 4))
+            for sched, sname in ((A.SCHED_PLAIN, "plain"), (A.SCHED_PER_LENGTH, "per_length"),
+                                 (A.SCHED_PERSISTENT, "persistent")):
+                fb.run_async(prec, sched)
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    fb.run_async(prec, sched)
+                ctx.sync()
+                dt = (time.perf_counter() - t0) / steps
+                out[f"{pname}_{sname}_tracks_per_s"] = round(n_tracks / dt, 1)
+                out[f"{pname}_{sname}_ms"] = round(dt * 1e3, 4)
+        out["mixed_vs_fp64_persistent"] = round(out["mixed_persistent_tracks_per_s"] /
+                                                out["fp64_persistent_tracks_per_s"], 3)
+        out["persistent_vs_per_length_mixed"] = round(out["mixed_persistent_tracks_per_s"] /
+                                                      out["mixed_per_length_tracks_per_s"], 3)
+    finally:
+        fb.free()
+    return out
+
+
+def adjoint_only(ctx, n_tracks=262144, steps=20):
+    """Adjoint workload: 262,144 quadruplets (seed 11, pT log-uniform 0.5-50
+    GeV) resident in HBM.  tracks/s of the plain FP64 fit against the mixed
+    fit + adjoint of L = chi2_line + chi2_circle + |params - truth|^2 with
+    respect to (x, y, z, sigma_rphi, sigma_z) of every hit: fused (one kernel
+    recomputes the forward and runs the reverse sweep) and as an rrule
+    (forward kernel, then the pullback kernel, which recomputes the forward)."""
+    from pixeltrack import _abi as A, ca, generator as G
+    tr = G.fit_tracks(n_tracks, seed=11, len_mix=(0.0, 1.0, 0.0), pt_min=0.5, pt_max=50.0)
+    fb = ca.FitBatch(ctx, tr["hits"], tr["ge"], tr["nhit"])
+    ab = ca.AdjointBatch(ctx, tr["hits"], tr["sig"], tr["det"], tr["nhit"], tr["truth"])
+    out = {"workload": f"{n_tracks} quadruplets (seed 11), pT 0.5-50 GeV", "tracks": n_tracks}
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) / steps
+
+    try:
+        for prec in (A.FIT_FP64, A.FIT_MIXED):
+            fb.autotune(prec)
+        t_fp64 = timed(lambda: fb.run_async(A.FIT_FP64))
+        t_mixed = timed(lambda: fb.run_async(A.FIT_MIXED))
+        t_fused = timed(ab.loss_grad_async)
+        N = np.full(n_tracks, 4)
+        ab.vjp_async(np.zeros((5, n_tracks), np.float32), (2 * N - 5).astype(np.float32))
+        o = ab.out
+
+        def rrule():
+            fb.run_async(A.FIT_MIXED)
+            A.check(ctx.lib.pt_fit_bl_vjp_async(ctx.h, n_tracks, ab.d_hits, ab.d_sig, ab.d_det, ab.d_nhit,
+                                                A.B_FIELD_INV_GEV, o["dstate"], o["dchi2"], o["grad"]),
+                    "pt_fit_bl_vjp_async")
+        t_rrule = timed(rrule)
+        for k, t in (("fp64_fit", t_fp64), ("mixed_fit", t_mixed), ("mixed_fit_adjoint_fused", t_fused),
+                     ("mixed_fit_then_pullback", t_rrule)):
+            out[f"{k}_tracks_per_s"] = round(n_tracks / t, 1)
+            out[f"{k}_ms"] = round(t * 1e3, 4)
+        out["fused_adjoint_vs_fp64_fit"] = round(t_fp64 / t_fused, 3)
+        out["rrule_vs_fp64_fit"] = round(t_fp64 / t_rrule, 3)
+    finally:
+        fb.free()
+        ab.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +203,7 @@ def main():
     ap.add_argument("--streams", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-fit", action="store_true", help="skip the fit-only workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the pixel-track stage shards by event with no exchange; "
@@ -174,6 +260,13 @@ def main():
         roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b)
     kernels_ms = {k: round(v, 4) for k, v in sorted(avg.items(), key=lambda kv: -kv[1])}
 
+    fit = adj = None
+    if not args.no_fit:
+        fit = fit_only(ctx)
+        print(f"[bench] fit-only: {fit}", file=sys.stderr, flush=True)
+        adj = adjoint_only(ctx)
+        print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(events, frames, budget_s=args.cpu_seconds)
@@ -203,6 +296,8 @@ def main():
         "kernels_ms_per_step": kernels_ms,
         "kernels_sum_ms_single_stream": round(sum(avg.values()), 4),
         "streams": args.streams,
+        "fit_only": fit,
+        "adjoint": adj,
     }
     print(json.dumps(line))
     batch.free()

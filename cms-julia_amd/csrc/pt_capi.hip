@@ -19,8 +19,10 @@ namespace pt {
 void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32_t max_hits, uint32_t max_slots,
                int zero_state, hipStream_t st, Marks *tm);
 void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm);
-void launch_fit_flat(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double B, int precision,
-                     float *state, float *cov, float *pt, float *eta, float *chi2, hipStream_t st);
+void launch_fit_flat(const FlatArgs &a, int precision, const FitLaunch &L, hipStream_t st);
+void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hipStream_t st);
+void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, const float *frames, float *ge,
+                            hipStream_t st);
 void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
 }  // namespace pt
 
@@ -146,6 +148,13 @@ struct pt_ctx {
     hipEvent_t ev_join[kMaxSub] = {};
     bool have_marks = false;
     uint32_t last_n = 0;
+    // fit-only batches: persistent-launch shape per precision, scratch for
+    // the per-length lists (grown on demand) and the counters / ticket
+    int n_cu = 256;
+    pt_fit_tuning tuning[3] = {{256, 512, 1024}, {256, 512, 1024}, {256, 256, 1024}};
+    uint32_t *fit_lists = nullptr;
+    size_t fit_lists_cap = 0;  // track ids
+    uint32_t *fit_counts = nullptr;
 };
 
 extern "C" {
@@ -228,8 +237,15 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         hipHostMalloc((void **)&c->h_cnt, sizeof(pt_counters) * max_events, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&c->d_evs, sizeof(pt::EvDesc) * max_events) != hipSuccess ||
         hipHostMalloc((void **)&c->h_evs, sizeof(pt::EvDesc) * max_events, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&c->d_frames, sizeof(float) * 9 * PT_N_MODULES) != hipSuccess)
+        hipMalloc((void **)&c->d_frames, sizeof(float) * 9 * PT_N_MODULES) != hipSuccess ||
+        hipMalloc((void **)&c->fit_counts, 4 * sizeof(uint32_t)) != hipSuccess)
         return cleanup(fail(PT_ERR_HIP, "context allocation failed"));
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+            c->n_cu = ncu;
+        for (int k = 0; k < 3; ++k) c->tuning[k].blocks = 4u * (uint32_t)c->n_cu;
+    }
     std::memset(c->h_cnt, 0, sizeof(pt_counters) * max_events);
     for (int k = 0; k < pt::Marks::kMax; ++k)
         if (hipEventCreate(&c->marks.ev[k]) != hipSuccess) return cleanup(fail(PT_ERR_HIP, "hipEventCreate failed"));
@@ -263,6 +279,8 @@ int pt_ctx_destroy(pt_ctx *c) {
     (void)hipHostFree(c->h_evs);
     (void)hipFree(c->d_frames);
     (void)hipFree(c->stg.block);
+    (void)hipFree(c->fit_lists);
+    (void)hipFree(c->fit_counts);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;
@@ -626,15 +644,174 @@ int pt_get_stage_times(pt_ctx *c, float *ms, const char **names, int *n_stages) 
     return PT_OK;
 }
 
-int pt_fit_bl_device_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
-                           double b_field, int precision, float *state, float *cov, float *pt, float *eta,
-                           float *chi2) {
+static int check_tuning(const pt_fit_tuning &t) {
+    if (t.block != 64 && t.block != 128 && t.block != 192 && t.block != 256)
+        return fail(PT_ERR_ARG, "block must be 64, 128, 192 or 256 threads");
+    if (t.tile < 64 || t.tile > 2048 || t.tile % 64) return fail(PT_ERR_ARG, "tile must be a multiple of 64 in 64..2048");
+    if (t.blocks < 1 || t.blocks > 65535u) return fail(PT_ERR_ARG, "blocks must be 1..65535");
+    return PT_OK;
+}
+
+static int fit_flat(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
+                    int precision, int schedule, const pt_fit_tuning &t, float *state, float *cov, float *pt,
+                    float *eta, float *chi2) {
     if (!c || (n && (!hits || !ge || !nhit || !state || !cov || !pt || !eta || !chi2)))
         return fail(PT_ERR_ARG, "null pointer");
     if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (schedule != PT_SCHED_PLAIN && schedule != PT_SCHED_PER_LENGTH && schedule != PT_SCHED_PERSISTENT)
+        return fail(PT_ERR_ARG, "unknown schedule");
+    if (n > (1u << 30)) return fail(PT_ERR_ARG, "n_tracks above 2^30");
     HIP_OK(hipSetDevice(c->device));
-    pt::launch_fit_flat(n, hits, ge, nhit, (double)(float)b_field, precision, state, cov, pt, eta, chi2, c->stream);
+    if (schedule == PT_SCHED_PER_LENGTH && (size_t)2 * n > c->fit_lists_cap) {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->fit_lists);
+        c->fit_lists = nullptr;
+        c->fit_lists_cap = 0;
+        HIP_OK(hipMalloc((void **)&c->fit_lists, (size_t)2 * n * sizeof(uint32_t)));
+        c->fit_lists_cap = (size_t)2 * n;
+    }
+    pt::FlatArgs a;
+    a.hits = hits, a.ge = ge, a.nhit = nhit, a.n = n, a.B = (double)(float)b_field;
+    a.state = state, a.cov = cov, a.pt = pt, a.eta = eta, a.chi2 = chi2;
+    pt::FitLaunch L;
+    L.schedule = schedule;
+    L.block = t.block, L.tile = t.tile, L.blocks = t.blocks;
+    L.lists = c->fit_lists;
+    L.counts = c->fit_counts;
+    pt::launch_fit_flat(a, precision, L, c->stream);
     HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+int pt_fit_bl_device_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                           double b_field, int precision, float *state, float *cov, float *pt, float *eta,
+                           float *chi2) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
+    return fit_flat(c, n, hits, ge, nhit, b_field, precision, PT_SCHED_PERSISTENT, c->tuning[pi], state, cov, pt,
+                    eta, chi2);
+}
+
+int pt_fit_bl_sched_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                          double b_field, int precision, int schedule, float *state, float *cov, float *pt,
+                          float *eta, float *chi2) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
+    return fit_flat(c, n, hits, ge, nhit, b_field, precision, schedule, c->tuning[pi], state, cov, pt, eta, chi2);
+}
+
+int pt_fit_set_tuning(pt_ctx *c, int precision, const pt_fit_tuning *t) {
+    if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+        return fail(PT_ERR_ARG, "unknown precision");
+    int rc = check_tuning(*t);
+    if (rc) return rc;
+    c->tuning[precision] = *t;
+    return PT_OK;
+}
+
+int pt_fit_get_tuning(pt_ctx *c, int precision, pt_fit_tuning *t) {
+    if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+        return fail(PT_ERR_ARG, "unknown precision");
+    *t = c->tuning[precision];
+    return PT_OK;
+}
+
+int pt_hit_errors_to_global_async(pt_ctx *c, uint32_t n, const float *sig, const int16_t *det, float *ge) {
+    if (!c || (n && (!sig || !det || !ge))) return fail(PT_ERR_ARG, "null pointer");
+    if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
+    if (n > (1u << 30)) return fail(PT_ERR_ARG, "n_tracks above 2^30");
+    HIP_OK(hipSetDevice(c->device));
+    pt::launch_local_to_global(n, sig, det, c->d_frames, ge, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+static int fit_adjoint(pt_ctx *c, uint32_t n, const float *hits, const float *sig, const int16_t *det,
+                       const uint8_t *nhit, double b_field, const float *truth, const float *dstate,
+                       const float *dchi2, float *state, float *chi2, float *loss, float *grad, int loss_mode) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (n && (!hits || !sig || !det || !nhit || !grad)) return fail(PT_ERR_ARG, "null pointer");
+    if (n && loss_mode && (!truth || !loss)) return fail(PT_ERR_ARG, "null truth or loss");
+    if (n && !loss_mode && (!dstate || !dchi2)) return fail(PT_ERR_ARG, "null cotangents");
+    if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
+    if (n > (1u << 30)) return fail(PT_ERR_ARG, "n_tracks above 2^30");
+    HIP_OK(hipSetDevice(c->device));
+    pt::AdjArgs a;
+    a.hits = hits, a.sig = sig, a.det = det, a.nhit = nhit, a.n = n, a.B = (double)(float)b_field;
+    a.frames = c->d_frames;
+    a.truth = truth, a.dstate = dstate, a.dchi2 = dchi2;
+    a.state = state, a.chi2 = chi2, a.loss = loss, a.grad = grad;
+    const pt_fit_tuning &t = c->tuning[PT_FIT_ADJOINT];
+    pt::FitLaunch L;
+    L.schedule = PT_SCHED_PERSISTENT;
+    L.block = t.block, L.tile = t.tile, L.blocks = t.blocks;
+    L.lists = c->fit_lists;
+    L.counts = c->fit_counts;
+    pt::launch_fit_adjoint(a, loss_mode, L, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+int pt_fit_bl_loss_grad_async(pt_ctx *c, uint32_t n, const float *hits, const float *sig, const int16_t *det,
+                              const uint8_t *nhit, const float *truth, double b_field, float *state, float *chi2,
+                              float *loss, float *grad) {
+    return fit_adjoint(c, n, hits, sig, det, nhit, b_field, truth, nullptr, nullptr, state, chi2, loss, grad, 1);
+}
+
+int pt_fit_bl_vjp_async(pt_ctx *c, uint32_t n, const float *hits, const float *sig, const int16_t *det,
+                        const uint8_t *nhit, double b_field, const float *dstate, const float *dchi2, float *grad) {
+    return fit_adjoint(c, n, hits, sig, det, nhit, b_field, nullptr, dstate, dchi2, nullptr, nullptr, nullptr, grad,
+                       0);
+}
+
+int pt_fit_autotune(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
+                    int precision, float *state, float *cov, float *pt, float *eta, float *chi2,
+                    pt_fit_tuning *best, float *ms) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (n == 0) return fail(PT_ERR_ARG, "empty batch");
+    HIP_OK(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    const uint32_t blocks_[3] = {64, 128, 256}, tiles[4] = {128, 256, 512, 1024}, per_cu[4] = {1, 2, 4, 8};
+    pt_fit_tuning top = c->tuning[precision == PT_FIT_MIXED ? 1 : 0];
+    float top_ms = 1e30f;
+    int rc = PT_OK;
+    for (uint32_t bs : blocks_)
+        for (uint32_t tl : tiles)
+            for (uint32_t pc : per_cu) {
+                pt_fit_tuning t{bs, tl, pc * (uint32_t)c->n_cu * (256u / bs)};
+                if (t.blocks > 65535u) continue;
+                float best_run = 1e30f;
+                for (int rep = 0; rep < 3 && rc == PT_OK; ++rep) {
+                    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(PT_ERR_HIP, "hipEventRecord failed");
+                    if (rc == PT_OK)
+                        rc = fit_flat(c, n, hits, ge, nhit, b_field, precision, PT_SCHED_PERSISTENT, t, state, cov,
+                                      pt, eta, chi2);
+                    if (rc == PT_OK && hipEventRecord(e1, c->stream) != hipSuccess)
+                        rc = fail(PT_ERR_HIP, "hipEventRecord failed");
+                    if (rc == PT_OK && hipEventSynchronize(e1) != hipSuccess)
+                        rc = fail(PT_ERR_HIP, "hipEventSynchronize failed");
+                    float dt = 0;
+                    if (rc == PT_OK && hipEventElapsedTime(&dt, e0, e1) != hipSuccess)
+                        rc = fail(PT_ERR_HIP, "hipEventElapsedTime failed");
+                    if (rep > 0 && dt < best_run) best_run = dt;  // first run warms up
+                }
+                if (rc != PT_OK) break;
+                if (best_run < top_ms) {
+                    top_ms = best_run;
+                    top = t;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PT_OK) return rc;
+    c->tuning[precision == PT_FIT_MIXED ? 1 : 0] = top;
+    if (best) *best = top;
+    if (ms) *ms = top_ms;
     return PT_OK;
 }
 

@@ -135,6 +135,42 @@ struct Config {
     PairTables pairs;
 };
 
+// Fit-only batch (pre-gathered tracks, track-minor layout of pixeltrack.h).
+struct FlatArgs {
+    PT_G const float *hits, *ge;
+    PT_G const uint8_t *nhit;
+    uint32_t n;
+    double B;
+    PT_G float *state, *cov, *pt, *eta, *chi2;
+};
+
+// Adjoint of the mixed fit over a fit-only batch (hit errors as local sigmas
+// sig[(c*5+k)*n+t], c = rphi, z, and module ids det[k*n+t]).
+struct AdjArgs {
+    PT_G const float *hits, *sig;
+    PT_G const int16_t *det;
+    PT_G const uint8_t *nhit;
+    uint32_t n;
+    double B;
+    PT_G const float *frames;  // [1856*9]
+    PT_G const float *truth;   // [5][n]  loss mode
+    PT_G const float *dstate;  // [5][n]  pullback mode
+    PT_G const float *dchi2;   // [n]     pullback mode
+    PT_G float *state, *chi2;  // forward outputs (may be null)
+    PT_G float *loss;          // [n]     loss mode
+    PT_G float *grad;          // [25][n] grad[(c*5+k)*n+t], c = x, y, z, sigma_rphi, sigma_z
+};
+
+// Launch shape of a fit-only batch: schedule (PT_SCHED_*), threads per block,
+// tracks per ticket (persistent) and blocks (persistent); device scratch for
+// the per-length lists (2 n ids) and the counters / ticket (4 words).
+struct FitLaunch {
+    int schedule;
+    uint32_t block, tile, blocks;
+    PT_G uint32_t *lists;
+    PT_G uint32_t *counts;
+};
+
 // Host-side kernel timing marks (HIP events recorded on the launch stream).
 struct Marks {
     static constexpr int kMax = 48;

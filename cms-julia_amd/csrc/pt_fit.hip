@@ -11,6 +11,7 @@
 
 #include <algorithm>
 
+#include "pt_bl_adjoint.hpp"
 #include "pt_bl_fit.hpp"
 #include "pt_device.hpp"
 
@@ -45,7 +46,7 @@ __device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, i
     for (uint32_t idx = idx0; idx < n; idx += stride) {
         const uint32_t tkid = e.s.tm_bins[b0 + idx];  // 1-based
         const uint32_t start = e.hit_off[tkid - 1] - 1u;
-        double hx[N], hy[N], hz[N];
+        float hx[N], hy[N], hz[N];
         float ge[6][N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -105,42 +106,229 @@ __global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ e
     }
 }
 
-// Pre-gathered tracks: hits[(c*5+k)*n+t], ge[(c*5+k)*n+t]; nhit 3 -> N=3, 4/5 -> N=4.
-template <int N>
-__device__ __forceinline__ void fit_flat_one(uint32_t n, uint32_t t, const float *__restrict__ hits,
-                                             const float *__restrict__ ge, double B, float *__restrict__ state,
-                                             float *__restrict__ cov, float *__restrict__ pt, float *__restrict__ eta,
-                                             float *__restrict__ chi2) {
-    double hx[N], hy[N], hz[N];
-    float g[6][N];
+// ------------------------------------------------------------------ fit-only batches
+// Pre-gathered tracks, track-minor: hits[(c*5+k)*n+t], ge[(c*5+k)*n+t];
+// nhit 3 -> N=3, 4 or 5 -> N=4 (fit5as4); any other count gives NaN outputs.
+// A "job" is the per-track work of a batch kernel: cls(t) gives the length
+// class (0: 3 hits, 1: 4 or 5 hits, 2: invalid), one<N>(t) the work,
+// invalid(t) the output for an invalid track.  The schedules below run any job.
+
+__device__ __forceinline__ int fit_class(uint8_t nh) { return nh == 3 ? 0 : ((nh == 4 || nh == 5) ? 1 : 2); }
+
+template <class P> struct FlatJob {
+    FlatArgs a;
+    __device__ __forceinline__ uint32_t size() const { return a.n; }
+    __device__ __forceinline__ int cls(uint32_t t) const { return fit_class(a.nhit[t]); }
+    template <int N> __device__ __forceinline__ void one(uint32_t t) const {
+        const size_t n = a.n;
+        float hx[N], hy[N], hz[N];
+        float g[6][N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        hx[k] = hits[(size_t)(0 * 5 + k) * n + t];
-        hy[k] = hits[(size_t)(1 * 5 + k) * n + t];
-        hz[k] = hits[(size_t)(2 * 5 + k) * n + t];
+        for (int k = 0; k < N; ++k) {
+            hx[k] = a.hits[(size_t)(0 * 5 + k) * n + t];
+            hy[k] = a.hits[(size_t)(1 * 5 + k) * n + t];
+            hz[k] = a.hits[(size_t)(2 * 5 + k) * n + t];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) g[c][k] = ge[(size_t)(c * 5 + k) * n + t];
+            for (int c = 0; c < 6; ++c) g[c][k] = a.ge[(size_t)(c * 5 + k) * n + t];
+        }
+        FitOut<typename P::Prop> o;
+        BLFit<P, N> f;
+        f.run(hx, hy, hz, g, a.B, o);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a.state[(size_t)k * n + t] = (float)o.state[k];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) a.cov[(size_t)k * n + t] = (float)o.cov[k];
+        a.pt[t] = (float)o.pt;
+        a.eta[t] = (float)o.eta;
+        a.chi2[t] = (float)o.chi2;
     }
-    FitOut<double> o;
-    bl_fit_fp64<N>(hx, hy, hz, g, B, o);
+    __device__ __forceinline__ void invalid(uint32_t t) const {
+        const float nan = __builtin_nanf("");
 #pragma unroll
-    for (int k = 0; k < 5; ++k) state[(size_t)k * n + t] = (float)o.state[k];
+        for (int k = 0; k < 5; ++k) a.state[(size_t)k * a.n + t] = nan;
+        a.pt[t] = nan;
+        a.eta[t] = nan;
+        a.chi2[t] = nan;
+    }
+};
+
+// Adjoint of the mixed fit per track: hit errors come as local sigmas and
+// module ids (global covariance from the module frame in float, as the CA
+// path does), the forward fit is recomputed and the reverse sweep gives
+// dL/d(x, y, z, sigma_rphi, sigma_z) of each hit:
+//   LOSS = true:  L = chi2_line + chi2_circle + |state - truth|^2 (phi
+//                 difference taken modulo 2 pi), written to loss[t]
+//   LOSS = false: the pullback of the cotangents dstate[5][n], dchi2[n] of
+//                 (state, chi2) (chi2 = (chi2_line + chi2_circle) / (2N - 5))
+// Hits a 3-hit track does not have, and the fifth hit of a 5-hit track
+// (fitted on its first four), get zero gradient.
+template <bool LOSS> struct AdjJob {
+    AdjArgs a;
+    __device__ __forceinline__ uint32_t size() const { return a.n; }
+    __device__ __forceinline__ int cls(uint32_t t) const { return fit_class(a.nhit[t]); }
+    template <int N> __device__ __forceinline__ void one(uint32_t t) const {
+        const size_t n = a.n;
+        float hx[N], hy[N], hz[N], sr[N], sz[N];
+        float g[6][N];
+        int det[N];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) cov[(size_t)k * n + t] = (float)o.cov[k];
-    pt[t] = (float)o.pt;
-    eta[t] = (float)o.eta;
-    chi2[t] = (float)o.chi2;
+        for (int k = 0; k < N; ++k) {
+            hx[k] = a.hits[(size_t)(0 * 5 + k) * n + t];
+            hy[k] = a.hits[(size_t)(1 * 5 + k) * n + t];
+            hz[k] = a.hits[(size_t)(2 * 5 + k) * n + t];
+            sr[k] = a.sig[(size_t)(0 * 5 + k) * n + t];
+            sz[k] = a.sig[(size_t)(1 * 5 + k) * n + t];
+            int d = a.det[(size_t)k * n + t];
+            det[k] = (d < 0 || d >= (int)PT_N_MODULES) ? 0 : d;
+            float gl[6];
+            to_global_err(a.frames + 9 * det[k], sr[k] * sr[k], sz[k] * sz[k], gl);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) g[c][k] = gl[c];
+        }
+        FitOut<float> o;
+        BLFit<PolicyMixed, N> f;
+        f.run(hx, hy, hz, g, a.B, o);
+        double bstate[5], bcl, bcc;
+        if (LOSS) {
+            double L = (double)o.chi2_line + (double)o.chi2_circle;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                double dlt = (double)o.state[k] - (double)a.truth[(size_t)k * n + t];
+                if (k == 0) dlt -= 2.0 * kPiD * rint(dlt / (2.0 * kPiD));
+                L += dlt * dlt;
+                bstate[k] = 2.0 * dlt;
+            }
+            bcl = bcc = 1.0;
+            a.loss[t] = (float)L;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) bstate[k] = a.dstate[(size_t)k * n + t];
+            bcl = bcc = (double)a.dchi2[t] / (double)(2 * N - 5);
+        }
+        double gx[N], gy[N], gz[N], gge[6][N];
+        bl_fit_vjp<PolicyMixed, N>(f, hx, hy, hz, g, a.B, bstate, bcl, bcc, gx, gy, gz, gge);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double gsr = 0.0, gsz = 0.0, gxk = 0.0, gyk = 0.0, gzk = 0.0;
+            if (k < N) {
+                double bg[6], bcxx, bcyy;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) bg[c] = gge[c][k];
+                local_err_vjp(a.frames + 9 * det[k], bg, bcxx, bcyy);
+                gsr = 2.0 * (double)sr[k] * bcxx;
+                gsz = 2.0 * (double)sz[k] * bcyy;
+                gxk = gx[k];
+                gyk = gy[k];
+                gzk = gz[k];
+            }
+            a.grad[(size_t)(0 * 5 + k) * n + t] = (float)gxk;
+            a.grad[(size_t)(1 * 5 + k) * n + t] = (float)gyk;
+            a.grad[(size_t)(2 * 5 + k) * n + t] = (float)gzk;
+            a.grad[(size_t)(3 * 5 + k) * n + t] = (float)gsr;
+            a.grad[(size_t)(4 * 5 + k) * n + t] = (float)gsz;
+        }
+        if (a.state) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) a.state[(size_t)k * n + t] = o.state[k];
+        }
+        if (a.chi2) a.chi2[t] = o.chi2;
+    }
+    __device__ __forceinline__ void invalid(uint32_t t) const {
+        const float nan = __builtin_nanf("");
+        for (int c = 0; c < 25; ++c) a.grad[(size_t)c * a.n + t] = nan;
+        if (LOSS) a.loss[t] = nan;
+        if (a.state)
+            for (int k = 0; k < 5; ++k) a.state[(size_t)k * a.n + t] = nan;
+        if (a.chi2) a.chi2[t] = nan;
+    }
+};
+
+// One thread per track in input order: lanes of one wavefront take both
+// lengths, so a mixed wavefront runs both bodies.
+template <class J> __global__ void __launch_bounds__(256) k_fit_plain(J j) {
+    const uint32_t n = j.size();
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int c = j.cls(t);
+        if (c == 0)
+            j.template one<3>(t);
+        else if (c == 1)
+            j.template one<4>(t);
+        else
+            j.invalid(t);
+    }
 }
 
-__global__ void __launch_bounds__(256) k_fit_flat_fp64(uint32_t n, const float *__restrict__ hits,
-                                                       const float *__restrict__ ge, const uint8_t *__restrict__ nhit,
-                                                       double B, float *state, float *cov, float *pt, float *eta,
-                                                       float *chi2) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        if (nhit[t] == 3)
-            fit_flat_one<3>(n, t, hits, ge, B, state, cov, pt, eta, chi2);
-        else
-            fit_flat_one<4>(n, t, hits, ge, B, state, cov, pt, eta, chi2);
+// Per-length schedule, step 1: split the track ids by length class into
+// lists[0..n) (3 hits) and lists[n..2n) (4 or 5 hits); counts[2] must start
+// at 0.  Each wavefront appends its ids in input order at a position taken
+// once per wavefront, so neighbouring ids stay together.
+template <class J> __global__ void __launch_bounds__(256) k_fit_partition(J j, uint32_t *lists, uint32_t *counts) {
+    const uint32_t n = j.size();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t t = base + lane;
+        const int c = (t < n) ? j.cls(t) : 3;
+        if (c == 2) j.invalid(t);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t m = __ballot(c == k);
+            if (!m) continue;
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&counts[k], (uint32_t)__popcll(m));
+            pos = __shfl(pos, 0, 64);
+            if (c == k) lists[(size_t)k * n + pos + __popcll(m & below)] = t;
+        }
+    }
+}
+
+// Per-length schedule, step 2: the tracks of one length class.
+template <class J, int N> __global__ void __launch_bounds__(256) k_fit_list(J j, const uint32_t *list,
+                                                                            const uint32_t *count) {
+    const uint32_t m = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        j.template one<N>(list[i]);
+}
+
+// One persistent launch over uneven lengths: every wavefront takes tiles of
+// `tile` consecutive tracks by ticket, sorts the tile's ids into one LDS list
+// per length class (ballot compaction, input order kept) and then runs each
+// list 64 tracks at a time, so all active lanes of a pass run the same body.
+// *ticket must start at 0; dynamic LDS = 8 * tile bytes per wavefront.
+template <class J> __global__ void __launch_bounds__(256) k_fit_persistent(J j, uint32_t tile, uint32_t *ticket) {
+    extern __shared__ uint32_t fit_lds[];
+    const uint32_t n = j.size();
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t *l3 = fit_lds + (size_t)wid * 2u * tile;
+    uint32_t *l4 = l3 + tile;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t n_tiles = (n + tile - 1u) / tile;
+    for (;;) {
+        uint32_t tk = 0;
+        if (lane == 0) tk = atomicAdd(ticket, 1u);
+        tk = __shfl(tk, 0, 64);
+        if (tk >= n_tiles) break;
+        const uint32_t t0 = tk * tile, t1 = min(t0 + tile, n);
+        uint32_t c3 = 0, c4 = 0;
+        for (uint32_t base = t0; base < t1; base += 64u) {
+            const uint32_t t = base + lane;
+            const int c = (t < t1) ? j.cls(t) : 3;
+            if (c == 2) j.invalid(t);
+            const uint64_t m3 = __ballot(c == 0), m4 = __ballot(c == 1);
+            if (c == 0) l3[c3 + __popcll(m3 & below)] = t;
+            if (c == 1) l4[c4 + __popcll(m4 & below)] = t;
+            c3 += (uint32_t)__popcll(m3);
+            c4 += (uint32_t)__popcll(m4);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t p = 0; p < c3; p += 64u)
+            if (p + lane < c3) j.template one<3>(l3[p + lane]);
+        for (uint32_t p = 0; p < c4; p += 64u)
+            if (p + lane < c4) j.template one<4>(l4[p + lane]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -163,13 +351,71 @@ void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st) {
     k_fill_bytes<<<(unsigned)blocks, 256, 0, st>>>((uint8_t *)dst, value, bytes);
 }
 
-void launch_fit_flat(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double B, int precision,
-                     float *state, float *cov, float *pt, float *eta, float *chi2, hipStream_t st) {
-    (void)precision;
-    uint32_t blocks = (n + 255) / 256;
-    if (blocks > 65535u * 4u) blocks = 65535u * 4u;
-    if (blocks == 0) return;
-    k_fit_flat_fp64<<<blocks, 256, 0, st>>>(n, hits, ge, nhit, B, state, cov, pt, eta, chi2);
+template <class J>
+static void launch_job(const J &j, uint32_t n, const FitLaunch &L, hipStream_t st) {
+    if (n == 0) return;
+    if (L.schedule == PT_SCHED_PLAIN) {
+        uint32_t blocks = (n + L.block - 1) / L.block;
+        if (blocks > 65535u * 4u) blocks = 65535u * 4u;
+        k_fit_plain<J><<<blocks, L.block, 0, st>>>(j);
+    } else if (L.schedule == PT_SCHED_PER_LENGTH) {
+        launch_fill_bytes(L.counts, 0, 2 * sizeof(uint32_t), st);
+        uint32_t blocks = (n + 255u) / 256u;
+        if (blocks > 4096u) blocks = 4096u;
+        k_fit_partition<J><<<blocks, 256, 0, st>>>(j, L.lists, L.counts);
+        uint32_t fb = (n + L.block - 1) / L.block;
+        if (fb > 65535u * 4u) fb = 65535u * 4u;
+        k_fit_list<J, 3><<<fb, L.block, 0, st>>>(j, L.lists, L.counts);
+        k_fit_list<J, 4><<<fb, L.block, 0, st>>>(j, L.lists + n, L.counts + 1);
+    } else {
+        launch_fill_bytes(L.counts, 0, sizeof(uint32_t), st);
+        const uint32_t waves = L.block / 64u;
+        const size_t lds = (size_t)waves * 8u * L.tile;
+        uint32_t blocks = L.blocks;
+        const uint32_t need = (n + L.tile * waves - 1) / (L.tile * waves);
+        if (blocks > need) blocks = need;
+        if (blocks < 1) blocks = 1;
+        k_fit_persistent<J><<<blocks, L.block, lds, st>>>(j, L.tile, L.counts);
+    }
+}
+
+void launch_fit_flat(const FlatArgs &a, int precision, const FitLaunch &L, hipStream_t st) {
+    if (precision == PT_FIT_MIXED)
+        launch_job(FlatJob<PolicyMixed>{a}, a.n, L, st);
+    else
+        launch_job(FlatJob<PolicyFP64>{a}, a.n, L, st);
+}
+
+void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hipStream_t st) {
+    if (loss_mode)
+        launch_job(AdjJob<true>{a}, a.n, L, st);
+    else
+        launch_job(AdjJob<false>{a}, a.n, L, st);
+}
+
+// Global covariances of pre-gathered hits from local sigmas and module ids
+// (SOARotation.jl:139-147, float): ge[(c*5+k)*n+t].
+__global__ void __launch_bounds__(256) k_local_to_global(uint32_t n, const float *__restrict__ sig,
+                                                         const int16_t *__restrict__ det,
+                                                         const float *__restrict__ frames, float *__restrict__ ge) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 5u * n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = i / n, t = i - k * n;
+        const float sr = sig[(size_t)(0 * 5 + k) * n + t], sz = sig[(size_t)(1 * 5 + k) * n + t];
+        int d = det[(size_t)k * n + t];
+        if (d < 0 || d >= (int)PT_N_MODULES) d = 0;
+        float gl[6];
+        to_global_err(frames + 9 * d, sr * sr, sz * sz, gl);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ge[(size_t)(c * 5 + k) * n + t] = gl[c];
+    }
+}
+
+void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, const float *frames, float *ge,
+                            hipStream_t st) {
+    if (n == 0) return;
+    uint32_t blocks = (5u * n + 255u) / 256u;
+    if (blocks > 65535u) blocks = 65535u;
+    k_local_to_global<<<blocks, 256, 0, st>>>(n, sig, det, frames, ge);
 }
 
 }  // namespace pt

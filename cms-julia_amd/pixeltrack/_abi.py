@@ -45,7 +45,8 @@ REF = {
     "FIT_NAN": 0x020, "FIT_CHARGE": 0x040, "SQRT_DOMAIN": 0x080, "BAD_MODULE": 0x100, "MULT": 0x200,
 }
 QUALITY_BAD, QUALITY_DUP, QUALITY_LOOSE = 0, 1, 2
-FIT_FP64, FIT_MIXED = 0, 1
+FIT_FP64, FIT_MIXED, FIT_ADJOINT = 0, 1, 2
+SCHED_PLAIN, SCHED_PER_LENGTH, SCHED_PERSISTENT = 0, 1, 2
 
 # bf used by CAHitNtuplet.produce (ca_hit_ntuplet.jl:11)
 B_FIELD_INV_GEV = 0.0114256972711507
@@ -169,11 +170,20 @@ class PtgFitConfig(C.Structure):
 
 
 # ---------------------------------------------------------------- loaders
+class PtFitTuning(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("tile", C.c_uint32), ("blocks", C.c_uint32)]
+
+    def as_dict(self):
+        return {"block": self.block, "tile": self.tile, "blocks": self.blocks}
+
+
 HIP_SYMBOLS = [
     "pt_params_default", "pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_device_count",
     "pt_ctx_set_frames", "pt_make_tuples", "pt_make_tuples_device_async", "pt_ctx_sync", "pt_get_counters",
     "pt_get_ca_debug", "pt_ctx_set_streams", "pt_ctx_enable_timing", "pt_get_stage_times", "pt_fit_bl_device_async",
-    "pt_malloc", "pt_free", "pt_memcpy_h2d", "pt_memcpy_d2h", "pt_memset",
+    "pt_malloc", "pt_free", "pt_memcpy_h2d", "pt_memcpy_d2h", "pt_memset", "pt_fit_bl_sched_async",
+    "pt_fit_set_tuning", "pt_fit_get_tuning", "pt_fit_autotune", "pt_hit_errors_to_global_async",
+    "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -222,6 +232,15 @@ def hip_lib():
         lib.pt_ctx_set_streams.argtypes = [vp, C.c_int]
         lib.pt_get_stage_times.argtypes = [vp, c_f32p, C.POINTER(C.c_char_p), C.POINTER(C.c_int)]
         lib.pt_fit_bl_device_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
+        lib.pt_fit_bl_sched_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
+                                              vp, vp]
+        lib.pt_fit_set_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
+        lib.pt_fit_get_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
+        lib.pt_fit_autotune.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp,
+                                        C.POINTER(PtFitTuning), C.POINTER(C.c_float)]
+        lib.pt_hit_errors_to_global_async.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        lib.pt_fit_bl_loss_grad_async.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp]
+        lib.pt_fit_bl_vjp_async.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, C.c_double, vp, vp, vp]
         lib.pt_malloc.argtypes = [vp, C.POINTER(vp), C.c_size_t]
         lib.pt_free.argtypes = [vp, vp]
         lib.pt_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]

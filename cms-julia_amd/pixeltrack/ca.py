@@ -252,28 +252,132 @@ class DeviceBatch:
         self.ptrs = []
 
 
-def fit_broken_line(ctx: Context, hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, precision=A.FIT_FP64):
-    """Fit pre-gathered tracks on the GPU (pt_fit_bl_device_async); returns host arrays."""
-    n = int(nhit.shape[0])
-    h = np.ascontiguousarray(hits, np.float32)
-    g = np.ascontiguousarray(ge, np.float32)
-    nh = np.ascontiguousarray(nhit, np.uint8)
-    bufs = {k: ctx.malloc(max(4, a.nbytes)) for k, a in (("h", h), ("g", g), ("n", nh))}
-    outs = {k: ctx.malloc(max(4, 4 * n * m)) for k, m in (("state", 5), ("cov", 15), ("pt", 1), ("eta", 1),
-                                                         ("chi2", 1))}
-    try:
-        ctx.h2d(bufs["h"], h)
-        ctx.h2d(bufs["g"], g)
-        ctx.h2d(bufs["n"], nh)
-        A.check(ctx.lib.pt_fit_bl_device_async(ctx.h, n, bufs["h"], bufs["g"], bufs["n"], b_field, precision,
-                                               outs["state"], outs["cov"], outs["pt"], outs["eta"], outs["chi2"]),
-                "pt_fit_bl_device_async")
-        ctx.sync()
-        res = dict(state=np.zeros((5, n), np.float32), cov=np.zeros((15, n), np.float32),
-                   pt=np.zeros(n, np.float32), eta=np.zeros(n, np.float32), chi2=np.zeros(n, np.float32))
+class FitBatch:
+    """Pre-gathered tracks resident in device memory (the fit-only entry
+    points of pixeltrack.h): hits [3,5,n], ge [6,5,n], nhit [n]."""
+
+    OUTS = (("state", 5), ("cov", 15), ("pt", 1), ("eta", 1), ("chi2", 1))
+
+    def __init__(self, ctx: Context, hits, ge, nhit):
+        self.ctx = ctx
+        self.n = int(nhit.shape[0])
+        h = np.ascontiguousarray(hits, np.float32)
+        g = np.ascontiguousarray(ge, np.float32)
+        nh = np.ascontiguousarray(nhit, np.uint8)
+        self.ptrs = []
+        self.d_hits, self.d_ge, self.d_nhit = (self._upload(x) for x in (h, g, nh))
+        self.out = {}
+        for k, m in self.OUTS:
+            self.out[k] = ctx.malloc(max(4, 4 * self.n * m))
+            self.ptrs.append(self.out[k])
+
+    def _upload(self, arr):
+        p = self.ctx.malloc(max(4, arr.nbytes))
+        self.ptrs.append(p)
+        self.ctx.h2d(p, arr)
+        return p
+
+    def _outs(self):
+        return [self.out[k] for k, _ in self.OUTS]
+
+    def run_async(self, precision=A.FIT_FP64, schedule=None, b_field=A.B_FIELD_INV_GEV):
+        if schedule is None:
+            A.check(self.ctx.lib.pt_fit_bl_device_async(self.ctx.h, self.n, self.d_hits, self.d_ge, self.d_nhit,
+                                                        b_field, precision, *self._outs()), "pt_fit_bl_device_async")
+        else:
+            A.check(self.ctx.lib.pt_fit_bl_sched_async(self.ctx.h, self.n, self.d_hits, self.d_ge, self.d_nhit,
+                                                       b_field, precision, schedule, *self._outs()),
+                    "pt_fit_bl_sched_async")
+
+    def autotune(self, precision=A.FIT_FP64, b_field=A.B_FIELD_INV_GEV):
+        best = A.PtFitTuning()
+        ms = C.c_float()
+        A.check(self.ctx.lib.pt_fit_autotune(self.ctx.h, self.n, self.d_hits, self.d_ge, self.d_nhit, b_field,
+                                             precision, *self._outs(), C.byref(best), C.byref(ms)),
+                "pt_fit_autotune")
+        return best.as_dict(), ms.value
+
+    def download(self):
+        self.ctx.sync()
+        res = dict(state=np.zeros((5, self.n), np.float32), cov=np.zeros((15, self.n), np.float32),
+                   pt=np.zeros(self.n, np.float32), eta=np.zeros(self.n, np.float32),
+                   chi2=np.zeros(self.n, np.float32))
         for k in res:
-            ctx.d2h(res[k], outs[k])
+            self.ctx.d2h(res[k], self.out[k])
         return res
+
+    def free(self):
+        for p in self.ptrs:
+            self.ctx.free(p)
+        self.ptrs = []
+
+
+class AdjointBatch:
+    """Tracks for the adjoint of the mixed fit, resident in device memory:
+    hits [3,5,n], local sigmas sig [2,5,n] (cm), module ids det [5,n], nhit
+    [n] and optionally truth [5,n].  The context needs the module frames."""
+
+    def __init__(self, ctx: Context, hits, sig, det, nhit, truth=None):
+        self.ctx = ctx
+        self.n = int(nhit.shape[0])
+        self.ptrs = []
+        self.d_hits = self._upload(np.ascontiguousarray(hits, np.float32))
+        self.d_sig = self._upload(np.ascontiguousarray(sig, np.float32))
+        self.d_det = self._upload(np.ascontiguousarray(det, np.int16))
+        self.d_nhit = self._upload(np.ascontiguousarray(nhit, np.uint8))
+        self.d_truth = self._upload(np.ascontiguousarray(truth, np.float32)) if truth is not None else None
+        self.out = {}
+        for k, m in (("state", 5), ("chi2", 1), ("loss", 1), ("grad", 25), ("ge", 30), ("dstate", 5),
+                     ("dchi2", 1)):
+            self.out[k] = ctx.malloc(max(4, 4 * self.n * m))
+            self.ptrs.append(self.out[k])
+
+    def _upload(self, arr):
+        p = self.ctx.malloc(max(4, arr.nbytes))
+        self.ptrs.append(p)
+        self.ctx.h2d(p, arr)
+        return p
+
+    def loss_grad_async(self, b_field=A.B_FIELD_INV_GEV):
+        o = self.out
+        A.check(self.ctx.lib.pt_fit_bl_loss_grad_async(self.ctx.h, self.n, self.d_hits, self.d_sig, self.d_det,
+                                                       self.d_nhit, self.d_truth, b_field, o["state"], o["chi2"],
+                                                       o["loss"], o["grad"]), "pt_fit_bl_loss_grad_async")
+
+    def vjp_async(self, dstate, dchi2, b_field=A.B_FIELD_INV_GEV):
+        self.ctx.h2d(self.out["dstate"], np.ascontiguousarray(dstate, np.float32))
+        self.ctx.h2d(self.out["dchi2"], np.ascontiguousarray(dchi2, np.float32))
+        o = self.out
+        A.check(self.ctx.lib.pt_fit_bl_vjp_async(self.ctx.h, self.n, self.d_hits, self.d_sig, self.d_det,
+                                                 self.d_nhit, b_field, o["dstate"], o["dchi2"], o["grad"]),
+                "pt_fit_bl_vjp_async")
+
+    def global_errors_async(self):
+        A.check(self.ctx.lib.pt_hit_errors_to_global_async(self.ctx.h, self.n, self.d_sig, self.d_det,
+                                                           self.out["ge"]), "pt_hit_errors_to_global_async")
+
+    def download(self, *names):
+        self.ctx.sync()
+        shapes = {"state": (5, self.n), "chi2": (self.n,), "loss": (self.n,), "grad": (5, 5, self.n),
+                  "ge": (6, 5, self.n)}
+        return {k: self.ctx.d2h(np.zeros(shapes[k], np.float32), self.out[k]) for k in names}
+
+    def free(self):
+        for p in self.ptrs:
+            self.ctx.free(p)
+        self.ptrs = []
+
+
+def set_fit_tuning(ctx: Context, precision, block, tile, blocks):
+    t = A.PtFitTuning(block, tile, blocks)
+    A.check(ctx.lib.pt_fit_set_tuning(ctx.h, precision, C.byref(t)), "pt_fit_set_tuning")
+
+
+def fit_broken_line(ctx: Context, hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, precision=A.FIT_FP64, schedule=None):
+    """Fit pre-gathered tracks on the GPU; returns host arrays."""
+    fb = FitBatch(ctx, hits, ge, nhit)
+    try:
+        fb.run_async(precision, schedule, b_field)
+        return fb.download()
     finally:
-        for p in list(bufs.values()) + list(outs.values()):
-            ctx.free(p)
+        fb.free()

@@ -245,10 +245,69 @@ int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages
  * Outputs (device): state[k*n + t] (5), cov[k*n + t] (15), pt/eta/chi2[t].
  * ------------------------------------------------------------------------- */
 #define PT_FIT_FP64   0   /* plain double-precision fit, the reference's method        */
-#define PT_FIT_MIXED  1   /* float bulk, double pre-fit curvature and band solve        */
+#define PT_FIT_MIXED  1   /* float weights, scattering terms and covariance propagation;
+                             double fast fit, arc lengths, band solve and translations   */
 int pt_fit_bl_device_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
                            const uint8_t *nhit, double b_field, int precision,
                            float *state, float *cov, float *pt, float *eta, float *chi2);
+
+/* Schedules of a fit-only batch of uneven lengths (no reference counterpart:
+ * the reference loops over one length at a time, broken_line_fit_on_gpu.jl:166-193).
+ *   PT_SCHED_PLAIN       one thread per track in input order (mixed lengths
+ *                        in one wavefront run both bodies)
+ *   PT_SCHED_PER_LENGTH  a partition kernel, then one launch per length
+ *   PT_SCHED_PERSISTENT  one launch; wavefronts take tiles of tracks by
+ *                        ticket and fit each tile's lengths separately
+ * pt_fit_bl_device_async uses PT_SCHED_PERSISTENT with the context's tuning. */
+#define PT_SCHED_PLAIN       0
+#define PT_SCHED_PER_LENGTH  1
+#define PT_SCHED_PERSISTENT  2
+typedef struct pt_fit_tuning {
+    uint32_t block;   /* threads per block: 64, 128, 192 or 256             */
+    uint32_t tile;    /* tracks per ticket, a multiple of 64 (persistent)   */
+    uint32_t blocks;  /* blocks of the persistent launch                    */
+} pt_fit_tuning;
+int pt_fit_bl_sched_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
+                          const uint8_t *nhit, double b_field, int precision, int schedule,
+                          float *state, float *cov, float *pt, float *eta, float *chi2);
+/* Tuning of the persistent schedule per precision (PT_FIT_FP64, PT_FIT_MIXED
+ * or PT_FIT_ADJOINT; default: 256 threads, 512-track tiles, 4 blocks per CU). */
+int pt_fit_set_tuning(pt_ctx *ctx, int precision, const pt_fit_tuning *t);
+int pt_fit_get_tuning(pt_ctx *ctx, int precision, pt_fit_tuning *t);
+/* Times every candidate launch shape of the persistent schedule on the given
+ * batch (device pointers; outputs are overwritten), keeps the fastest in the
+ * context and returns it.  *ms: its time per launch.  precision: PT_FIT_FP64
+ * or PT_FIT_MIXED. */
+int pt_fit_autotune(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit,
+                    double b_field, int precision, float *state, float *cov, float *pt, float *eta,
+                    float *chi2, pt_fit_tuning *best, float *ms);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of the mixed-precision broken-line fit (no reference counterpart:
+ * the reference fit has no derivatives).  Tracks as for the fit-only entry
+ * points, but hit errors are given as local sigmas and module ids, as the
+ * rechit producer stores them (gpuPixelRecHits.jl:303-306), and rotated to
+ * the global frame with the context's module frames (pt_ctx_set_frames):
+ *   sig[(c*5 + k)*n + t]  c = sigma_rphi, sigma_z (cm)
+ *   det[k*n + t]          0-based module id
+ * Gradients: grad[(c*5 + k)*n + t], c = x, y, z, sigma_rphi, sigma_z.  The
+ * forward fit is recomputed inside the backward kernel (nothing is stored
+ * between the two).  Hits a track does not use get zero gradient.
+ * ------------------------------------------------------------------------- */
+#define PT_FIT_ADJOINT 2   /* tuning slot of the adjoint kernels (pt_fit_set_tuning) */
+/* ge[(c*5 + k)*n + t] from sig and det: toGlobal of SOARotation.jl:139-147. */
+int pt_hit_errors_to_global_async(pt_ctx *ctx, uint32_t n_tracks, const float *sig, const int16_t *det, float *ge);
+/* L = chi2_line + chi2_circle + |state - truth|^2 per track (phi difference
+ * modulo 2 pi; truth[k*n + t] in the state convention) and dL/d(hits).
+ * state, chi2 (forward results) may be NULL. */
+int pt_fit_bl_loss_grad_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *sig,
+                              const int16_t *det, const uint8_t *nhit, const float *truth, double b_field,
+                              float *state, float *chi2, float *loss, float *grad);
+/* Pullback (vector-Jacobian product) of (state[5], chi2) -> hits: the
+ * backward half of a ChainRulesCore rrule around the mixed fit.  dstate[k*n + t],
+ * dchi2[t] are the output cotangents. */
+int pt_fit_bl_vjp_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *sig, const int16_t *det,
+                        const uint8_t *nhit, double b_field, const float *dstate, const float *dchi2, float *grad);
 
 /* Device memory helpers (so hosts without a device allocator can stage data). */
 int pt_malloc(pt_ctx *ctx, void **ptr, size_t bytes);

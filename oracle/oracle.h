@@ -38,7 +38,7 @@ int oracle_bl_detail(int N, const double *hits, const float *ge, double B, doubl
  * [2]=div, [3]=sqrt, [4]=transcendental calls, [5]=comparisons. */
 int oracle_fit_opcount(int N, const double *hits, const float *ge, double B, uint64_t *counts);
 
-/* Forward-mode derivative of L = chi2_line + chi2_circle (+ ||params - truth||^2
+/* Forward-mode derivative of L = chi2_line + chi2_circle (+ ||params - truth||^2, phi difference modulo 2 pi,
  * if truth != NULL) with respect to (x, y, z, sigma_rphi, sigma_z) of each of
  * the N hits, through local variances rotated by the module frames rot[9*k].
  * grad[5*k + c]; value[0] = L. */

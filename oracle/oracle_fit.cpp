@@ -171,6 +171,7 @@ int grad_impl(const double *xyz, const double *sig, const float *rot, double B, 
     if (truth) {
         for (int k = 0; k < 5; ++k) {
             Dual dlt = o.state[k] - truth[k];
+            if (k == 0) dlt = dlt - 2.0 * oracle::kPi * std::rint(dlt.v / (2.0 * oracle::kPi));  // phi modulo 2 pi
             L = L + dlt * dlt;
         }
     }
