@@ -745,7 +745,11 @@ static int fit_adjoint(pt_ctx *c, uint32_t n, const float *hits, const float *si
     a.state = state, a.chi2 = chi2, a.loss = loss, a.grad = grad;
     const pt_fit_tuning &t = c->tuning[PT_FIT_ADJOINT];
     pt::FitLaunch L;
-    L.schedule = PT_SCHED_PERSISTENT;
+    // One thread per track in input order.  The persistent schedule's
+    // instantiation of this job gave wrong gradients for some tracks when
+    // built with -O3 (correct at -O1 and under this schedule), so it is not
+    // used for the adjoint.
+    L.schedule = PT_SCHED_PLAIN;
     L.block = t.block, L.tile = t.tile, L.blocks = t.blocks;
     L.lists = c->fit_lists;
     L.counts = c->fit_counts;
