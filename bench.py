@@ -67,6 +67,19 @@ def kernel_bytes(name, c):
     return None if b is None else int(np.sum(b))
 
 
+def profiled_traffic(kernel, path=ROOT / "profiles" / "hbm_traffic_pu50_batch100.md"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by a plain copy), or None."""
+    try:
+        for line in path.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0].strip("`").split("::")[-1] == kernel:
+                return int(float(cells[3]) * 1e6)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def run_cpu_baseline(events, frames, budget_s=12.0, max_threads=16):
     """Oracle (serial restatement of the reference stage, incl. classification)
     on host threads, one event per thread at a time like the reference's
@@ -193,6 +206,55 @@ def adjoint_only(ctx, n_tracks=262144, steps=20):
     return out
 
 
+def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
+    """PU200-like stream: 200 events (seeds 1001..1200, 7000 helices from 200
+    vertices, +5% noise), resident in HBM, run through the graph pipeline with
+    `group_size` events per captured hipGraph and `n_groups` groups in flight
+    on separate streams, the stage's fit in mixed precision.  events/s over the
+    stream, and per-event latency = GPU time of the replay that carried the
+    event (p50 / p99)."""
+    from pixeltrack import _abi as A, ca, generator as G
+    events = [G.generate_event("pu200", seed=1001 + i) for i in range(n_events)]
+    hits = np.array([e.n_hits for e in events])
+    ctx = ca.Context(max_events=group_size * n_groups, frames=frames)
+    batch = ca.DeviceBatch(ctx, events)
+    try:
+        ctx.set_fit_precision(A.FIT_MIXED)
+        pl = ca.Pipeline(ctx, group_size, n_groups)
+        n_sub = n_events // group_size
+
+        def run():
+            lat, pending = [], [False] * n_groups
+            for i in range(n_sub):
+                g = i % n_groups
+                if pending[g]:
+                    lat.append(pl.wait(g))
+                pl.submit(g, batch, i * group_size)
+                pending[g] = True
+            for g in range(n_groups):
+                if pending[g]:
+                    lat.append(pl.wait(g))
+            return lat
+
+        run()  # warm-up pass
+        t0 = time.perf_counter()
+        lat = run()
+        wall = time.perf_counter() - t0
+        per_event = np.repeat(np.array(lat), group_size)
+        cnt = [pl.counters(g, k) for g in range(n_groups) for k in range(group_size)]
+        return {"workload": f"{n_events} pu200 events (seeds 1001..{1000 + n_events}), mean {hits.mean():.0f} hits "
+                            f"(max {hits.max()}), mixed-precision fit",
+                "events_in_flight": group_size * n_groups, "events_per_graph": group_size, "graphs_in_flight": n_groups,
+                "events_per_s": round(n_sub * group_size / wall, 1),
+                "latency_ms_p50": round(float(np.percentile(per_event, 50)), 4),
+                "latency_ms_p99": round(float(np.percentile(per_event, 99)), 4),
+                "max_hits_over_capacity": bool(hits.max() > A.MAX_HITS),
+                "overflow_events": int(sum(1 for c in cnt if c["overflow"]))}
+    finally:
+        batch.free()
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,7 +265,10 @@ def main():
     ap.add_argument("--streams", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-fit", action="store_true", help="skip the fit-only workload")
+    ap.add_argument("--no-fit", action="store_true", help="skip the fit-only, adjoint and PU200 workloads")
+    ap.add_argument("--only", default="all", choices=("all", "pu50", "fit", "adjoint", "pu200", "copy"),
+                    help="run one workload alone (for profiling); 'copy' is a plain 256 MiB device copy used "
+                         "to scale the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the pixel-track stage shards by event with no exchange; "
@@ -213,6 +278,25 @@ def main():
     from pixeltrack import ca, generator as G
 
     frames = G.frames()
+    if args.only in ("fit", "adjoint", "pu200", "copy"):
+        if args.only == "copy":
+            import torch
+            a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(5):
+                b.copy_(a)
+            torch.cuda.synchronize()
+            res = {"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "copies": 5}
+        elif args.only == "pu200":
+            res = pu200_stream(frames)
+        else:
+            c = ca.Context(max_events=1, frames=frames)
+            res = fit_only(c) if args.only == "fit" else adjoint_only(c)
+            c.close()
+        print(json.dumps({"workload": args.only, "result": res}))
+        return
+    if args.only == "pu50":
+        args.no_fit = True
     seed0 = 1 if args.preset == "pu50" else 1001
     events = [G.generate_event(args.preset, seed=seed0 + i) for i in range(args.events)]
     print(f"[bench] generated {len(events)} {args.preset} events, mean hits "
@@ -252,7 +336,10 @@ def main():
     avg = {k: v / n_inst for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
     dom_ms = avg[dom]
-    roof = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None, traffic=None,
+    # traffic: measured HBM bytes per launch of the same kernel and workload
+    # (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled, profiles/hbm_traffic_pu50_batch100.md)
+    roof = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
+                traffic=profiled_traffic(dom) if args.events == 100 and args.preset == "pu50" else None,
                 avg_launch_ms=round(dom_ms, 5))
     b = kernel_bytes(dom, cnt)
     if b is not None and dom_ms > 0:
@@ -260,12 +347,14 @@ def main():
         roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b)
     kernels_ms = {k: round(v, 4) for k, v in sorted(avg.items(), key=lambda kv: -kv[1])}
 
-    fit = adj = None
+    fit = adj = stream = None
     if not args.no_fit:
         fit = fit_only(ctx)
         print(f"[bench] fit-only: {fit}", file=sys.stderr, flush=True)
         adj = adjoint_only(ctx)
         print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+        stream = pu200_stream(frames)
+        print(f"[bench] pu200 stream: {stream}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -298,6 +387,7 @@ def main():
         "streams": args.streams,
         "fit_only": fit,
         "adjoint": adj,
+        "pu200_stream": stream,
     }
     print(json.dumps(line))
     batch.free()

@@ -502,6 +502,14 @@ template <class P, int N> struct BLFit {
             lcv[0][0] = cv[0][0];
             lcv[0][1] = cv[0][1];
             lcv[1][1] = cv[1][1];
+            // the line's outputs are final here: storing them now frees their
+            // registers for the circle fit
+            o.state[3] = (Q)lpar0;
+            o.state[4] = (Q)lpar1;
+            o.cov[12] = (Q)lcv[0][0];
+            o.cov[13] = (Q)lcv[0][1];
+            o.cov[14] = (Q)lcv[1][1];
+            o.eta = asinh_r((Q)lpar0);
             lchi2 = S_(0.0);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
@@ -686,8 +694,6 @@ template <class P, int N> struct BLFit {
         o.state[0] = (Q)cpar[0];
         o.state[1] = (Q)cpar[1];
         o.state[2] = (Q)(cpar2_d * (D)(1.0 / B));
-        o.state[3] = (Q)lpar0;
-        o.state[4] = (Q)lpar1;
         o.cov[0] = ccov[0][0];
         o.cov[1] = ccov[0][1];
         o.cov[2] = b * ccov[0][2];
@@ -700,11 +706,7 @@ template <class P, int N> struct BLFit {
         o.cov[9] = (b * b) * ccov[2][2];
         o.cov[10] = Q(0.0);
         o.cov[11] = Q(0.0);
-        o.cov[12] = (Q)lcv[0][0];
-        o.cov[13] = (Q)lcv[0][1];
-        o.cov[14] = (Q)lcv[1][1];
         o.pt = (Q)(B / r_abs(cpar2_d));
-        o.eta = asinh_r((Q)lpar0);
         o.chi2_line = (Q)lchi2;
         o.chi2_circle = (Q)cchi2;
         o.chi2 = (Q)((lchi2 + cchi2) / (S_)(2 * N - 5));

@@ -373,6 +373,7 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
         k -= kHead;
         if (k < e.n_hits) {
             e.s.ohc_cursor[k] = 0u;
+            e.s.h2t_cursor[k] = 0u;
             const float z = e.zg[k], r = e.rg[k];
             e.s.hxyzr[k] = make_float4(e.xg[k], e.yg[k], z, r);
             e.s.hdbl[k] = make_uint4(__float_as_uint(z), __float_as_uint(r),

@@ -19,6 +19,7 @@ namespace pt {
 void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32_t max_hits, uint32_t max_slots,
                int zero_state, hipStream_t st, Marks *tm);
 void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm);
+void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm);
 void launch_fit_flat(const FlatArgs &a, int precision, const FitLaunch &L, hipStream_t st);
 void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hipStream_t st);
 void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, const float *frames, float *ge,
@@ -103,6 +104,12 @@ size_t carve(char *base, pt::Scratch *s) {
     t.hdbl = (uint4 *)take(kMaxHits * 16);
     t.binrec = (uint4 *)take(kMaxHits * 16);
     t.binxy = (float2 *)take(kMaxHits * 8);
+    t.dc_slot = (uint32_t *)take(kMaxCells * 4);
+    t.dc_cells = (uint32_t *)take(kDcCap * 4);
+    t.dc_off = (uint32_t *)take(kDcCap * 4);
+    t.dc_cur = (uint32_t *)take(kDcCap * 4);
+    t.dc_done = (uint32_t *)take(kDcCap * 4);
+    t.dc_ent = (uint32_t *)take(4u * kMaxTuples * 4);
     if (s) *s = t;
     return off;
 }
@@ -155,6 +162,14 @@ struct pt_ctx {
     uint32_t *fit_lists = nullptr;
     size_t fit_lists_cap = 0;  // track ids
     uint32_t *fit_counts = nullptr;
+    int fit_precision = PT_FIT_FP64;  // precision of the stage's fit
+    // pipeline: group g = event slots [g*pl_size, (g+1)*pl_size) on stream sub[g],
+    // its launch sequence captured once into a graph
+    uint32_t pl_size = 0, pl_groups = 0;
+    hipGraph_t pl_graph[kMaxSub] = {};
+    hipGraphExec_t pl_exec[kMaxSub] = {};
+    hipEvent_t pl_t0[kMaxSub] = {}, pl_t1[kMaxSub] = {};
+    bool pl_pending[kMaxSub] = {};
 };
 
 extern "C" {
@@ -266,6 +281,10 @@ int pt_ctx_destroy(pt_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < pt_ctx::kMaxSub; ++k) {
         if (c->sub[k]) (void)hipStreamSynchronize(c->sub[k]);
+        if (c->pl_exec[k]) (void)hipGraphExecDestroy(c->pl_exec[k]);
+        if (c->pl_graph[k]) (void)hipGraphDestroy(c->pl_graph[k]);
+        if (c->pl_t0[k]) (void)hipEventDestroy(c->pl_t0[k]);
+        if (c->pl_t1[k]) (void)hipEventDestroy(c->pl_t1[k]);
         if (c->sub[k]) (void)hipStreamDestroy(c->sub[k]);
         if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
     }
@@ -339,6 +358,7 @@ static pt::Config make_config(const pt_ctx *c, double b_field) {
     cfg.p = c->p;
     cfg.B = (double)(float)b_field;  // HelixFitOnGPU(Float32(b_field)) then Float64
     cfg.frames = c->d_frames;
+    cfg.fit_precision = c->fit_precision;
     std::memset(cfg.pairs.by_inner, 0, sizeof(cfg.pairs.by_inner));
     std::memset(cfg.pairs.pair_of, -1, sizeof(cfg.pairs.pair_of));
     for (int p = 0; p < 19; ++p) cfg.pairs.pair_of[kLP[2 * p]][kLP[2 * p + 1]] = (int8_t)p;
@@ -385,6 +405,7 @@ static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_t
         }
         pt::launch_ca(c->d_evs + e0, e1 - e0, cfg, max_hits, max_slots, out_zeroed ? 0 : 1, st, marks);
         pt::launch_fit_tuples(c->d_evs + e0, e1 - e0, cfg, st, marks);
+        pt::launch_classify(c->d_evs + e0, e1 - e0, cfg, st, marks);
     };
     if (n_sub == 1) {
         launch_range(0, n, c->stream);
@@ -613,6 +634,101 @@ int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
             d->trk_off[i + 1] = pos;
         }
     }
+    return PT_OK;
+}
+
+int pt_ctx_set_fit_precision(pt_ctx *c, int precision) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (c->pl_groups) return fail(PT_ERR_ARG, "the pipeline's graphs are captured: set the precision before pt_pipeline_init");
+    c->fit_precision = precision;
+    return PT_OK;
+}
+
+int pt_pipeline_init(pt_ctx *c, uint32_t group_size, uint32_t n_groups, double b_field, int out_zeroed) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
+    if (group_size == 0 || n_groups == 0 || n_groups > (uint32_t)pt_ctx::kMaxSub)
+        return fail(PT_ERR_ARG, "group_size must be >= 1 and n_groups 1..%d", pt_ctx::kMaxSub);
+    if ((uint64_t)group_size * n_groups > c->max_events)
+        return fail(PT_ERR_ARG, "%u groups of %u events exceed the context's %u slots", n_groups, group_size,
+                    c->max_events);
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipDeviceSynchronize());
+    for (int g = 0; g < pt_ctx::kMaxSub; ++g) {
+        if (c->pl_exec[g]) (void)hipGraphExecDestroy(c->pl_exec[g]);
+        if (c->pl_graph[g]) (void)hipGraphDestroy(c->pl_graph[g]);
+        c->pl_exec[g] = nullptr;
+        c->pl_graph[g] = nullptr;
+        c->pl_pending[g] = false;
+    }
+    c->pl_size = c->pl_groups = 0;
+    const pt::Config cfg = make_config(c, b_field);
+    for (uint32_t g = 0; g < n_groups; ++g) {
+        if (!c->pl_t0[g]) HIP_OK(hipEventCreate(&c->pl_t0[g]));
+        if (!c->pl_t1[g]) HIP_OK(hipEventCreate(&c->pl_t1[g]));
+        hipStream_t st = c->sub[g];
+        const uint32_t e0 = g * group_size;
+        // grids sized for the capacity limits, so any event of the stream fits the captured shape
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        pt::launch_ca(c->d_evs + e0, group_size, cfg, PT_MAX_HITS, pt::kMaxSlots, out_zeroed ? 0 : 1, st, nullptr);
+        pt::launch_fit_tuples(c->d_evs + e0, group_size, cfg, st, nullptr);
+        pt::launch_classify(c->d_evs + e0, group_size, cfg, st, nullptr);
+        hipError_t e1 = hipMemcpyAsync(c->h_cnt + e0, c->d_cnt + e0, sizeof(pt_counters) * group_size,
+                                       hipMemcpyDeviceToHost, st);
+        hipError_t e2 = hipStreamEndCapture(st, &c->pl_graph[g]);
+        if (e1 != hipSuccess || e2 != hipSuccess)
+            return fail(PT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        HIP_OK(hipGraphInstantiate(&c->pl_exec[g], c->pl_graph[g], nullptr, nullptr, 0));
+    }
+    c->pl_size = group_size;
+    c->pl_groups = n_groups;
+    return PT_OK;
+}
+
+int pt_pipeline_submit(pt_ctx *c, uint32_t group, const pt_hits_view *hits, const pt_track_soa *out) {
+    if (!c || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
+    if (group >= c->pl_groups) return fail(PT_ERR_ARG, "group %u not in the pipeline (%u groups)", group, c->pl_groups);
+    HIP_OK(hipSetDevice(c->device));
+    // the descriptors of this group's previous submission must have been copied
+    if (c->pl_pending[group]) HIP_OK(hipEventSynchronize(c->pl_t1[group]));
+    c->pl_pending[group] = false;
+    const uint32_t n = c->pl_size, e0 = group * n;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t ns, np;
+        int rc = build_desc(c, e0 + i, hits[i], out[i], ns, np);
+        if (rc) return rc;
+    }
+    hipStream_t st = c->sub[group];
+    HIP_OK(hipMemcpyAsync(c->d_evs + e0, c->h_evs + e0, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipEventRecord(c->pl_t0[group], st));
+    HIP_OK(hipGraphLaunch(c->pl_exec[group], st));
+    HIP_OK(hipEventRecord(c->pl_t1[group], st));
+    c->pl_pending[group] = true;
+    return PT_OK;
+}
+
+int pt_pipeline_wait(pt_ctx *c, uint32_t group, float *ms) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (group >= c->pl_groups) return fail(PT_ERR_ARG, "group %u not in the pipeline (%u groups)", group, c->pl_groups);
+    HIP_OK(hipSetDevice(c->device));
+    if (!c->pl_pending[group]) return fail(PT_ERR_ARG, "group %u has nothing in flight", group);
+    HIP_OK(hipEventSynchronize(c->pl_t1[group]));
+    c->pl_pending[group] = false;
+    if (ms) HIP_OK(hipEventElapsedTime(ms, c->pl_t0[group], c->pl_t1[group]));
+    for (uint32_t i = 0; i < c->pl_size; ++i) {
+        const pt_counters &k = c->h_cnt[group * c->pl_size + i];
+        if (k.internal_error)
+            return fail(PT_ERR_CAPACITY, "group %u event %u: internal limit hit (PT_INT_* bits 0x%x)", group, i,
+                        k.internal_error);
+    }
+    return PT_OK;
+}
+
+int pt_pipeline_get_counters(pt_ctx *c, uint32_t group, uint32_t event, pt_counters *cnt) {
+    if (!c || !cnt) return fail(PT_ERR_ARG, "null pointer");
+    if (group >= c->pl_groups || event >= c->pl_size) return fail(PT_ERR_ARG, "no such pipeline event");
+    *cnt = c->h_cnt[group * c->pl_size + event];
     return PT_OK;
 }
 

@@ -35,6 +35,7 @@ constexpr uint32_t kDblAgg = (kMaxSlots + kSlotTile - 1) / kSlotTile;
 constexpr uint32_t kDfsTile = 1024;               // cells per prefix-sum tile of the ntuplet scan
 constexpr uint32_t kDfsAgg = (kMaxCells + kDfsTile - 1) / kDfsTile;
 constexpr uint32_t kTupBuf = 2u * PT_MAX_TUPLES;   // tuples kept between the ntuplet count and emit
+constexpr uint32_t kDcCap = 2u * PT_MAX_TUPLES;    // cells on >= 2 stored tuples (<= 4 cells per tuple / 2)
 
 // Layer pairs (gpu_pixel_doublets.jl:20-28) and per-pair cuts (:29-56)
 struct PairTables {
@@ -86,6 +87,13 @@ struct Scratch {
     PT_G uint4 *binrec;          // [kMaxHits]     per histogram position: (z, r, iphi | det << 16,
                             //                ysize | hit << 16) of the hit stored there
     PT_G float2 *binxy;          // [kMaxHits]     per histogram position: (x, y)
+    // classification: cells on >= 2 stored tuples ("shared cells") and their tuple lists
+    PT_G uint32_t *dc_slot;      // [kMaxCells]    cell -> shared-cell slot, ~0 if not shared
+    PT_G uint32_t *dc_cells;     // [kDcCap]       slot -> cell
+    PT_G uint32_t *dc_off;       // [kDcCap]       slot -> start of its list in dc_ent
+    PT_G uint32_t *dc_cur;       // [kDcCap]       slot -> list length
+    PT_G uint32_t *dc_done;      // [kDcCap]       round in which the duplicate remover resolved it
+    PT_G uint32_t *dc_ent;       // [4*kMaxTuples] tuple ids (0-based)
 };
 
 constexpr uint8_t kUsed = 1, kKilled = 2;
@@ -103,6 +111,8 @@ enum Misc : int {
     kMiscTkConn = 10,     // ... of connect
     kMiscTkFish = 11,     // ... of the fishbone
     kMiscTupBuf = 12,     // entries of tup_buf
+    kMiscDc = 13,         // shared cells (classification)
+    kMiscDcEnt = 14,      // entries of their tuple lists
 };
 
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,
@@ -133,6 +143,7 @@ struct Config {
     double B;                 // Float64(Float32(b_field))
     PT_G const float *frames;      // [1856*9]
     PairTables pairs;
+    int fit_precision;        // PT_FIT_FP64 (the reference's) or PT_FIT_MIXED
 };
 
 // Fit-only batch (pre-gathered tracks, track-minor layout of pixeltrack.h).

@@ -37,7 +37,7 @@ __device__ __forceinline__ uint32_t fit_flag_bits(uint32_t f) {
 }
 
 // The fits of one length bucket, items idx0, idx0 + stride, ...
-template <int N>
+template <class P, int N>
 __device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, int bucket, uint32_t idx0,
                                            uint32_t stride, uint32_t &ref, uint32_t &nfit) {
     const uint32_t b0 = e.s.tm_off[bucket - 1], b1 = e.s.tm_off[bucket];
@@ -64,8 +64,9 @@ __device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, i
             hy[k] = e.yg[h];
             hz[k] = e.zg[h];
         }
-        FitOut<double> o;
-        bl_fit_fp64<N>(hx, hy, hz, ge, cfg.B, o);
+        FitOut<typename P::Prop> o;
+        BLFit<P, N> f;
+        f.run(hx, hy, hz, ge, cfg.B, o);
         ref |= fit_flag_bits(o.flags);
         const uint32_t row = tkid - 1u;
 #pragma unroll
@@ -81,6 +82,7 @@ __device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, i
 
 // All three length buckets in one launch (3 -> N=3, 4 -> N=4, 5 -> N=4 on the
 // first four hits when fit_5_as_4): the event's blocks are split in three.
+template <class P>
 __global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
@@ -89,11 +91,11 @@ __global__ void __launch_bounds__(256) k_fit_tuples(const EvDesc *__restrict__ e
     const uint32_t bk = b / per, bb = b - bk * per;
     uint32_t ref = 0, nfit = 0;
     if (bk == 0)
-        fit_bucket<3>(e, cfg, 3, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
+        fit_bucket<P, 3>(e, cfg, 3, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
     else if (bk == 1)
-        fit_bucket<4>(e, cfg, 4, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
+        fit_bucket<P, 4>(e, cfg, 4, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
     else if (bk == 2 && cfg.p.fit_5_as_4)
-        fit_bucket<4>(e, cfg, 5, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
+        fit_bucket<P, 4>(e, cfg, 5, bb * blockDim.x + threadIdx.x, per * blockDim.x, ref, nfit);
     uint32_t v = nfit;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -339,7 +341,10 @@ void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg
     if (per > target) per = target;
     if (per < 1) per = 1;
     const EvGrid grid{n_events, 3u * per};
-    k_fit_tuples<<<ev_grid_dim(grid), 256, 0, st>>>(d_evs, grid, cfg);
+    if (cfg.fit_precision == PT_FIT_MIXED)
+        k_fit_tuples<PolicyMixed><<<ev_grid_dim(grid), 256, 0, st>>>(d_evs, grid, cfg);
+    else
+        k_fit_tuples<PolicyFP64><<<ev_grid_dim(grid), 256, 0, st>>>(d_evs, grid, cfg);
     if (tm) tm->record("k_fit_tuples", st);
 }
 

@@ -88,6 +88,9 @@ class Context:
         A.check(self.lib.pt_get_counters(self.h, event, C.byref(c)), "pt_get_counters")
         return c.as_dict()
 
+    def set_fit_precision(self, precision):
+        A.check(self.lib.pt_ctx_set_fit_precision(self.h, precision), "pt_ctx_set_fit_precision")
+
     def set_streams(self, n):
         A.check(self.lib.pt_ctx_set_streams(self.h, n), "pt_ctx_set_streams")
 
@@ -250,6 +253,34 @@ class DeviceBatch:
         for p in self.ptrs:
             self.ctx.free(p)
         self.ptrs = []
+
+
+class Pipeline:
+    """Events in flight under captured hipGraphs (pt_pipeline_*): groups of
+    `group_size` device-resident events, each group replayed on its own stream."""
+
+    def __init__(self, ctx: Context, group_size, n_groups, b_field=A.B_FIELD_INV_GEV, out_zeroed=True):
+        self.ctx, self.group_size, self.n_groups = ctx, group_size, n_groups
+        A.check(ctx.lib.pt_pipeline_init(ctx.h, group_size, n_groups, b_field, 1 if out_zeroed else 0),
+                "pt_pipeline_init")
+
+    def submit(self, group, batch, first):
+        """Enqueue events [first, first + group_size) of a DeviceBatch as group `group`."""
+        n = self.group_size
+        views = (A.PtHitsView * n)(*batch.views[first:first + n])
+        outs = (A.PtTrackSoA * n)(*batch.outs[first:first + n])
+        A.check(self.ctx.lib.pt_pipeline_submit(self.ctx.h, group, views, outs), "pt_pipeline_submit")
+
+    def wait(self, group):
+        ms = C.c_float()
+        A.check(self.ctx.lib.pt_pipeline_wait(self.ctx.h, group, C.byref(ms)), "pt_pipeline_wait")
+        return ms.value
+
+    def counters(self, group, event):
+        c = A.PtCounters()
+        A.check(self.ctx.lib.pt_pipeline_get_counters(self.ctx.h, group, event, C.byref(c)),
+                "pt_pipeline_get_counters")
+        return c.as_dict()
 
 
 class FitBatch:

@@ -227,6 +227,28 @@ int pt_get_ca_debug(pt_ctx *ctx, uint32_t event, pt_ca_debug *dbg);
  * the reference runs events on host threads (EventProcessor.jl:43). */
 int pt_ctx_set_streams(pt_ctx *ctx, int n_streams);
 
+/* Precision of the stage's fit: PT_FIT_FP64 (default, the reference's) or
+ * PT_FIT_MIXED (see pt_fit_bl_device_async).  Set before pt_pipeline_init. */
+int pt_ctx_set_fit_precision(pt_ctx *ctx, int precision);
+
+/* ---------------------------------------------------------------------------
+ * Events in flight under captured hipGraphs (no reference counterpart: the
+ * reference runs one event per host thread, EventProcessor.jl:43).  Group g
+ * (0 <= g < n_groups <= 8) owns the context's event slots
+ * [g*group_size, (g+1)*group_size) and a stream of its own; the stage's launch
+ * sequence for one group (grids sized for the capacity limits) is captured
+ * once into a hipGraph and replayed for every submission.  Needs
+ * max_events >= group_size * n_groups and the module frames.
+ * pt_pipeline_submit enqueues exactly group_size device-resident events and
+ * returns; it first waits for the group's previous submission.
+ * pt_pipeline_wait blocks until the group is done; *ms = GPU time of the
+ * replay (HIP events on the group's stream).
+ * ------------------------------------------------------------------------- */
+int pt_pipeline_init(pt_ctx *ctx, uint32_t group_size, uint32_t n_groups, double b_field, int out_zeroed);
+int pt_pipeline_submit(pt_ctx *ctx, uint32_t group, const pt_hits_view *hits, const pt_track_soa *out);
+int pt_pipeline_wait(pt_ctx *ctx, uint32_t group, float *ms);
+int pt_pipeline_get_counters(pt_ctx *ctx, uint32_t group, uint32_t event, pt_counters *cnt);
+
 /* Per-kernel timing of the last batch, milliseconds, from HIP events recorded
  * on the context stream after every kernel when enabled (enable=1 before the
  * batch; a timed batch runs on one stream).  *n_stages: capacity of ms/names
