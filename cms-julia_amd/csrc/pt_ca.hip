@@ -373,7 +373,9 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
         k -= kHead;
         if (k < e.n_hits) {
             e.s.ohc_cursor[k] = 0u;
-            e.s.h2t_cursor[k] = 0u;
+            e.s.h2t_cnt[k] = 0u;
+            e.s.h2t_max[k] = 0u;
+            e.s.h2t_win[k] = ~0ull;
             const float z = e.zg[k], r = e.rg[k];
             e.s.hxyzr[k] = make_float4(e.xg[k], e.yg[k], z, r);
             e.s.hdbl[k] = make_uint4(__float_as_uint(z), __float_as_uint(r),
@@ -1396,6 +1398,16 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
                         const uint32_t before = atomicAdd(&e.s.trk[cq].x, 1u);
                         atomicMax(&e.s.trk[cq].y, len + 1u);
                         if (before == 0u) ++lists;
+                        if (before == 1u) {  // the cell now carries two tuples: a shared cell
+                            const uint32_t sl = atomicAdd(&e.s.misc[kMiscDc], 1u);
+                            if (sl < kDcCap) {
+                                e.s.dc_cells[sl] = cq;
+                                e.s.dc_slot[cq] = sl;
+                            } else {
+                                e.s.dc_slot[cq] = ~0u;
+                                err |= PT_INT_OVF_LIST;
+                            }
+                        }
                         if (before == PT_MAX_CELL_TRACKS) {  // list capped: exact path
                             ovf |= PT_OVF_CELL_TRACKS;
                             const uint32_t k = atomicAdd(&e.s.misc[kMiscTrkOvf], 1u);
@@ -1563,6 +1575,22 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
             if (o) atomicOr(&e.cnt->overflow, o);
         }
         __syncthreads();
+    }
+    {
+        // offsets of the shared cells' tuple lists (classification), in slot order
+        const uint32_t nd = min(e.s.misc[kMiscDc], kDcCap);
+        const uint32_t dch = (nd + blockDim.x - 1) / blockDim.x;
+        const uint32_t d0 = min(nd, threadIdx.x * dch), d1 = min(nd, d0 + dch);
+        uint32_t sum = 0;
+        for (uint32_t d = d0; d < d1; ++d) sum += e.s.trk[e.s.dc_cells[d]].x;
+        uint32_t total;
+        uint32_t run = block_excl_scan(sum, lds[0], total);
+        for (uint32_t d = d0; d < d1; ++d) {
+            e.s.dc_off[d] = run;
+            run += e.s.trk[e.s.dc_cells[d]].x;
+            e.s.dc_cur[d] = 0u;
+            e.s.dc_done[d] = 0u;
+        }
     }
     const uint32_t chunk = (m + blockDim.x - 1) / blockDim.x;
     const uint32_t t0 = threadIdx.x * chunk, t1 = min(m, t0 + chunk);

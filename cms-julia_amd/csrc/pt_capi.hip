@@ -96,9 +96,9 @@ size_t carve(char *base, pt::Scratch *s) {
     t.tuple_cells = (uint32_t *)take(kMaxTuples * 4 * 4);
     t.tm_off = (uint32_t *)take(16 * 4);
     t.tm_bins = (uint16_t *)take(kMaxTuples * 2);
-    t.h2t_off = (uint32_t *)take((kMaxHits + 2) * 4);
-    t.h2t_bins = (uint16_t *)take(4u * 49152u * 2);
-    t.h2t_cursor = (uint32_t *)take(kMaxHits * 4);
+    t.h2t_cnt = (uint32_t *)take(kMaxHits * 4);
+    t.h2t_max = (uint32_t *)take(kMaxHits * 4);
+    t.h2t_win = (uint64_t *)take((size_t)kMaxHits * 8);
     t.misc = (uint32_t *)take(16 * 4);
     t.hxyzr = (float4 *)take(kMaxHits * 16);
     t.hdbl = (uint4 *)take(kMaxHits * 16);
@@ -109,6 +109,7 @@ size_t carve(char *base, pt::Scratch *s) {
     t.dc_off = (uint32_t *)take(kDcCap * 4);
     t.dc_cur = (uint32_t *)take(kDcCap * 4);
     t.dc_done = (uint32_t *)take(kDcCap * 4);
+    t.dc_pred = (uint4 *)take((size_t)kDcCap * 16);
     t.dc_ent = (uint32_t *)take(4u * kMaxTuples * 4);
     if (s) *s = t;
     return off;

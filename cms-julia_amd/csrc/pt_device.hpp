@@ -78,9 +78,9 @@ struct Scratch {
     PT_G uint32_t *tuple_cells;  // [kMaxTuples*4] cells of each stored tuple
     PT_G uint32_t *tm_off;       // [16]           tuple multiplicity (9 used)
     PT_G uint16_t *tm_bins;      // [kMaxTuples]
-    PT_G uint32_t *h2t_off;      // [kMaxHits+2]   hit -> loose tuples (classification)
-    PT_G uint16_t *h2t_bins;     // [4*PT_MAX_TUPLES... 196608]
-    PT_G uint32_t *h2t_cursor;   // [kMaxHits]
+    PT_G uint32_t *h2t_cnt;      // [kMaxHits]     loose tuples on the hit (classification)
+    PT_G uint32_t *h2t_max;      // [kMaxHits]     their largest hit count
+    PT_G uint64_t *h2t_win;      // [kMaxHits]     min over them of (|tip| bits << 32 | ~tuple id)
     PT_G uint32_t *misc;         // [16]           internal scalars (see Misc)
     PT_G float4 *hxyzr;          // [kMaxHits]     packed (x, y, z, r) per hit
     PT_G uint4 *hdbl;            // [kMaxHits]     packed (z, r, iphi | det << 16, ysize) per hit
@@ -88,11 +88,12 @@ struct Scratch {
                             //                ysize | hit << 16) of the hit stored there
     PT_G float2 *binxy;          // [kMaxHits]     per histogram position: (x, y)
     // classification: cells on >= 2 stored tuples ("shared cells") and their tuple lists
-    PT_G uint32_t *dc_slot;      // [kMaxCells]    cell -> shared-cell slot, ~0 if not shared
+    PT_G uint32_t *dc_slot;      // [kMaxCells]    cell -> shared-cell slot (valid when trk.x >= 2)
     PT_G uint32_t *dc_cells;     // [kDcCap]       slot -> cell
     PT_G uint32_t *dc_off;       // [kDcCap]       slot -> start of its list in dc_ent
     PT_G uint32_t *dc_cur;       // [kDcCap]       slot -> list length
     PT_G uint32_t *dc_done;      // [kDcCap]       round in which the duplicate remover resolved it
+    PT_G uint4 *dc_pred;         // [kDcCap]       up to 4 earlier shared cells on its tuples (slots)
     PT_G uint32_t *dc_ent;       // [4*kMaxTuples] tuple ids (0-based)
 };
 
@@ -112,7 +113,7 @@ enum Misc : int {
     kMiscTkFish = 11,     // ... of the fishbone
     kMiscTupBuf = 12,     // entries of tup_buf
     kMiscDc = 13,         // shared cells (classification)
-    kMiscDcEnt = 14,      // entries of their tuple lists
+    kMiscH2tEnt = 14,     // hit -> loose-tuple entries (classification)
 };
 
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,

@@ -3,7 +3,8 @@
 // k_fit_tuples: the fit stage of make_tuples (launchBrokenLineKernelsOnCPU,
 //   broken_line_fit_on_gpu.jl:166-203): for each length bucket (3 -> N=3,
 //   4 -> N=4, 5 -> N=4 on the first four hits, fit5as4) fit the first 24,576
-//   entries of the tuple-multiplicity bucket, i.e. the highest tuple ids.
+//   entries of the tuple-multiplicity bucket, i.e. the highest tuple ids, and
+//   classify each fitted tuple (the first step of classify_tuples).
 //   Gather per hit: global position, module frame, local variances rotated to
 //   the global covariance in float (SOARotation.jl:139-147).
 // k_fit_flat: fit of pre-gathered tracks (fit-only workloads).
@@ -69,13 +70,36 @@ __device__ __forceinline__ void fit_bucket(const EvDesc &e, const Config &cfg, i
         f.run(hx, hy, hz, ge, cfg.B, o);
         ref |= fit_flag_bits(o.flags);
         const uint32_t row = tkid - 1u;
+        float st[5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) e.state[k * kMaxTuples + row] = (float)o.state[k];
+        for (int k = 0; k < 5; ++k) {
+            st[k] = (float)o.state[k];
+            e.state[k * kMaxTuples + row] = st[k];
+        }
 #pragma unroll
         for (int k = 0; k < 15; ++k) e.cov[k * kMaxTuples + row] = (float)o.cov[k];
-        e.pt[row] = (float)o.pt;
+        const float ptv = (float)o.pt, chi2 = (float)o.chi2;
+        e.pt[row] = ptv;
         e.eta[row] = (float)o.eta;
-        e.chi2[row] = (float)o.chi2;
+        e.chi2[row] = chi2;
+        // kernel_classify_tracks (ca_hit_ntuplet_generator_kernels_impl.jl:213-247)
+        // on the stored float values: the tuple is not dup (the buckets hold none)
+        // and has `bucket` hits
+        bool is_nan = false;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) is_nan |= isnan(st[k]);
+        if (!is_nan) {
+            const float pt = isnan(ptv) ? ptv : (cfg.p.chi2_max_pt < ptv ? cfg.p.chi2_max_pt : ptv);
+            const float *cf = cfg.p.chi2_coeff;
+            const float chi2_cut = cfg.p.chi2_scale * (cf[0] + pt * (cf[1] + pt * (cf[2] + pt * cf[3])));
+            if (!(3.0f * chi2 >= chi2_cut)) {
+                const bool quad = bucket > 3;
+                const float max_tip = quad ? cfg.p.quadruplet_max_tip : cfg.p.triplet_max_tip;
+                const float min_pt = quad ? cfg.p.quadruplet_min_pt : cfg.p.triplet_min_pt;
+                const float max_zip = quad ? cfg.p.quadruplet_max_zip : cfg.p.triplet_max_zip;
+                if (fabsf(st[1]) < max_tip && ptv > min_pt && fabsf(st[4]) < max_zip) e.quality[row] = PT_QUALITY_LOOSE;
+            }
+        }
         ++nfit;
     }
 }
