@@ -25,6 +25,7 @@ void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hip
 void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, const float *frames, float *ge,
                             hipStream_t st);
 void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
+void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, hipStream_t st);
 }  // namespace pt
 
 namespace {
@@ -111,6 +112,8 @@ size_t carve(char *base, pt::Scratch *s) {
     t.dc_done = (uint32_t *)take(kDcCap * 4);
     t.dc_pred = (uint4 *)take((size_t)kDcCap * 16);
     t.dc_ent = (uint32_t *)take(4u * kMaxTuples * 4);
+    t.hist_off = (uint32_t *)take(PT_HIST_NOFF * 4);
+    t.hist_bins = (uint16_t *)take(kMaxHits * 2);
     if (s) *s = t;
     return off;
 }
@@ -143,7 +146,16 @@ struct pt_ctx {
     pt_counters *d_cnt = nullptr;
     pt_counters *h_cnt = nullptr;  // pinned
     pt::EvDesc *d_evs = nullptr;
-    pt::EvDesc *h_evs = nullptr;  // pinned
+    // host descriptors (pinned): a ring of buffers for the eager batches, so a
+    // batch can be prepared while the copy of an earlier one is still queued
+    // (a slot is reused once its copy has run), and one for the pipeline
+    static constexpr int kDescRing = 4;
+    pt::EvDesc *h_ring[kDescRing] = {};
+    hipEvent_t ev_ring[kDescRing] = {};
+    bool ring_pending[kDescRing] = {};
+    int ring_next = 0;
+    pt::EvDesc *pl_h_evs = nullptr;
+    pt::EvDesc *h_evs = nullptr;  // the descriptors of the last eager batch (one of h_ring)
     Staging stg;
     pt::Marks marks;
     bool timing = false;
@@ -252,7 +264,7 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
     if (hipMalloc((void **)&c->d_cnt, sizeof(pt_counters) * max_events) != hipSuccess ||
         hipHostMalloc((void **)&c->h_cnt, sizeof(pt_counters) * max_events, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&c->d_evs, sizeof(pt::EvDesc) * max_events) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_evs, sizeof(pt::EvDesc) * max_events, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->pl_h_evs, sizeof(pt::EvDesc) * max_events, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&c->d_frames, sizeof(float) * 9 * PT_N_MODULES) != hipSuccess ||
         hipMalloc((void **)&c->fit_counts, 4 * sizeof(uint32_t)) != hipSuccess)
         return cleanup(fail(PT_ERR_HIP, "context allocation failed"));
@@ -263,6 +275,12 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         for (int k = 0; k < 3; ++k) c->tuning[k].blocks = 4u * (uint32_t)c->n_cu;
     }
     std::memset(c->h_cnt, 0, sizeof(pt_counters) * max_events);
+    for (int k = 0; k < pt_ctx::kDescRing; ++k)
+        if (hipHostMalloc((void **)&c->h_ring[k], sizeof(pt::EvDesc) * max_events, hipHostMallocDefault) !=
+                hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_ring[k], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(PT_ERR_HIP, "descriptor buffer allocation failed"));
+    c->h_evs = c->h_ring[0];
     for (int k = 0; k < pt::Marks::kMax; ++k)
         if (hipEventCreate(&c->marks.ev[k]) != hipSuccess) return cleanup(fail(PT_ERR_HIP, "hipEventCreate failed"));
     c->have_marks = true;
@@ -296,7 +314,11 @@ int pt_ctx_destroy(pt_ctx *c) {
     (void)hipFree(c->d_cnt);
     (void)hipHostFree(c->h_cnt);
     (void)hipFree(c->d_evs);
-    (void)hipHostFree(c->h_evs);
+    for (int k = 0; k < pt_ctx::kDescRing; ++k) {
+        if (c->h_ring[k]) (void)hipHostFree(c->h_ring[k]);
+        if (c->ev_ring[k]) (void)hipEventDestroy(c->ev_ring[k]);
+    }
+    (void)hipHostFree(c->pl_h_evs);
     (void)hipFree(c->d_frames);
     (void)hipFree(c->stg.block);
     (void)hipFree(c->fit_lists);
@@ -315,9 +337,21 @@ int pt_ctx_set_frames(pt_ctx *c, const float *rot, uint32_t n_modules) {
     return PT_OK;
 }
 
-static int build_desc(pt_ctx *c, uint32_t i, const pt_hits_view &h, const pt_track_soa &o, uint32_t &n_slots,
-                      uint32_t &n_pairs) {
-    pt::EvDesc &e = c->h_evs[i];
+// Checks the phi-histogram inputs of one event: layer starts from 0, sorted,
+// ending at most at n_hits (the histogram covers hits below layer_start[10]).
+static int check_layer_start(uint32_t i, const pt_hits_view &h) {
+    if (h.n_hits > PT_MAX_HITS)
+        return fail(PT_ERR_CAPACITY, "event %u has %u hits, above MAX_NUMBER_OF_HITS = %u", i, h.n_hits, PT_MAX_HITS);
+    if (h.layer_start[0] != 0 || h.layer_start[10] > h.n_hits)
+        return fail(PT_ERR_ARG, "event %u: layer_start must start at 0 and end at most at n_hits", i);
+    for (int L = 0; L < 10; ++L)
+        if (h.layer_start[L + 1] < h.layer_start[L]) return fail(PT_ERR_ARG, "event %u: layer_start not sorted", i);
+    return PT_OK;
+}
+
+// Descriptor of event i (scratch slot i) into e.
+static int build_desc(pt_ctx *c, pt::EvDesc &e, uint32_t i, const pt_hits_view &h, const pt_track_soa &o,
+                      uint32_t &n_slots, uint32_t &n_pairs) {
     std::memset(&e, 0, sizeof(e));
     if (h.n_hits > PT_MAX_HITS)
         return fail(PT_ERR_CAPACITY, "event %u has %u hits, above MAX_NUMBER_OF_HITS = %u", i, h.n_hits, PT_MAX_HITS);
@@ -325,15 +359,20 @@ static int build_desc(pt_ctx *c, uint32_t i, const pt_hits_view &h, const pt_tra
         return fail(PT_ERR_ARG, "event %u: layer_start must start at 0 and end at n_hits", i);
     for (int L = 0; L < 10; ++L)
         if (h.layer_start[L + 1] < h.layer_start[L]) return fail(PT_ERR_ARG, "event %u: layer_start not sorted", i);
-    if (h.n_hits && (!h.xg || !h.yg || !h.zg || !h.rg || !h.xerr || !h.yerr || !h.iphi || !h.ysize || !h.det_ind ||
-                     !h.hist_off || !h.hist_bins))
+    if (h.n_hits && (!h.xg || !h.yg || !h.zg || !h.rg || !h.xerr || !h.yerr || !h.iphi || !h.ysize || !h.det_ind))
         return fail(PT_ERR_ARG, "event %u: null hit array", i);
+    if (!h.hist_off != !h.hist_bins)
+        return fail(PT_ERR_ARG, "event %u: give both histogram arrays, or neither to build it on the device", i);
     if (!o.quality || !o.chi2 || !o.eta || !o.pt || !o.state || !o.covariance || !o.hit_off || !o.hit_bins ||
         !o.det_off || !o.det_bins)
         return fail(PT_ERR_ARG, "event %u: null TrackSOA array", i);
     e.xg = h.xg, e.yg = h.yg, e.zg = h.zg, e.rg = h.rg, e.xerr = h.xerr, e.yerr = h.yerr;
     e.iphi = h.iphi, e.ysize = h.ysize, e.det = h.det_ind;
     e.hist_off = h.hist_off, e.hist_bins = h.hist_bins;
+    if (!h.hist_off) {
+        e.hist_off = e.hist_off_build = c->scratch[i].hist_off;
+        e.hist_bins = e.hist_bins_build = c->scratch[i].hist_bins;
+    }
     e.n_hits = h.n_hits;
     for (int L = 0; L < 11; ++L) e.layer_start[L] = h.layer_start[L];
     n_pairs = 19;
@@ -382,23 +421,56 @@ static pt::Config make_config(const pt_ctx *c, double b_field) {
     return cfg;
 }
 
+// The next host descriptor buffer of the ring, once the copy that last read
+// it has run.
+static int next_desc_buffer(pt_ctx *c, pt::EvDesc *&h) {
+    const int k = c->ring_next;
+    if (c->ring_pending[k]) HIP_OK(hipEventSynchronize(c->ev_ring[k]));
+    c->ring_pending[k] = false;
+    h = c->h_ring[k];
+    return PT_OK;
+}
+
+// Copies n descriptors of the current ring buffer to the device (stream
+// order: after the kernels of the previous batch, which read d_evs).  A
+// stage batch's descriptors stay readable for pt_get_ca_debug.
+static int upload_descs(pt_ctx *c, pt::EvDesc *h, uint32_t n, bool stage) {
+    const int k = c->ring_next;
+    HIP_OK(hipMemcpyAsync(c->d_evs, h, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipEventRecord(c->ev_ring[k], c->stream));
+    c->ring_pending[k] = true;
+    c->ring_next = (k + 1) % pt_ctx::kDescRing;
+    if (stage) c->h_evs = h;
+    return PT_OK;
+}
+
 static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_track_soa *outs, double b_field,
                      int out_zeroed) {
     if (!c->frames_set) return fail(PT_ERR_ARG, "module frames not set (pt_ctx_set_frames)");
     if (n == 0) return PT_OK;
     if (n > c->max_events) return fail(PT_ERR_ARG, "%u events exceed the context's %u slots", n, c->max_events);
     std::vector<uint32_t> nslots(n);
+    pt::EvDesc *h;
+    int rc = next_desc_buffer(c, h);
+    if (rc) return rc;
+    bool any_hist = false;
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t np;
-        int rc = build_desc(c, i, hits[i], outs[i], nslots[i], np);
+        rc = build_desc(c, h[i], i, hits[i], outs[i], nslots[i], np);
         if (rc) return rc;
+        any_hist |= h[i].hist_bins_build != nullptr;
     }
-    HIP_OK(hipMemcpyAsync(c->d_evs, c->h_evs, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, c->stream));
+    rc = upload_descs(c, h, n, true);
+    if (rc) return rc;
     const pt::Config cfg = make_config(c, b_field);
     pt::Marks *marks = c->timing ? &c->marks : nullptr;
     if (marks) marks->n = 0;
     const uint32_t n_sub = marks ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)c->n_sub, n));
     auto launch_range = [&](uint32_t e0, uint32_t e1, hipStream_t st) {
+        if (any_hist) {
+            pt::launch_phi_hist(c->d_evs + e0, e1 - e0, st);
+            if (marks) marks->record("k_phi_hist", st);
+        }
         uint32_t max_hits = 0, max_slots = 0;
         for (uint32_t i = e0; i < e1; ++i) {
             max_hits = std::max(max_hits, hits[i].n_hits);
@@ -431,6 +503,42 @@ int pt_make_tuples_device_async(pt_ctx *c, uint32_t n_events, const pt_hits_view
     if (!c || (n_events && (!hits || !out))) return fail(PT_ERR_ARG, "null pointer");
     HIP_OK(hipSetDevice(c->device));
     return run_batch(c, n_events, hits, out, b_field, out_zeroed);
+}
+
+int pt_fill_phi_histogram_async(pt_ctx *c, uint32_t n_events, const pt_hits_view *hits, uint32_t *const *hist_off,
+                                uint16_t *const *hist_bins) {
+    if (!c || (n_events && (!hits || !hist_off || !hist_bins))) return fail(PT_ERR_ARG, "null pointer");
+    if (n_events > c->max_events)
+        return fail(PT_ERR_ARG, "%u events exceed the context's %u slots", n_events, c->max_events);
+    if (n_events == 0) return PT_OK;
+    HIP_OK(hipSetDevice(c->device));
+    pt::EvDesc *h;
+    int rc = next_desc_buffer(c, h);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n_events; ++i) {
+        rc = check_layer_start(i, hits[i]);
+        if (rc) return rc;
+        if ((hits[i].layer_start[10] && !hits[i].iphi) || !hist_off[i] || (hits[i].layer_start[10] && !hist_bins[i]))
+            return fail(PT_ERR_ARG, "event %u: null array", i);
+        pt::EvDesc &e = h[i];
+        std::memset(&e, 0, sizeof(e));
+        e.iphi = hits[i].iphi;
+        e.n_hits = hits[i].n_hits;
+        for (int L = 0; L < 11; ++L) e.layer_start[L] = hits[i].layer_start[L];
+        e.hist_off_build = hist_off[i];
+        e.hist_bins_build = hist_bins[i];
+    }
+    rc = upload_descs(c, h, n_events, false);
+    if (rc) return rc;
+    pt::launch_phi_hist(c->d_evs, n_events, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+void pt_set_hits_layer_start(const uint32_t *hits_module_start, uint32_t layer_start[11]) {
+    // phase1PixelTopology.jl:24-36
+    static const uint32_t kModuleLayerStart[11] = {0, 96, 320, 672, 1184, 1296, 1408, 1520, 1632, 1744, 1856};
+    for (int L = 0; L < 11; ++L) layer_start[L] = hits_module_start[kModuleLayerStart[L]];
 }
 
 // After the batch has completed: an event whose kernels hit one of this
@@ -510,13 +618,15 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
         HIP_OK(hipMemcpyAsync(s.iphi, hits->iphi, n * 2, hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(s.ysize, hits->ysize, n * 2, hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(s.det, hits->det_ind, n * 2, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.hist_bins, hits->hist_bins, n * 2, hipMemcpyHostToDevice, st));
+        if (hits->hist_bins) HIP_OK(hipMemcpyAsync(s.hist_bins, hits->hist_bins, n * 2, hipMemcpyHostToDevice, st));
     }
     if (hits->hist_off)
         HIP_OK(hipMemcpyAsync(s.hist_off, hits->hist_off, PT_HIST_NOFF * 4, hipMemcpyHostToDevice, st));
     pt_hits_view dv = *hits;
     dv.xg = s.xg, dv.yg = s.yg, dv.zg = s.zg, dv.rg = s.rg, dv.xerr = s.xerr, dv.yerr = s.yerr;
-    dv.iphi = s.iphi, dv.ysize = s.ysize, dv.det_ind = s.det, dv.hist_off = s.hist_off, dv.hist_bins = s.hist_bins;
+    dv.iphi = s.iphi, dv.ysize = s.ysize, dv.det_ind = s.det;
+    dv.hist_off = hits->hist_off ? s.hist_off : nullptr;  // none given: built on the device
+    dv.hist_bins = hits->hist_bins ? s.hist_bins : nullptr;
     pt_track_soa dout;
     dout.quality = s.quality, dout.chi2 = s.chi2, dout.eta = s.eta, dout.pt = s.pt, dout.state = s.state;
     dout.covariance = s.cov, dout.hit_off = s.hit_off, dout.hit_bins = s.hit_bins, dout.det_off = s.det_off;
@@ -672,6 +782,7 @@ int pt_pipeline_init(pt_ctx *c, uint32_t group_size, uint32_t n_groups, double b
         const uint32_t e0 = g * group_size;
         // grids sized for the capacity limits, so any event of the stream fits the captured shape
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        pt::launch_phi_hist(c->d_evs + e0, group_size, st);  // returns at once for events with a histogram
         pt::launch_ca(c->d_evs + e0, group_size, cfg, PT_MAX_HITS, pt::kMaxSlots, out_zeroed ? 0 : 1, st, nullptr);
         pt::launch_fit_tuples(c->d_evs + e0, group_size, cfg, st, nullptr);
         pt::launch_classify(c->d_evs + e0, group_size, cfg, st, nullptr);
@@ -697,11 +808,11 @@ int pt_pipeline_submit(pt_ctx *c, uint32_t group, const pt_hits_view *hits, cons
     const uint32_t n = c->pl_size, e0 = group * n;
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t ns, np;
-        int rc = build_desc(c, e0 + i, hits[i], out[i], ns, np);
+        int rc = build_desc(c, c->pl_h_evs[e0 + i], e0 + i, hits[i], out[i], ns, np);
         if (rc) return rc;
     }
     hipStream_t st = c->sub[group];
-    HIP_OK(hipMemcpyAsync(c->d_evs + e0, c->h_evs + e0, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->d_evs + e0, c->pl_h_evs + e0, sizeof(pt::EvDesc) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipEventRecord(c->pl_t0[group], st));
     HIP_OK(hipGraphLaunch(c->pl_exec[group], st));
     HIP_OK(hipEventRecord(c->pl_t1[group], st));

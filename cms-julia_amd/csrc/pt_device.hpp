@@ -95,6 +95,9 @@ struct Scratch {
     PT_G uint32_t *dc_done;      // [kDcCap]       round in which the duplicate remover resolved it
     PT_G uint4 *dc_pred;         // [kDcCap]       up to 4 earlier shared cells on its tuples (slots)
     PT_G uint32_t *dc_ent;       // [4*kMaxTuples] tuple ids (0-based)
+    // phi histogram built on the device when the caller gives none
+    PT_G uint32_t *hist_off;     // [1281]
+    PT_G uint16_t *hist_bins;    // [kMaxHits]
 };
 
 constexpr uint8_t kUsed = 1, kKilled = 2;
@@ -128,6 +131,10 @@ struct EvDesc {
     uint32_t n_slots;         // sum of inner-layer sizes over the pairs
     uint32_t layer_start[11];
     uint32_t slot_cum[20];    // inclusive cumulative inner-layer sizes per pair
+    // when set, the phi histogram is built here from iphi and layer_start
+    // first (hist_off / hist_bins then point to the same arrays)
+    PT_G uint32_t *hist_off_build;
+    PT_G uint16_t *hist_bins_build;
     // outputs (TrackSOA)
     PT_G int32_t *quality;
     PT_G float *chi2, *eta, *pt, *state, *cov;

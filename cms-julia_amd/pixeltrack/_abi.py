@@ -184,7 +184,8 @@ HIP_SYMBOLS = [
     "pt_malloc", "pt_free", "pt_memcpy_h2d", "pt_memcpy_d2h", "pt_memset", "pt_fit_bl_sched_async",
     "pt_fit_set_tuning", "pt_fit_get_tuning", "pt_fit_autotune", "pt_hit_errors_to_global_async",
     "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async", "pt_ctx_set_fit_precision", "pt_pipeline_init",
-    "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters",
+    "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
+    "pt_set_hits_layer_start",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -247,6 +248,10 @@ def hip_lib():
         lib.pt_pipeline_submit.argtypes = [vp, C.c_uint32, C.POINTER(PtHitsView), C.POINTER(PtTrackSoA)]
         lib.pt_pipeline_wait.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float)]
         lib.pt_pipeline_get_counters.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(PtCounters)]
+        lib.pt_fill_phi_histogram_async.argtypes = [vp, C.c_uint32, C.POINTER(PtHitsView), C.POINTER(vp),
+                                                    C.POINTER(vp)]
+        lib.pt_set_hits_layer_start.argtypes = [c_u32p, c_u32p]
+        lib.pt_set_hits_layer_start.restype = None
         lib.pt_malloc.argtypes = [vp, C.POINTER(vp), C.c_size_t]
         lib.pt_free.argtypes = [vp, vp]
         lib.pt_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]

@@ -206,7 +206,9 @@ class DeviceBatch:
                   ("hit_off", np.uint32, A.MAX_TUPLES + 1), ("hit_bins", np.uint16, A.MAX_TUPLE_HITS),
                   ("det_off", np.uint32, A.MAX_TUPLES + 1), ("det_bins", np.uint16, A.MAX_TUPLE_HITS))
 
-    def __init__(self, ctx: Context, events):
+    def __init__(self, ctx: Context, events, device_hist=False):
+        """device_hist: leave the phi histogram out (NULL), so the stage
+        builds it on the device from iphi and layer_start."""
         self.ctx = ctx
         self.n = len(events)
         self.events = events
@@ -216,6 +218,8 @@ class DeviceBatch:
         for i, h in enumerate(events):
             v = self.views[i]
             for name, dt in self.HIT_FIELDS:
+                if device_hist and name in ("hist_off", "hist_bins"):
+                    continue
                 arr = np.ascontiguousarray(getattr(h, name), dt)
                 p = ctx.malloc(max(arr.nbytes, 4))
                 self.ptrs.append(p)
@@ -253,6 +257,54 @@ class DeviceBatch:
         for p in self.ptrs:
             self.ctx.free(p)
         self.ptrs = []
+
+
+def fill_phi_histogram(ctx: Context, events):
+    """The phi histogram of each event built on the device
+    (pt_fill_phi_histogram_async): returns [(hist_off[1281], hist_bins[n])]."""
+    n = len(events)
+    views = (A.PtHitsView * n)()
+    offs = (C.c_void_p * n)()
+    bins = (C.c_void_p * n)()
+    ptrs = []
+    try:
+        for i, h in enumerate(events):
+            iphi = np.ascontiguousarray(h.iphi, np.int16)
+            p = ctx.malloc(max(iphi.nbytes, 4))
+            ptrs.append(p)
+            if iphi.nbytes:
+                ctx.h2d(p, iphi)
+            views[i].iphi = C.cast(C.c_void_p(p), type(views[i].iphi))
+            views[i].n_hits = h.n_hits
+            for L in range(11):
+                views[i].layer_start[L] = int(h.layer_start[L])
+            offs[i] = ctx.malloc(4 * 1281)
+            bins[i] = ctx.malloc(max(2 * int(h.layer_start[10]), 4))
+            ptrs += [offs[i], bins[i]]
+        A.check(ctx.lib.pt_fill_phi_histogram_async(ctx.h, n, views, offs, bins), "pt_fill_phi_histogram_async")
+        ctx.sync()
+        res = []
+        for i, h in enumerate(events):
+            off = np.zeros(1281, np.uint32)
+            b = np.zeros(int(h.layer_start[10]), np.uint16)
+            ctx.d2h(off, offs[i])
+            if b.size:
+                ctx.d2h(b, bins[i])
+            res.append((off, b))
+        return res
+    finally:
+        for p in ptrs:
+            ctx.free(p)
+
+
+def hits_layer_start(hits_module_start):
+    """setHitsLayerStart (PixelRecHits.jl:20-27) through the library."""
+    hms = np.ascontiguousarray(hits_module_start, np.uint32)
+    if hms.size < 1857:
+        raise ValueError("hits_module_start needs 1857 entries")
+    out = np.zeros(11, np.uint32)
+    A.hip_lib().pt_set_hits_layer_start(A.ptr(hms, C.c_uint32), A.ptr(out, C.c_uint32))
+    return out
 
 
 class Pipeline:

@@ -118,14 +118,17 @@ typedef struct pt_params {
 
 /* Hits of one event: TrackingRecHit2DSOAView (TrackingRecHit2DSOAView.jl:35-54)
  * plus the phi histogram `Hist` (TrackingRecHit2DHeterogeneous.jl:28) and the
- * layer starts.  Arrays have n_hits entries (hist_off has 1281). */
+ * layer starts.  Arrays have n_hits entries (hist_off has 1281).
+ * hist_off and hist_bins both NULL: the stage builds the histogram on the
+ * device from iphi and layer_start first, as makeHits does on the host
+ * (PixelRecHits.jl:46-48), into the context's event scratch. */
 typedef struct pt_hits_view {
     const float    *xg, *yg, *zg, *rg;   /* global position (cm), rg = sqrt(xg^2+yg^2) in f32 */
     const float    *xerr, *yerr;         /* LOCAL variances (gpuPixelRecHits.jl:303-306)      */
     const int16_t  *iphi;                /* unsafe_atan2s(yg, xg, 7)                          */
     const int16_t  *ysize;               /* cluster size y                                    */
     const int16_t  *det_ind;             /* 0-based module index                              */
-    const uint32_t *hist_off;            /* [1281] exclusive bin offsets                      */
+    const uint32_t *hist_off;            /* [1281] exclusive bin offsets, or NULL (see below) */
     const uint16_t *hist_bins;           /* [n_hits] 1-based hit ids, descending in a bin     */
     uint32_t        n_hits;
     uint32_t        layer_start[11];     /* host copy of hits_layer_start (0-based)           */
@@ -218,6 +221,23 @@ int pt_make_tuples_device_async(pt_ctx *ctx, uint32_t n_events, const pt_hits_vi
                                 double b_field, const pt_track_soa *out, int out_zeroed);
 int pt_ctx_sync(pt_ctx *ctx);
 int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
+
+/* The phi histogram of n events on the device: fill_many_from_vector
+ * (hist_to_container.jl:265-270) as called by makeHits (PixelRecHits.jl:46-48)
+ * over hits[i].iphi (device memory) with hits[i].layer_start as the offsets,
+ * hits below layer_start[10] (the searchsortedfirst layer lookup, including
+ * its filing of the first hit after an empty layer into that layer).
+ * hist_off[i] (device, 1281 entries) gets the exclusive bin starts and
+ * hist_bins[i] (device, layer_start[10] entries) the 1-based hit ids in
+ * descending order inside each bin.  Only iphi, n_hits and layer_start of
+ * hits[i] are read.  Enqueued on the context's stream. */
+int pt_fill_phi_histogram_async(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits,
+                                uint32_t *const *hist_off, uint16_t *const *hist_bins);
+
+/* setHitsLayerStart (PixelRecHits.jl:20-27): layer_start[L] =
+ * hits_module_start[first module of layer L] for the phase-1 layer table
+ * (phase1PixelTopology.jl:24-36); hits_module_start has 1857 entries. Host only. */
+void pt_set_hits_layer_start(const uint32_t *hits_module_start, uint32_t layer_start[11]);
 
 /* Internal CA state of event `event` of the last batch (device -> host copy). */
 int pt_get_ca_debug(pt_ctx *ctx, uint32_t event, pt_ca_debug *dbg);

@@ -3,6 +3,8 @@
 The reference ships no golden outputs for this path, so the oracle is pinned
 by independent recomputation on small events:
   * histogram: fill_many_from_vector invariants (his_to_container_cpu.jl:39-107)
+    and a hit-by-hit restatement with numpy's searchsorted, on events with
+    empty layers
   * doublets: brute force over ALL hit pairs of each layer pair with the cut
     predicates, against the oracle's phi-window walk (set and serial order)
   * connect: brute force over all cell pairs sharing a hit
@@ -57,6 +59,69 @@ def test_histogram_invariants():
             assert np.all((i0 >= h.layer_start[layer]) & (i0 < h.layer_start[layer + 1]))
             t = h.iphi[i0].astype(np.int64)
             assert np.all(((t & 0xFFFF) >> 9) == pb)
+
+
+def py_phi_hist(iphi, layer_start):
+    """Independent restatement of fill_many_from_vector (hist_to_container.jl:217-270):
+    the layer by searchsortedfirst (numpy's searchsorted, side='left', is the same
+    search), then count, inclusive scan and the decrementing fill, hit by hit."""
+    ls = np.asarray(layer_start, np.int64)
+    off = np.zeros(1281, np.int64)
+    n = int(ls[10])
+    layer = []
+    for i in range(n):
+        k = int(np.searchsorted(ls, i, side="left")) + 1  # 1-based searchsortedfirst
+        if k <= 11 and ls[k - 1] == i:
+            k += 1
+        layer.append(k - 2)
+    b = [((int(iphi[i]) & 0xFFFF) >> 9) + 128 * layer[i] for i in range(n)]  # 0-based bin
+    for i in range(n):
+        off[b[i]] += 1
+    off = np.cumsum(off)
+    bins = np.zeros(n, np.int64)
+    for i in range(n):
+        w = off[b[i]]
+        off[b[i]] -= 1
+        bins[w - 1] = i + 1
+    return off, bins, layer
+
+
+def test_histogram_empty_layer_quirk():
+    """A layer with no hits files the first hit of the next layer into its own
+    bins (searchsortedfirst finds the empty layer's offset first); two empty
+    layers in a row and an empty first layer do the same."""
+    rng = np.random.default_rng(5)
+    import ctypes as C
+    from pixeltrack import _abi as A
+    for ls in ([0, 10, 10, 25, 40, 52, 60, 71, 80, 95, 110], [0, 0, 12, 12, 12, 30, 44, 50, 50, 61, 70],
+               [0, 9, 20, 31, 42, 53, 64, 75, 86, 97, 97]):
+        ls = np.asarray(ls, np.uint32)
+        n = int(ls[10])
+        iphi = rng.integers(-32768, 32768, n).astype(np.int16)
+        roff, rbins = np.zeros(1281, np.uint32), np.zeros(n, np.uint16)
+        assert O.lib().oracle_build_phi_hist(A.ptr(iphi, C.c_int16), n, A.ptr(ls, C.c_uint32),
+                                             A.ptr(roff, C.c_uint32), A.ptr(rbins, C.c_uint16)) == 0
+        off, bins, layer = py_phi_hist(iphi, ls)
+        np.testing.assert_array_equal(roff, off)
+        np.testing.assert_array_equal(rbins, bins)
+        for L in range(10):
+            if ls[L] == ls[L + 1] and ls[L] < n and (L == 0 or ls[L - 1] < ls[L]):
+                assert layer[int(ls[L])] == L  # misfiled into the (first) empty layer
+        assert off[0] == 0 and off[-1] == n
+
+
+def test_hits_layer_start_from_module_start():
+    """setHitsLayerStart (PixelRecHits.jl:20-27) with the phase-1 module table
+    (phase1PixelTopology.jl:24-36)."""
+    import ctypes as C
+    from pixeltrack import _abi as A
+    from pixeltrack import ca
+    hms = np.cumsum(np.r_[0, np.random.default_rng(2).integers(0, 9, 1856)]).astype(np.uint32)
+    got = ca.hits_layer_start(hms)
+    first_module = [0, 96, 320, 672, 1184, 1296, 1408, 1520, 1632, 1744, 1856]
+    np.testing.assert_array_equal(got, hms[first_module])
+    with pytest.raises(ValueError):
+        ca.hits_layer_start(hms[:100])
 
 
 def brute_doublets(h):
