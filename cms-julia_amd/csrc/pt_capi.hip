@@ -26,6 +26,7 @@ void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, co
                             hipStream_t st);
 void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
 void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, hipStream_t st);
+void launch_load_tracks(const EvDesc *d_evs, uint32_t n_events, float pt_min, hipStream_t st);
 }  // namespace pt
 
 namespace {
@@ -531,6 +532,35 @@ int pt_fill_phi_histogram_async(pt_ctx *c, uint32_t n_events, const pt_hits_view
     rc = upload_descs(c, h, n_events, false);
     if (rc) return rc;
     pt::launch_phi_hist(c->d_evs, n_events, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
+}
+
+int pt_load_tracks_async(pt_ctx *c, uint32_t n_events, const pt_track_soa *tracks, float pt_min,
+                         const pt_vertex_ws *ws) {
+    if (!c || (n_events && (!tracks || !ws))) return fail(PT_ERR_ARG, "null pointer");
+    if (n_events > c->max_events)
+        return fail(PT_ERR_ARG, "%u events exceed the context's %u slots", n_events, c->max_events);
+    if (n_events == 0) return PT_OK;
+    HIP_OK(hipSetDevice(c->device));
+    pt::EvDesc *h;
+    int rc = next_desc_buffer(c, h);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n_events; ++i) {
+        const pt_track_soa &t = tracks[i];
+        const pt_vertex_ws &w = ws[i];
+        if (!t.quality || !t.pt || !t.state || !t.covariance || !t.det_off)
+            return fail(PT_ERR_ARG, "event %u: null TrackSOA array", i);
+        if (!w.n_tracks || !w.index_track || !w.zt || !w.ezt2 || !w.ptt2)
+            return fail(PT_ERR_ARG, "event %u: null WorkSpace array", i);
+        pt::EvDesc &e = h[i];
+        std::memset(&e, 0, sizeof(e));
+        e.quality = t.quality, e.pt = t.pt, e.state = t.state, e.cov = t.covariance, e.det_off = t.det_off;
+        e.ws_n = w.n_tracks, e.ws_index = w.index_track, e.ws_zt = w.zt, e.ws_ezt2 = w.ezt2, e.ws_ptt2 = w.ptt2;
+    }
+    rc = upload_descs(c, h, n_events, false);
+    if (rc) return rc;
+    pt::launch_load_tracks(c->d_evs, n_events, pt_min, c->stream);
     HIP_OK(hipGetLastError());
     return PT_OK;
 }

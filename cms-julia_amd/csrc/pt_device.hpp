@@ -135,6 +135,10 @@ struct EvDesc {
     // first (hist_off / hist_bins then point to the same arrays)
     PT_G uint32_t *hist_off_build;
     PT_G uint16_t *hist_bins_build;
+    // vertex-finder input (load_tracks) written from the TrackSOA outputs
+    PT_G uint32_t *ws_n;
+    PT_G uint16_t *ws_index;
+    PT_G float *ws_zt, *ws_ezt2, *ws_ptt2;
     // outputs (TrackSOA)
     PT_G int32_t *quality;
     PT_G float *chi2, *eta, *pt, *state, *cov;

@@ -177,6 +177,12 @@ class PtFitTuning(C.Structure):
         return {"block": self.block, "tile": self.tile, "blocks": self.blocks}
 
 
+class PtVertexWs(C.Structure):
+    """pt_vertex_ws: the WorkSpace arrays load_tracks fills (device pointers)."""
+    _fields_ = [("n_tracks", c_u32p), ("index_track", c_u16p), ("zt", c_f32p), ("ezt2", c_f32p),
+                ("ptt2", c_f32p)]
+
+
 HIP_SYMBOLS = [
     "pt_params_default", "pt_ctx_create", "pt_ctx_destroy", "pt_last_error", "pt_device_count",
     "pt_ctx_set_frames", "pt_make_tuples", "pt_make_tuples_device_async", "pt_ctx_sync", "pt_get_counters",
@@ -185,7 +191,7 @@ HIP_SYMBOLS = [
     "pt_fit_set_tuning", "pt_fit_get_tuning", "pt_fit_autotune", "pt_hit_errors_to_global_async",
     "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async", "pt_ctx_set_fit_precision", "pt_pipeline_init",
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
-    "pt_set_hits_layer_start",
+    "pt_set_hits_layer_start", "pt_load_tracks_async",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -252,6 +258,8 @@ def hip_lib():
                                                     C.POINTER(vp)]
         lib.pt_set_hits_layer_start.argtypes = [c_u32p, c_u32p]
         lib.pt_set_hits_layer_start.restype = None
+        lib.pt_load_tracks_async.argtypes = [vp, C.c_uint32, C.POINTER(PtTrackSoA), C.c_float,
+                                             C.POINTER(PtVertexWs)]
         lib.pt_malloc.argtypes = [vp, C.POINTER(vp), C.c_size_t]
         lib.pt_free.argtypes = [vp, vp]
         lib.pt_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]

@@ -297,6 +297,42 @@ def fill_phi_histogram(ctx: Context, events):
             ctx.free(p)
 
 
+def load_tracks(ctx: Context, tracks, pt_min=0.5):
+    """load_tracks of the vertex finder (pt_load_tracks_async) over device
+    TrackSOAs (a ctypes array of PtTrackSoA, e.g. DeviceBatch.outs); returns a
+    dict of host arrays per event."""
+    n = len(tracks)
+    S = A.MAX_TUPLES
+    ws = (A.PtVertexWs * n)()
+    ptrs = []
+    try:
+        for i in range(n):
+            for name, size in (("n_tracks", 4), ("index_track", 2 * S), ("zt", 4 * S), ("ezt2", 4 * S),
+                               ("ptt2", 4 * S)):
+                p = ctx.malloc(size)
+                ptrs.append(p)
+                setattr(ws[i], name, C.cast(C.c_void_p(p), type(getattr(ws[i], name))))
+        A.check(ctx.lib.pt_load_tracks_async(ctx.h, n, tracks, pt_min, ws), "pt_load_tracks_async")
+        ctx.sync()
+        res = []
+        for i in range(n):
+            k = np.zeros(1, np.uint32)
+            ctx.d2h(k, C.cast(ws[i].n_tracks, C.c_void_p).value)
+            m = int(k[0])
+            r = {"n_tracks": m}
+            for name, dt in (("index_track", np.uint16), ("zt", np.float32), ("ezt2", np.float32),
+                             ("ptt2", np.float32)):
+                arr = np.zeros(m, dt)
+                if m:
+                    ctx.d2h(arr, C.cast(getattr(ws[i], name), C.c_void_p).value)
+                r[name] = arr
+            res.append(r)
+        return res
+    finally:
+        for p in ptrs:
+            ctx.free(p)
+
+
 def hits_layer_start(hits_module_start):
     """setHitsLayerStart (PixelRecHits.jl:20-27) through the library."""
     hms = np.ascontiguousarray(hits_module_start, np.uint32)

@@ -234,6 +234,26 @@ int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
 int pt_fill_phi_histogram_async(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits,
                                 uint32_t *const *hist_off, uint16_t *const *hist_bins);
 
+/* Input of the pixel vertex finder for one event: the WorkSpace fields
+ * (gpu_vertex_finder.jl:9-32) that load_tracks fills.  Device arrays of
+ * PT_MAX_TUPLES (= MAX_TRACKS, z_vertex_soa.jl:2) entries. */
+typedef struct pt_vertex_ws {
+    uint32_t *n_tracks;     /* [1] selected tracks                     */
+    uint16_t *index_track;  /* 1-based TrackSOA row of each            */
+    float    *zt;           /* zip (state column 5)                    */
+    float    *ezt2;         /* covariance column 15                    */
+    float    *ptt2;         /* pt^2                                    */
+} pt_vertex_ws;
+
+/* load_tracks (gpu_vertex_finder.jl:51-82), the track selection of
+ * gpuVertexFinder.make (:208-211; pt_min = 0.5 in PixelVertexProducerCUDA,
+ * pixel_vertex_producer_cuda.jl:19), over n device TrackSOAs: rows in order
+ * up to the first row without hits; kept when n_hits >= 4, quality == loose
+ * and !(pt < pt_min).  Enqueued on the context's stream (after a batch of
+ * pt_make_tuples_device_async, in stream order). */
+int pt_load_tracks_async(pt_ctx *ctx, uint32_t n_events, const pt_track_soa *tracks, float pt_min,
+                         const pt_vertex_ws *ws);
+
 /* setHitsLayerStart (PixelRecHits.jl:20-27): layer_start[L] =
  * hits_module_start[first module of layer L] for the phase-1 layer table
  * (phase1PixelTopology.jl:24-36); hits_module_start has 1857 entries. Host only. */

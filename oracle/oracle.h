@@ -23,6 +23,11 @@ int oracle_make_tuples(const pt_params *p, const pt_hits_view *hits, const float
 int oracle_build_phi_hist(const int16_t *iphi, uint32_t n_hits, const uint32_t *layer_start, uint32_t *off,
                           uint16_t *bins);
 
+/* load_tracks (gpu_vertex_finder.jl:51-82) on one host TrackSOA: n = kept
+ * rows, index_track (1-based row), zt, ezt2, ptt2 as in the WorkSpace. */
+int oracle_load_tracks(const pt_track_soa *t, float pt_min, uint32_t *n, uint16_t *index_track, float *zt,
+                       float *ezt2, float *ptt2);
+
 /* FP64 broken-line fit of pre-gathered tracks; same input layout as
  * pt_fit_bl_device_async.  Double outputs: out64[t*23 + k] = state[0..4],
  * cov15[5..19], pt, eta, chi2.  flags[t] = oracle fit flags. */

@@ -895,4 +895,28 @@ int oracle_build_phi_hist(const int16_t *iphi, uint32_t n_hits, const uint32_t *
     return PT_OK;
 }
 
+// load_tracks (gpu_vertex_finder.jl:51-82): n_hits_track is the size of the
+// det_indices bin (PixelTrackHeterogeneous.jl:62); the loop ends at the first
+// row without hits.
+int oracle_load_tracks(const pt_track_soa *t, float pt_min, uint32_t *n, uint16_t *index_track, float *zt,
+                       float *ezt2, float *ptt2) {
+    const size_t S = PT_MAX_TUPLES;
+    uint32_t nt = 0;
+    for (uint32_t idx = 1; idx <= S; ++idx) {
+        const uint32_t n_hits = t->det_off[idx] - t->det_off[idx - 1];
+        if (n_hits == 0) break;
+        if (n_hits < 4) continue;
+        if (t->quality[idx - 1] != PT_QUALITY_LOOSE) continue;
+        const float pt = t->pt[idx - 1];
+        if (pt < pt_min) continue;
+        nt += 1;
+        index_track[nt - 1] = (uint16_t)idx;
+        zt[nt - 1] = t->state[4 * S + idx - 1];
+        ezt2[nt - 1] = t->covariance[14 * S + idx - 1];
+        ptt2[nt - 1] = pt * pt;
+    }
+    *n = nt;
+    return PT_OK;
+}
+
 }  // extern "C"

@@ -33,6 +33,7 @@ def lib():
                                          C.c_int]
         L.oracle_build_phi_hist.argtypes = [A.c_i16p, C.c_uint32, A.c_u32p, A.c_u32p, A.c_u16p]
         L.oracle_fit_bl.argtypes = [C.c_uint32, vp, vp, vp, C.c_double, vp, vp, C.c_int]
+        L.oracle_load_tracks.argtypes = [C.POINTER(A.PtTrackSoA), C.c_float, vp, vp, vp, vp, vp]
         L.oracle_bl_detail.argtypes = [C.c_int, vp, vp, C.c_double, vp, C.c_int]
         L.oracle_fit_opcount.argtypes = [C.c_int, vp, vp, C.c_double, vp]
         L.oracle_fit_grad.argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, vp, vp]
@@ -109,6 +110,19 @@ def make_tuples(hits, frames, b_field=A.B_FIELD_INV_GEV, stop_after=0, debug=Tru
                                   C.byref(cnt), C.byref(dbg.struct) if dbg else None, stop_after)
     assert rc == 0, rc
     return out, cnt.as_dict(), dbg
+
+
+def load_tracks(tracks, pt_min=0.5):
+    """The oracle's load_tracks on a host TrackSOA -> dict of the WorkSpace arrays."""
+    S = A.MAX_TUPLES
+    n = np.zeros(1, np.uint32)
+    idx = np.zeros(S, np.uint16)
+    zt, ezt2, ptt2 = (np.zeros(S, np.float32) for _ in range(3))
+    tv = tracks.view()
+    assert lib().oracle_load_tracks(C.byref(tv), pt_min, n.ctypes.data, idx.ctypes.data, zt.ctypes.data,
+                                    ezt2.ctypes.data, ptt2.ctypes.data) == 0
+    k = int(n[0])
+    return {"n_tracks": k, "index_track": idx[:k], "zt": zt[:k], "ezt2": ezt2[:k], "ptt2": ptt2[:k]}
 
 
 def fit_bl(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):
