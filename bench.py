@@ -249,7 +249,11 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
                 "latency_ms_p50": round(float(np.percentile(per_event, 50)), 4),
                 "latency_ms_p99": round(float(np.percentile(per_event, 99)), 4),
                 "max_hits_over_capacity": bool(hits.max() > A.MAX_HITS),
-                "overflow_events": int(sum(1 for c in cnt if c["overflow"]))}
+                # the counters of the last replay of each group (the last 32 events): PU200 events pass the
+                # reference's doublet cap (458,752), which truncates exactly as max_num_of_doublets does
+                "last_events_with_overflow": int(sum(1 for c in cnt if c["overflow"])),
+                "last_events_overflow_bits": hex(int(np.bitwise_or.reduce([c["overflow"] for c in cnt]))),
+                "last_events_doublet_cap_hit": int(sum(1 for c in cnt if c["overflow"] & 0x1))}
     finally:
         batch.free()
         ctx.close()

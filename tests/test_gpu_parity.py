@@ -15,7 +15,8 @@ from pixeltrack import ca, generator as G
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2)]
+# pu200 passes the doublet cap (458,752): the truncation must keep the same cells
+CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2), ("pu200", 1001)]
 FIT_RTOL, FIT_ATOL = 2e-6, 1e-12
 COUNTER_KEYS = ("n_hits", "n_inner_slots", "n_cells", "n_cells_found", "n_connections", "n_connect_pairs",
                 "n_killed", "n_tuples", "n_tuple_hits", "n_by_len", "n_fitted", "n_loose", "overflow", "ref_assert")
@@ -48,6 +49,8 @@ def diff_report(a, b):
 @pytest.mark.parametrize("preset,seed", CASES)
 def test_stage_parity(gen_gpu, frames, preset, seed):
     hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
+    if preset == "pu200":
+        assert rcnt["overflow"] & A.OVF["DOUBLETS"] and rcnt["n_cells_found"] > rcnt["n_cells"] == A.MAX_DOUBLETS
     errors = []
     for k in COUNTER_KEYS:
         if gcnt[k] != rcnt[k]:
