@@ -351,6 +351,33 @@ def main():
         roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b)
     kernels_ms = {k: round(v, 4) for k, v in sorted(avg.items(), key=lambda kv: -kv[1])}
 
+    # the same stage building the phi histogram on the device first (no
+    # histogram handed in), and the host-buffer entry point (PCIe included)
+    dev_hist = host_path = None
+    if args.only == "all":
+        hb = ca.DeviceBatch(ctx, events, device_hist=True)
+        hb.run_async(out_zeroed=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            hb.run_async(out_zeroed=True)
+        ctx.sync()
+        w_h = time.perf_counter() - t0
+        hb.free()
+        dev_hist = {"events_per_s": round(len(events) * args.steps / w_h, 1),
+                    "ms_per_step": round(w_h * 1e3 / args.steps, 4)}
+        gen = ca.CAHitNtupletGeneratorOnGPU(frames)
+        gen.make_tuples(events[0])
+        n_host = min(len(events), 50)
+        t0 = time.perf_counter()
+        for h in events[:n_host]:
+            gen.make_tuples(h)
+        w_p = time.perf_counter() - t0
+        host_path = {"events_per_s": round(n_host / w_p, 1), "events": n_host,
+                     "note": "pt_make_tuples: host hits in, host TrackSOA out, one event per call, synchronous"}
+        gen.ctx.close()
+        print(f"[bench] device histogram: {dev_hist}; host buffers: {host_path}", file=sys.stderr, flush=True)
+
     fit = adj = stream = None
     if not args.no_fit:
         fit = fit_only(ctx)
@@ -382,7 +409,7 @@ def main():
                         f"mean {np.mean(cnt['n_hits']):.0f} hits, {np.mean(cnt['n_cells']):.0f} doublets, "
                         f"{np.mean(cnt['n_tuples']):.0f} tuples, {np.mean(cnt['n_fitted']):.0f} fits per event",
             "events_per_step": len(events),
-            "parallelism": "events batched in one launch sequence (blockIdx.y = event)",
+            "parallelism": "events batched in one launch sequence; an event's blocks on one XCD",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -392,6 +419,8 @@ def main():
         "fit_only": fit,
         "adjoint": adj,
         "pu200_stream": stream,
+        "stage_with_device_histogram": dev_hist,
+        "host_buffers_pcie_inclusive": host_path,
     }
     print(json.dumps(line))
     batch.free()

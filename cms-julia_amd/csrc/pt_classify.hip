@@ -122,22 +122,26 @@ __global__ void __launch_bounds__(256) k_dc_prepare(const EvDesc *__restrict__ e
 }
 
 // K-C3, one block per event: kernel_fast_duplicate_remover in rounds; the
-// round that resolved a shared cell is its stamp (0: not yet).
+// round that resolved a shared cell is its stamp (0: not yet).  The count of
+// cells still pending after round r is s_pending[r % 3]: thread 0 clears the
+// counter of round r + 1 during round r, when every wave has passed the
+// barrier that follows its read of that counter in round r - 2, so no wave
+// can read a counter that was cleared under it.
 __global__ void __launch_bounds__(1024) k_dup_remover(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint32_t s_done[kDoneLds];
-    __shared__ uint32_t s_pending;
+    __shared__ uint32_t s_pending[3];
     uint32_t ev, blk;
     if (!ev_block(grid, ev, blk)) return;
     const EvDesc &e = evs[ev];
     const uint32_t nd = n_shared(e);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     for (uint32_t s = tid; s < min(nd, kDoneLds); s += nthr) s_done[s] = 0u;
+    if (tid == 0) s_pending[1] = 0u;
     // the stamps beyond kDoneLds were cleared by the multiplicity kernel
     auto done_of = [&](uint32_t s) -> uint32_t { return s < kDoneLds ? s_done[s] : e.s.dc_done[s]; };
     __syncthreads();
     for (uint32_t round = 1;; ++round) {
-        if (tid == 0) s_pending = 0u;
-        __syncthreads();
+        if (tid == 0) s_pending[(round + 1u) % 3u] = 0u;
         uint32_t pending = 0;
         for (uint32_t s = tid; s < nd; s += nthr) {
             if (done_of(s)) continue;
@@ -195,9 +199,13 @@ __global__ void __launch_bounds__(1024) k_dup_remover(const EvDesc *__restrict__
             else
                 e.s.dc_done[s] = round;
         }
-        if (pending) atomicAdd(&s_pending, pending);
+        if (pending) atomicAdd(&s_pending[round % 3u], pending);
         __syncthreads();
-        if (s_pending == 0u) break;
+        if (s_pending[round % 3u] == 0u) break;
+        if (round >= nd) {  // every round resolves at least one cell: not reached
+            if (tid == 0) atomicOr(&e.cnt->internal_error, PT_INT_ROUNDS);
+            break;
+        }
     }
 }
 

@@ -79,6 +79,7 @@ extern "C" {
 #define PT_INT_LOOKBACK        0x001u /* a prefix-scan wait exceeded its spin bound */
 #define PT_INT_NBR_POOL        0x002u /* neighbour pool (lists longer than 4) full   */
 #define PT_INT_OVF_LIST        0x004u /* internal overflow list full                 */
+#define PT_INT_ROUNDS          0x008u /* duplicate-remover rounds exceeded the shared cells */
 
 /* Quality enum (track_quality.jl:129-136) */
 #define PT_QUALITY_BAD         0
