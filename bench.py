@@ -25,11 +25,12 @@ sys.path.insert(0, str(ROOT / "cms-julia_amd"))
 METRIC = "events/s CA+fit at PU50 (1 GPU); FP32 fit+adjoint tracks/s vs FP64 fit @1e-4"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
-# Flops of one FP64 broken-line fit (add/sub + mul + div + sqrt), counted by
-# running the fit restatement with an op-counting scalar; transcendental calls
-# are excluded.  N=3 and N=4 hits used.
-FIT_FLOPS = {3: None, 4: None}
+# Flops of one broken-line fit (add/sub + mul + div + sqrt), counted by running
+# the fit restatement with an op-counting scalar (tests/test_oracle_fit.py pins
+# the counts); transcendental calls are excluded.  N = hits used (5 -> 4).
+FIT_FLOPS = {3: 1468, 4: 2007}
 
 
 def kernel_bytes(name, c):
@@ -149,6 +150,14 @@ def fit_only(ctx, n_tracks=1 << 20, steps=20):
                 out[f"{pname}_{sname}_ms"] = round(dt * 1e3, 4)
         out["mixed_vs_fp64_persistent"] = round(out["mixed_persistent_tracks_per_s"] /
                                                 out["fp64_persistent_tracks_per_s"], 3)
+        # vector flop rate of the persistent fits (the mixed fit's flops are partly float, partly
+        # double: its fraction is quoted against the FP32 peak, an upper bound of its ceiling)
+        flops = float(np.mean([FIT_FLOPS[3] if n == 3 else FIT_FLOPS[4] for n in np.asarray(tr["nhit"])]))
+        out["flops_per_track"] = round(flops, 1)
+        for pname, peak in (("fp64", FP64_PEAK_TFLOPS), ("mixed", FP32_PEAK_TFLOPS)):
+            tf = out[f"{pname}_persistent_tracks_per_s"] * flops / 1e12
+            out[f"{pname}_persistent_tflops"] = round(tf, 2)
+            out[f"{pname}_persistent_frac_of_peak"] = round(tf / peak, 4)
         out["persistent_vs_per_length_mixed"] = round(out["mixed_persistent_tracks_per_s"] /
                                                       out["mixed_per_length_tracks_per_s"], 3)
     finally:

@@ -344,3 +344,9 @@ def test_op_counts_are_stable():
     # (add/sub, mul, div, sqrt, transcendental, compare)
     assert sum(counts[4][:4]) > sum(counts[3][:4]) > 300
     assert counts[3][4] > 0 and counts[4][4] > counts[3][4] - 1
+    # the flop figures bench.py quotes the fit's FLOP/s with
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_consts", Path(__file__).resolve().parent.parent / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.FIT_FLOPS == {3: sum(counts[3][:4]), 4: sum(counts[4][:4])}
