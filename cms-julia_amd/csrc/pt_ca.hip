@@ -1060,14 +1060,18 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
 // Kills never feed back into the loop, so it is evaluated per cell pair: with
 // first(ic) = smallest qualifying jc with |jc| < |ic|, ic dies if first(ic)
 // exists, and jc dies if some ic qualifies with it and jc < first(ic).
+// Cells that connect did not use take no part (about a tenth of the cells are
+// used at PU50), so only the used entries of each list are kept, in order.
 // A wavefront takes kHitTile hits per ticket, packs them into groups of at
-// most kFishRows list entries (rows, loaded once into LDS) and spreads every
-// unordered row pair of a group over the lanes; a second step decides each row.
+// most kFishRows list entries, keeps the used ones (rows) in LDS and spreads
+// every unordered row pair of a hit over the lanes; a second step decides
+// each row.
 constexpr uint32_t kFishRows = 128;
 constexpr uint32_t kHitTile = 16;  // hits per ticket
 
 struct FishLds {
-    uint32_t id[kFishRows], ud[kFishRows], first[kFishRows];  // ud: module (16 b) | used << 16
+    uint32_t id[kFishRows], mod[kFishRows], first[kFishRows];
+    uint32_t upre[kFishRows];  // used entries of the group before entry rr
     float dx[kFishRows], dy[kFishRows], dz[kFishRows], n[kFishRows];
     unsigned long long qt[2][kFishRows];  // rows (index in the hit) that qualify with this one as jc
 };
@@ -1110,38 +1114,54 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
             if (rows) {
                 const uint32_t rexcl = rincl - v;
                 wave_sync();  // the previous group's reads of S are done
+                // the group's list entries; the used ones become rows, in order
+                uint32_t n_used = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < kFishRows / 64u; ++u) {
                     const uint32_t rr = 64u * u + lane;
                     const int o = owner_of(rincl, rr);
                     const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    uint2 en = make_uint2(0u, 0u);
+                    bool used = false;
                     if (rr < rows) {
-                        const uint32_t hh = h0 + (uint32_t)o;
-                        const uint2 en = e.s.ohc_list[(size_t)hh * kOhcStride + (rr - s0)];
-                        const float4 po = e.s.hxyzr[hh];
+                        en = e.s.ohc_list[(size_t)(h0 + (uint32_t)o) * kOhcStride + (rr - s0)];
+                        used = (e.s.cell_flags[en.x] & kUsed) != 0;
+                    }
+                    const uint64_t um = __ballot(used);
+                    const uint32_t pos = n_used + (uint32_t)__popcll(um & lanes_below());
+                    S.upre[rr] = pos;
+                    if (used) {
+                        const float4 po = e.s.hxyzr[h0 + (uint32_t)o];
                         const float4 pi = e.s.hxyzr[en.y];
                         const float dx = pi.x - po.x, dy = pi.y - po.y, dz = pi.z - po.z;
-                        S.id[rr] = en.x;
-                        S.ud[rr] = (e.s.hdbl[en.y].z >> 16) | ((e.s.cell_flags[en.x] & kUsed) ? (1u << 16) : 0u);
-                        S.dx[rr] = dx;
-                        S.dy[rr] = dy;
-                        S.dz[rr] = dz;
-                        S.n[rr] = dx * dx + dy * dy + dz * dz;
-                        S.first[rr] = ~0u;
-                        S.qt[0][rr] = 0ull;
-                        S.qt[1][rr] = 0ull;
+                        S.id[pos] = en.x;
+                        S.mod[pos] = e.s.hdbl[en.y].z >> 16;
+                        S.dx[pos] = dx;
+                        S.dy[pos] = dy;
+                        S.dz[pos] = dz;
+                        S.n[pos] = dx * dx + dy * dy + dz * dz;
+                        S.first[pos] = ~0u;
+                        S.qt[0][pos] = 0ull;
+                        S.qt[1][pos] = 0ull;
                     }
+                    n_used += (uint32_t)__popcll(um);
                 }
                 wave_sync();
+                // per hit of the group: its first row and its number of rows
+                uint32_t us0 = 0, uR = 0;
+                if (lane >= first && lane <= last && v) {
+                    us0 = S.upre[rexcl];
+                    uR = (rincl < kFishRows ? S.upre[rincl] : n_used) - us0;
+                }
                 // every unordered pair of rows of the same hit
-                const uint32_t items = (lane >= first && lane <= last) ? R * (R - 1u) / 2u : 0u;
+                const uint32_t items = uR * (uR - (uR ? 1u : 0u)) / 2u;
                 const uint32_t iincl = wave_incl_scan(items);
                 const uint32_t iexcl = iincl - items;
                 const uint32_t total = __shfl(iincl, 63, 64);
                 for (uint32_t base = 0; base < total; base += 64u) {
                     const uint32_t g = base + lane;
                     const int o = owner_of(iincl, g);
-                    const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    const uint32_t ox = (uint32_t)__shfl(iexcl, o, 64), s0 = (uint32_t)__shfl(us0, o, 64);
                     if (g < total) {
                         const uint32_t k = g - ox;
                         const uint32_t bb = tri_b(k), aa = k - bb * (bb - 1u) / 2u;
@@ -1149,8 +1169,7 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                         const uint32_t ida = S.id[ra], idb = S.id[rb];
                         const bool a_ic = ida < idb;
                         const uint32_t ri = a_ic ? ra : rb, rj = a_ic ? rb : ra;  // ic, jc rows
-                        const uint32_t ui = S.ud[ri], uj = S.ud[rj];
-                        if (((ui & uj) >> 16) && (ui & 0xFFFFu) != (uj & 0xFFFFu)) {
+                        if (S.mod[ri] != S.mod[rj]) {
                             const float ni = S.n[ri], nj = S.n[rj];
                             const float cos12 = S.dx[ri] * S.dx[rj] + S.dy[ri] * S.dy[rj] + S.dz[ri] * S.dz[rj];
                             if (cos12 * cos12 >= 0.99999f * ni * nj) {
@@ -1163,11 +1182,12 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                 }
                 wave_sync();
                 // decide each row
-                for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
+                const uint32_t uincl = wave_incl_scan(uR);
+                for (uint32_t r0 = 0; r0 < n_used; r0 += 64u) {
                     const uint32_t rr = r0 + lane;
-                    const int o = owner_of(rincl, rr);
-                    const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
-                    if (rr < rows) {
+                    const int o = owner_of(uincl, rr);
+                    const uint32_t s0 = (uint32_t)__shfl(us0, o, 64);
+                    if (rr < n_used) {
                         const uint32_t myid = S.id[rr];
                         bool kill = S.first[rr] != ~0u;
                         // the rows ic that qualify with this row as jc (rare): killed if jc < first(ic)
