@@ -17,6 +17,14 @@ pytestmark = pytest.mark.gpu
 
 # pu200 passes the doublet cap (458,752): the truncation must keep the same cells
 CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2), ("pu200", 1001)]
+# the caps further on: a dense cone of 2,500 helices gives about 52,000 tuples (the
+# 32,768-tuple container fills, and the triplet bucket passes the 24,576 fits), 8,000
+# helices pass the fit cap alone; ten hits, one per layer, give no doublet at all
+EDGE = {
+    "tuple_cap": ("pu50", 79, dict(n_helices=2500, eta_max=0.3, z_sigma=1.0, n_vertices=5), 0x40 | 0x80),
+    "fit_cap": ("pu200", 78, dict(n_helices=8000), 0x80),
+    "no_cells": ("single_event", 5, dict(n_helices=0, noise_frac=0.0), 0),
+}
 FIT_RTOL, FIT_ATOL = 2e-6, 1e-12
 COUNTER_KEYS = ("n_hits", "n_inner_slots", "n_cells", "n_cells_found", "n_connections", "n_connect_pairs",
                 "n_killed", "n_tuples", "n_tuple_hits", "n_by_len", "n_fitted", "n_loose", "overflow", "ref_assert")
@@ -27,8 +35,8 @@ def gen_gpu(frames):
     return ca.CAHitNtupletGeneratorOnGPU(frames)
 
 
-def run_both(gen_gpu, frames, preset, seed):
-    hits = G.generate_event(preset, seed=seed)
+def run_both(gen_gpu, frames, preset, seed, **kw):
+    hits = G.generate_event(preset, seed=seed, **kw)
     print(f"[parity] {preset} seed={seed} hits={hits.n_hits}", flush=True)
     ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=0)
     out = gen_gpu.make_tuples(hits)
@@ -46,11 +54,7 @@ def diff_report(a, b):
                                     f"{b[bad[:5]].tolist()}"
 
 
-@pytest.mark.parametrize("preset,seed", CASES)
-def test_stage_parity(gen_gpu, frames, preset, seed):
-    hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
-    if preset == "pu200":
-        assert rcnt["overflow"] & A.OVF["DOUBLETS"] and rcnt["n_cells_found"] > rcnt["n_cells"] == A.MAX_DOUBLETS
+def stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg):
     errors = []
     for k in COUNTER_KEYS:
         if gcnt[k] != rcnt[k]:
@@ -78,7 +82,48 @@ def test_stage_parity(gen_gpu, frames, preset, seed):
         d = diff_report(g, r)
         if d:
             errors.append(f"{name}: {d}")
+    return errors
+
+
+@pytest.mark.parametrize("preset,seed", CASES)
+def test_stage_parity(gen_gpu, frames, preset, seed):
+    hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
+    if preset == "pu200":
+        assert rcnt["overflow"] & A.OVF["DOUBLETS"] and rcnt["n_cells_found"] > rcnt["n_cells"] == A.MAX_DOUBLETS
+    errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
     assert not errors, "\n".join(errors)
+
+
+@pytest.mark.parametrize("case", sorted(EDGE))
+def test_stage_parity_at_the_caps(gen_gpu, frames, case):
+    preset, seed, kw, bits = EDGE[case]
+    hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed, **kw)
+    assert rcnt["overflow"] & bits == bits, hex(rcnt["overflow"])
+    if case == "no_cells":
+        assert rcnt["n_cells"] == 0 and rcnt["ref_assert"] & A.REF["NO_CELLS"]
+    errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
+    assert not errors, "\n".join(errors)
+    fitted = np.nonzero(ref.state[2] != 0)[0]
+    np.testing.assert_allclose(out.state[:, fitted], ref.state[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+
+
+def test_stage_on_an_event_without_hits(gen_gpu, frames):
+    """n_hits = 0: every layer empty; the reference's loops run zero times."""
+    from pixeltrack.formats import HitsSoA
+    z = lambda dt: np.zeros(0, dt)
+    hits = HitsSoA(xg=z(np.float32), yg=z(np.float32), zg=z(np.float32), rg=z(np.float32), xerr=z(np.float32),
+                   yerr=z(np.float32), iphi=z(np.int16), ysize=z(np.int16), det_ind=z(np.int16),
+                   layer_start=np.zeros(11, np.uint32), hist_off=np.zeros(1281, np.uint32),
+                   hist_bins=z(np.uint16), truth_track=z(np.int32))
+    print("[parity] event without hits", flush=True)
+    ref, rcnt, _ = O.make_tuples(hits, frames, stop_after=0)
+    out = gen_gpu.make_tuples(hits)
+    gcnt = gen_gpu.last_counters
+    for k in COUNTER_KEYS:
+        assert gcnt[k] == rcnt[k], k
+    assert gcnt["n_tuples"] == 0
+    np.testing.assert_array_equal(out.hit_off, ref.hit_off)
+    assert not out.state.any()
 
 
 @pytest.mark.parametrize("preset,seed", CASES)
