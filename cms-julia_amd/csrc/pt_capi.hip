@@ -661,7 +661,8 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     dout.quality = s.quality, dout.chi2 = s.chi2, dout.eta = s.eta, dout.pt = s.pt, dout.state = s.state;
     dout.covariance = s.cov, dout.hit_off = s.hit_off, dout.hit_bins = s.hit_bins, dout.det_off = s.det_off;
     dout.det_bins = s.det_bins;
-    (void)out_zeroed;  // the device copy is always zero-filled; the full state/covariance is copied back
+    // the device copy is always zero-filled; rows past the stored tuples are
+    // not copied back (zero-filled on the host unless out_zeroed)
     rc = run_batch(c, 1, &dv, &dout, b_field, 0);
     if (rc) return rc;
     HIP_OK(hipStreamSynchronize(st));
@@ -677,15 +678,31 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
         HIP_OK(hipMemcpyAsync(out->eta, s.eta, m * 4, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(out->pt, s.pt, m * 4, hipMemcpyDeviceToHost, st));
     }
-    HIP_OK(hipMemcpyAsync(out->state, s.state, 5 * T * 4, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(out->covariance, s.cov, 15 * T * 4, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(out->hit_off, s.hit_off, (T + 1) * 4, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(out->det_off, s.det_off, (T + 1) * 4, hipMemcpyDeviceToHost, st));
+    // state and covariance: only rows of stored tuples can be non-zero, so the
+    // first m rows of each component come back and the rest is zero on the host
+    if (m) {
+        HIP_OK(hipMemcpy2DAsync(out->state, T * 4, s.state, T * 4, m * 4, 5, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpy2DAsync(out->covariance, T * 4, s.cov, T * 4, m * 4, 15, hipMemcpyDeviceToHost, st));
+    }
+    // offsets: past the stored tuples they all equal off[m] (bulk_finalize_fill),
+    // unless the container is full; the copy stops there and the host fills the rest
+    const size_t n_off = (k.n_tuples < T) ? m + 1 : T + 1;
+    HIP_OK(hipMemcpyAsync(out->hit_off, s.hit_off, n_off * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(out->det_off, s.det_off, n_off * 4, hipMemcpyDeviceToHost, st));
     if (nb) {
         HIP_OK(hipMemcpyAsync(out->hit_bins, s.hit_bins, nb * 2, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(out->det_bins, s.det_bins, nb * 2, hipMemcpyDeviceToHost, st));
     }
+    if (!out_zeroed)
+        for (int comp = 0; comp < 20; ++comp) {
+            float *col = (comp < 5 ? out->state + comp * T : out->covariance + (comp - 5) * T);
+            std::memset(col + m, 0, (T - m) * 4);
+        }
     HIP_OK(hipStreamSynchronize(st));
+    for (size_t i = n_off; i <= T; ++i) {
+        out->hit_off[i] = out->hit_off[m];
+        out->det_off[i] = out->det_off[m];
+    }
     if (cnt) *cnt = k;
     return PT_OK;
 }

@@ -176,3 +176,24 @@ def test_fit_only_fp64_matches_oracle():
     np.testing.assert_allclose(res["pt"][ok], out64[ok, 20], rtol=FIT_RTOL)
     np.testing.assert_allclose(res["chi2"][ok], out64[ok, 22], rtol=FIT_RTOL, atol=1e-9)
     ctx.close()
+
+
+def test_host_entry_fills_what_it_does_not_copy(gen_gpu, frames):
+    """pt_make_tuples copies back only the rows of stored tuples; with out_zeroed = 0
+    it zero-fills the rest of state / covariance itself and fills the offsets past
+    the stored tuples, so a dirty TrackSOA comes out equal to a fresh one."""
+    import ctypes as C
+    from pixeltrack.formats import TrackSOA
+    hits = G.generate_event("pu50", seed=9)
+    print("[parity] host entry point, dirty output buffers", flush=True)
+    clean = gen_gpu.make_tuples(hits)
+    dirty = TrackSOA()
+    for k in ("state", "covariance"):
+        getattr(dirty, k)[...] = np.nan
+    for k in ("hit_off", "det_off"):
+        getattr(dirty, k)[...] = 0xDEADBEEF
+    view, tv, cnt = hits.view(), dirty.view(), A.PtCounters()
+    A.check(gen_gpu.ctx.lib.pt_make_tuples(gen_gpu.ctx.h, C.byref(view), A.B_FIELD_INV_GEV, C.byref(tv), 0,
+                                           C.byref(cnt)), "pt_make_tuples")
+    for k in ("state", "covariance", "hit_off", "det_off"):
+        np.testing.assert_array_equal(getattr(dirty, k), getattr(clean, k), err_msg=k)
