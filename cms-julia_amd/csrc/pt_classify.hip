@@ -3,10 +3,9 @@
 //   kernel_classify_tracks          (ca_hit_ntuplet_generator_kernels_impl.jl:213-247)
 //                                   runs in the fit epilogue (pt_fit.hip): only fitted
 //                                   tuples can pass its cuts
-//   kernel_fast_duplicate_remover   (:250-279)                 k_classify_lists, k_dc_prepare,
-//                                                              k_dup_remover
-//   kernel_count/fill_hit_in_tracks (:281-311) + finalize!    k_h2t_reduce
-//   kernel_triplet_cleaner          (:313-362)                 k_h2t_decide
+//   kernel_fast_duplicate_remover   (:250-279)                 phases 1-3 of k_classify_event
+//   kernel_count/fill_hit_in_tracks (:281-311) + finalize!    phase 4
+//   kernel_triplet_cleaner          (:313-362)                 phase 5
 //
 // Order in the reference loops and how it is kept here:
 //   * the fast duplicate remover walks the cells in id order and reads the
@@ -53,10 +52,15 @@ constexpr uint32_t kDoneLds = 8192u;
 
 }  // namespace
 
-// K-C1: every stored tuple appends its id to the lists of its shared cells
+// One block of 1024 threads per event runs the whole of classify_tuples after
+// the fit (classification itself is in the fit epilogue), phase by phase with
+// block barriers between them: every phase reads what the previous one wrote,
+// and all of it belongs to the block's own event.
+
+// Phase 1: every stored tuple appends its id to the lists of its shared cells
 // (list offsets come from the multiplicity scan, cursors start at zero).
-__global__ void __launch_bounds__(256) k_classify_lists(const EvDesc *__restrict__ evs, EvGrid grid) {
-    PT_EVENT_LOOP(e, t, n_stored(e)) {
+__device__ __forceinline__ void cl_lists(const EvDesc &e) {
+    for (uint32_t t = threadIdx.x; t < n_stored(e); t += blockDim.x) {
         const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[t];
         const uint32_t cs[4] = {tc.x, tc.y, tc.z, tc.w};
 #pragma unroll
@@ -72,12 +76,12 @@ __global__ void __launch_bounds__(256) k_classify_lists(const EvDesc *__restrict
     }
 }
 
-// K-C2, per shared cell: a list longer than the CellTracks capacity keeps its
-// 64 smallest tuple ids in ascending order in front (the reference's list is
-// the first 64 pushed, in tuple id order); then the predecessors: the shared
-// cells with a smaller id on the tuples of the (capped) list.
-__global__ void __launch_bounds__(256) k_dc_prepare(const EvDesc *__restrict__ evs, EvGrid grid) {
-    PT_EVENT_LOOP(e, s, n_shared(e)) {
+// Phase 2, per shared cell: a list longer than the CellTracks capacity keeps
+// its 64 smallest tuple ids in ascending order in front (the reference's list
+// is the first 64 pushed, in tuple id order); then the predecessors: the
+// shared cells with a smaller id on the tuples of the (capped) list.
+__device__ __forceinline__ void cl_prepare(const EvDesc &e) {
+    for (uint32_t s = threadIdx.x; s < n_shared(e); s += blockDim.x) {
         uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
         const uint32_t n_all = e.s.dc_cur[s];
         if (n_all > PT_MAX_CELL_TRACKS) {
@@ -121,18 +125,13 @@ __global__ void __launch_bounds__(256) k_dc_prepare(const EvDesc *__restrict__ e
     }
 }
 
-// K-C3, one block per event: kernel_fast_duplicate_remover in rounds; the
-// round that resolved a shared cell is its stamp (0: not yet).  The count of
-// cells still pending after round r is s_pending[r % 3]: thread 0 clears the
-// counter of round r + 1 during round r, when every wave has passed the
-// barrier that follows its read of that counter in round r - 2, so no wave
-// can read a counter that was cleared under it.
-__global__ void __launch_bounds__(1024) k_dup_remover(const EvDesc *__restrict__ evs, EvGrid grid) {
-    __shared__ uint32_t s_done[kDoneLds];
-    __shared__ uint32_t s_pending[3];
-    uint32_t ev, blk;
-    if (!ev_block(grid, ev, blk)) return;
-    const EvDesc &e = evs[ev];
+// Phase 3: kernel_fast_duplicate_remover in rounds; the round that resolved a
+// shared cell is its stamp (0: not yet), in LDS for the first kDoneLds cells.
+// The count of cells still pending after round r is s_pending[r % 3]: thread
+// 0 clears the counter of round r + 1 during round r, when every wave has
+// passed the barrier that follows its read of that counter in round r - 2, so
+// no wave can read a counter that was cleared under it.
+__device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uint32_t *s_pending) {
     const uint32_t nd = n_shared(e);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     for (uint32_t s = tid; s < min(nd, kDoneLds); s += nthr) s_done[s] = 0u;
@@ -209,11 +208,11 @@ __global__ void __launch_bounds__(1024) k_dup_remover(const EvDesc *__restrict__
     }
 }
 
-// K-C4: per loose tuple and hit, the hit's loose-tuple count, their largest
+// Phase 4: per loose tuple and hit, the hit's loose-tuple count, their largest
 // hit count and the (|tip|, largest id) minimum; the association's entries
 // are counted against its capacity.  The per-hit fields start from k_reset.
-__global__ void __launch_bounds__(256) k_h2t_reduce(const EvDesc *__restrict__ evs, EvGrid grid) {
-    PT_EVENT_LOOP(e, t, n_classified(e)) {
+__device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
+    for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
         uint32_t nh = 0;
         if (e.quality[t] == PT_QUALITY_LOOSE) {
             const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
@@ -234,12 +233,12 @@ __global__ void __launch_bounds__(256) k_h2t_reduce(const EvDesc *__restrict__ e
     }
 }
 
-// K-C5: kernel_triplet_cleaner per loose tuple: on any hit shared with
+// Phase 5: kernel_triplet_cleaner per loose tuple: on any hit shared with
 // another loose tuple it becomes dup when shorter than the longest there, or,
 // when they are all triplets, when it is not the one with the smallest |tip|.
 // The survivors are the loose count.
-__global__ void __launch_bounds__(256) k_h2t_decide(const EvDesc *__restrict__ evs, EvGrid grid) {
-    PT_EVENT_LOOP(e, t, n_classified(e)) {
+__device__ __forceinline__ void cl_h2t_decide(const EvDesc &e) {
+    for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
         bool keep = false;
         if (e.quality[t] == PT_QUALITY_LOOSE) {
             const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
@@ -258,30 +257,35 @@ __global__ void __launch_bounds__(256) k_h2t_decide(const EvDesc *__restrict__ e
         }
         const uint64_t m = __ballot(keep);
         if ((threadIdx.x & 63u) == 0 && m) atomicAdd(&e.cnt->n_loose, (uint32_t)__popcll(m));
-        if (t == 0 && e.s.misc[kMiscH2tEnt] > kH2tCap) atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
     }
+    // the entry count was summed by atomics in this kernel: read it from L2 (an
+    // earlier plain load of misc may have left the line in this CU's cache)
+    if (threadIdx.x == 0 && atomicAdd(&e.s.misc[kMiscH2tEnt], 0u) > kH2tCap)
+        atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
+}
+
+__global__ void __launch_bounds__(1024) k_classify_event(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ uint32_t s_done[kDoneLds];
+    __shared__ uint32_t s_pending[3];
+    uint32_t ev, blk;
+    if (!ev_block(grid, ev, blk)) return;
+    const EvDesc &e = evs[ev];
+    cl_lists(e);
+    __syncthreads();
+    cl_prepare(e);
+    __syncthreads();
+    cl_rounds(e, s_done, s_pending);
+    __syncthreads();
+    cl_h2t_reduce(e);
+    __syncthreads();
+    cl_h2t_decide(e);
 }
 
 void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm) {
     (void)cfg;
-    const uint32_t target = (2048u + n_events - 1) / n_events;
-    auto grid_for = [&](uint32_t items) {
-        uint32_t per = (items + 255u) / 256u;
-        if (per > target) per = target;
-        if (per < 1) per = 1;
-        return EvGrid{n_events, per};
-    };
-    const EvGrid gt = grid_for(kMaxTuples), gs = grid_for(4096u), one{n_events, 1u};
-    k_classify_lists<<<ev_grid_dim(gt), 256, 0, st>>>(d_evs, gt);
-    if (tm) tm->record("k_classify_lists", st);
-    k_dc_prepare<<<ev_grid_dim(gs), 256, 0, st>>>(d_evs, gs);
-    if (tm) tm->record("k_dc_prepare", st);
-    k_dup_remover<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
-    if (tm) tm->record("k_dup_remover", st);
-    k_h2t_reduce<<<ev_grid_dim(gt), 256, 0, st>>>(d_evs, gt);
-    if (tm) tm->record("k_h2t_reduce", st);
-    k_h2t_decide<<<ev_grid_dim(gt), 256, 0, st>>>(d_evs, gt);
-    if (tm) tm->record("k_h2t_decide", st);
+    const EvGrid one{n_events, 1u};
+    k_classify_event<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
+    if (tm) tm->record("k_classify_event", st);
 }
 
 }  // namespace pt
