@@ -1669,28 +1669,26 @@ static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t bs = 25
 // the blocks that are resident at once on the events' XCDs (occupancy from
 // the runtime), so every wavefront starts at once and none waits for a slot.
 static std::mutex g_occ_mu;
-static std::map<const void *, int> g_occ;
+static std::map<std::pair<const void *, int>, std::pair<int, int>> g_occ;  // (kernel, device) -> (blocks/CU, CUs)
 template <class K>
 static EvGrid ev_grid_resident(K kernel, uint32_t n_events, int bs = 256) {
-    int per_cu = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int per_cu = 0, cus = 0;
     {
         std::lock_guard<std::mutex> lk(g_occ_mu);
-        auto it = g_occ.find((const void *)kernel);
+        const auto key = std::make_pair((const void *)kernel, dev);
+        auto it = g_occ.find(key);
         if (it == g_occ.end()) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, 0) != hipSuccess || per_cu < 1)
                 per_cu = 1;
-            g_occ[(const void *)kernel] = per_cu;
+            hipDeviceProp_t prop;
+            cus = (hipGetDeviceProperties(&prop, dev) == hipSuccess) ? prop.multiProcessorCount : 256;
+            g_occ[key] = std::make_pair(per_cu, cus);
         } else {
-            per_cu = it->second;
+            per_cu = it->second.first;
+            cus = it->second.second;
         }
-    }
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        hipDeviceProp_t prop;
-        cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-                  ? prop.multiProcessorCount
-                  : 256;
     }
     const uint32_t ev_per_xcd = (n_events + 7u) / 8u;
     uint32_t per = (uint32_t)(cus / 8) * (uint32_t)per_cu / ev_per_xcd;

@@ -1,7 +1,8 @@
 // pt_capi.hip - C ABI of the pixel-track library (include/pixeltrack.h):
 // contexts, the event-slot arena, descriptor upload and kernel sequencing.
 // Host code only (compiled without a device pass); the kernels and their
-// launchers live in pt_ca.hip and pt_fit.hip.
+// launchers live in pt_ca.hip (CA), pt_fit.hip (fits), pt_classify.hip
+// (classification), pt_hist.hip (phi histogram) and pt_load_tracks.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
