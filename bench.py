@@ -279,11 +279,15 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=100)
     ap.add_argument("--preset", default="pu50")
-    ap.add_argument("--streams", type=int, default=1)
+    ap.add_argument("--streams", type=int, default=1, help="split each batch over this many streams")
+    ap.add_argument("--inflight", type=int, default=4,
+                    help="batches in flight: each batch of --events events has its own context and stream, and "
+                         "step k runs on context k %% inflight, so a batch's latency-bound kernels overlap the "
+                         "next batch's")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-fit", action="store_true", help="skip the fit-only, adjoint and PU200 workloads")
@@ -323,23 +327,34 @@ def main():
     print(f"[bench] generated {len(events)} {args.preset} events, mean hits "
           f"{np.mean([e.n_hits for e in events]):.0f}", file=sys.stderr, flush=True)
 
-    ctx = ca.Context(max_events=len(events), frames=frames)
-    batch = ca.DeviceBatch(ctx, events)
+    n_fl = max(1, args.inflight)
+    ctxs = [ca.Context(max_events=len(events), frames=frames) for _ in range(n_fl)]
+    batches = [ca.DeviceBatch(c, events) for c in ctxs]
+    ctx, batch = ctxs[0], batches[0]
     for _ in range(args.warmup):
-        batch.run_async(out_zeroed=True)
-    ctx.sync()
+        for b in batches:
+            b.run_async(out_zeroed=True)
+    for c in ctxs:
+        c.sync()
     counters = [ctx.counters(i) for i in range(len(events))]
     cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0] if k != "n_by_len"}
 
-    # timed region: production mode (events split over streams, no timing marks)
-    ctx.set_streams(args.streams)
-    ctx.sync()
+    # timed region: production mode (no timing marks); step k is one pass of the
+    # stage over the batch on context k % inflight
+    for c in ctxs:
+        c.set_streams(args.streams)
+        c.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run_async(out_zeroed=True)
-    ctx.sync()
+    for k in range(args.steps):
+        batches[k % n_fl].run_async(out_zeroed=True)
+    for c in ctxs:
+        c.sync()
     wall = time.perf_counter() - t0
-    print(f"[bench] timed {args.steps} steps in {wall:.4f} s ({args.streams} streams)", file=sys.stderr, flush=True)
+    print(f"[bench] timed {args.steps} steps in {wall:.4f} s ({n_fl} batches in flight, {args.streams} streams "
+          f"per batch)", file=sys.stderr, flush=True)
+    for b, c in zip(batches[1:], ctxs[1:]):
+        b.free()
+        c.close()
 
     # per-kernel durations: HIP events recorded after every kernel on the
     # stream the kernels run on (a timed batch runs on one stream)
@@ -426,7 +441,9 @@ def main():
                         f"mean {np.mean(cnt['n_hits']):.0f} hits, {np.mean(cnt['n_cells']):.0f} doublets, "
                         f"{np.mean(cnt['n_tuples']):.0f} tuples, {np.mean(cnt['n_fitted']):.0f} fits per event",
             "events_per_step": len(events),
-            "parallelism": "events batched in one launch sequence; an event's blocks on one XCD",
+            "batches_in_flight": n_fl,
+            "parallelism": "events batched in one launch sequence; an event's blocks on one XCD; batches in flight "
+                           "on separate streams",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
