@@ -327,6 +327,22 @@ def main():
     print(f"[bench] generated {len(events)} {args.preset} events, mean hits "
           f"{np.mean([e.n_hits for e in events]):.0f}", file=sys.stderr, flush=True)
 
+    # the host-buffer entry point, one event per call (PCIe included), measured
+    # first: contexts created and destroyed earlier in a process were seen to slow
+    # later calls of it about threefold
+    host_path = gen = None
+    if args.only == "all":
+        gen = ca.CAHitNtupletGeneratorOnGPU(frames)
+        gen.make_tuples(events[0])
+        n_host = min(len(events), 50)
+        t0 = time.perf_counter()
+        for h in events[:n_host]:
+            gen.make_tuples(h)
+        w_p = time.perf_counter() - t0
+        host_path = {"events_per_s": round(n_host / w_p, 1), "events": n_host,
+                     "note": "pt_make_tuples: host hits in, host TrackSOA out, one event per call, synchronous"}
+        print(f"[bench] host buffers: {host_path}", file=sys.stderr, flush=True)
+
     n_fl = max(1, args.inflight)
     ctxs = [ca.Context(max_events=len(events), frames=frames) for _ in range(n_fl)]
     batches = [ca.DeviceBatch(c, events) for c in ctxs]
@@ -385,7 +401,7 @@ def main():
 
     # the same stage building the phi histogram on the device first (no
     # histogram handed in), and the host-buffer entry point (PCIe included)
-    dev_hist = host_path = None
+    dev_hist = None
     if args.only == "all":
         hb = ca.DeviceBatch(ctx, events, device_hist=True)
         hb.run_async(out_zeroed=True)
@@ -398,17 +414,7 @@ def main():
         hb.free()
         dev_hist = {"events_per_s": round(len(events) * args.steps / w_h, 1),
                     "ms_per_step": round(w_h * 1e3 / args.steps, 4)}
-        gen = ca.CAHitNtupletGeneratorOnGPU(frames)
-        gen.make_tuples(events[0])
-        n_host = min(len(events), 50)
-        t0 = time.perf_counter()
-        for h in events[:n_host]:
-            gen.make_tuples(h)
-        w_p = time.perf_counter() - t0
-        host_path = {"events_per_s": round(n_host / w_p, 1), "events": n_host,
-                     "note": "pt_make_tuples: host hits in, host TrackSOA out, one event per call, synchronous"}
-        gen.ctx.close()
-        print(f"[bench] device histogram: {dev_hist}; host buffers: {host_path}", file=sys.stderr, flush=True)
+        print(f"[bench] device histogram: {dev_hist}", file=sys.stderr, flush=True)
 
     fit = adj = stream = None
     if not args.no_fit:
@@ -459,6 +465,8 @@ def main():
     print(json.dumps(line))
     batch.free()
     ctx.close()
+    if gen is not None:
+        gen.ctx.close()
 
 
 if __name__ == "__main__":
