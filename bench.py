@@ -401,7 +401,7 @@ def main():
 
     # the same stage building the phi histogram on the device first (no
     # histogram handed in), and the host-buffer entry point (PCIe included)
-    dev_hist = None
+    dev_hist = mixed_stage = None
     if args.only == "all":
         hb = ca.DeviceBatch(ctx, events, device_hist=True)
         hb.run_async(out_zeroed=True)
@@ -413,8 +413,22 @@ def main():
         w_h = time.perf_counter() - t0
         hb.free()
         dev_hist = {"events_per_s": round(len(events) * args.steps / w_h, 1),
-                    "ms_per_step": round(w_h * 1e3 / args.steps, 4)}
+                    "ms_per_step": round(w_h * 1e3 / args.steps, 4), "batches_in_flight": 1}
         print(f"[bench] device histogram: {dev_hist}", file=sys.stderr, flush=True)
+        # the same stage with the mixed-precision fit (one batch at a time)
+        from pixeltrack import _abi as A
+        ctx.set_fit_precision(A.FIT_MIXED)
+        batch.run_async(out_zeroed=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.run_async(out_zeroed=True)
+        ctx.sync()
+        w_m = time.perf_counter() - t0
+        ctx.set_fit_precision(A.FIT_FP64)
+        mixed_stage = {"events_per_s": round(len(events) * args.steps / w_m, 1),
+                       "ms_per_step": round(w_m * 1e3 / args.steps, 4), "batches_in_flight": 1}
+        print(f"[bench] stage with the mixed fit: {mixed_stage}", file=sys.stderr, flush=True)
 
     fit = adj = stream = None
     if not args.no_fit:
@@ -460,6 +474,7 @@ def main():
         "adjoint": adj,
         "pu200_stream": stream,
         "stage_with_device_histogram": dev_hist,
+        "stage_with_mixed_fit": mixed_stage,
         "host_buffers_pcie_inclusive": host_path,
     }
     print(json.dumps(line))
