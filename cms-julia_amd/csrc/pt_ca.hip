@@ -309,46 +309,47 @@ __device__ __forceinline__ uint32_t cand_pos(uint32_t a0, uint32_t na, uint32_t 
 __device__ __forceinline__ uint32_t cand_pos(const SlotCtx &s, uint32_t k) { return cand_pos(s.a0, s.na, s.b0, k); }
 
 // Candidate cuts (gpu_pixel_doublets.jl:317-340) on the hit record q stored at
-// a histogram position (q.w >> 16 = 1-based hit id).
+// a histogram position (q.w >> 16 = 1-based hit id).  The cuts are combined
+// into one flag instead of returning at the first failure, so the wavefront's
+// lanes (candidates of different slots and layer pairs) do not split into
+// branches at every cut; a reference assertion is flagged only where the
+// serial loop would reach it (all earlier cuts passed).
 __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, const SlotCtx &s, uint4 q,
                                           uint32_t &ref_bits) {
     const pt_params &P = cfg.p;
     const int p = slot_pair(s);
     const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
     const uint32_t oi = q.w >> 16;
-    if (!(oi > e.layer_start[outer] && oi <= e.layer_start[outer + 1])) {
-        ref_bits |= PT_REF_HIST_LAYER;
-        if (oi == 0 || oi > e.n_hits) return false;
-    }
-    if ((int16_t)(q.z >> 16) > 2000) return false;
+    const bool in_layer = oi > e.layer_start[outer] && oi <= e.layer_start[outer + 1];
+    ref_bits |= in_layer ? 0u : PT_REF_HIST_LAYER;
+    bool ok = in_layer || (oi != 0 && oi <= e.n_hits);
+    ok = ok && (int16_t)(q.z >> 16) <= 2000;
     const float zo = __uint_as_float(q.x), ro = __uint_as_float(q.y);
     if (P.do_z0_cut) {
         // gpu_pixel_doublets.jl:69-74
         const float dr = ro - s.me_r;
-        if (dr > cfg.pairs.max_r[p] || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr) return false;
+        ok = ok && !(dr > cfg.pairs.max_r[p] || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr);
     }
     const int16_t cut = cfg.pairs.phi_cut[p];
     int16_t i_dphi = (int16_t)((int16_t)(q.z & 0xFFFFu) - slot_me_p(s));
     i_dphi = (int16_t)(i_dphi < 0 ? -i_dphi : i_dphi);  // Int16 abs wraps at -32768
-    if (i_dphi > cut) return false;
+    ok = ok && !(i_dphi > cut);
     if (P.do_cluster_cut) {
         // gpu_pixel_doublets.jl:79-88
         const int me_s = s.me_s;
         if (outer < 4) {
             const int so = (int16_t)(q.w & 0xFFFFu);
-            if (me_s > 0 && so > 0 && abs(so - me_s) > ((inner == 0) ? 28 : 20)) return false;
-        } else if (inner < 4 && me_s > 0) {
+            ok = ok && !(me_s > 0 && so > 0 && abs(so - me_s) > ((inner == 0) ? 28 : 20));
+        } else if (inner < 4 && me_s > 0 && ok) {
             const float qq = fabsf((s.me_z - zo) / (s.me_r - ro)) * (float)(8 * 0.0285 / 0.015);
             const double t = trunc((double)qq + 0.5);
-            if (!isfinite(t) || fabs(t) > 9.0e15) {
-                ref_bits |= PT_REF_SIZE_CUT_INF;
-                return false;
-            }
-            if (llabs((long long)me_s - (long long)t) > 20) return false;
+            const bool bad = !isfinite(t) || fabs(t) > 9.0e15;
+            ref_bits |= bad ? PT_REF_SIZE_CUT_INF : 0u;
+            ok = !bad && !(llabs((long long)me_s - (long long)(bad ? 0.0 : t)) > 20);
         }
     }
-    if (P.do_pt_cut && pt_cut(ro, i_dphi, s.me_r)) return false;
-    return true;
+    if (P.do_pt_cut) ok = ok && !pt_cut(ro, i_dphi, s.me_r);
+    return ok;
 }
 
 // ------------------------------------------------------------------ K0: reset per event
