@@ -822,7 +822,7 @@ constexpr int kConnCols = 4;
 
 struct ConnLds {
     float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
-    uint32_t cols[kConnHits], a[6][kConnHits], cum[6][kConnHits];
+    uint32_t cols[kConnHits], off[6][kConnHits], cum[6][kConnHits];  // column c of range q: cell off[q] + c
     // pairs that passed the alignment cut, waiting for the dca cut
     float qx1[64], qy1[64], qx2[64], qy2[64], qx3[64], qy3[64], qreg[64];
 };
@@ -856,7 +856,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 }
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    S.a[q][lane] = ra[q];
+                    S.off[q][lane] = ra[q] - C;
                     C += rb[q] - ra[q];
                     S.cum[q][lane] = C;
                 }
@@ -899,6 +899,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
             const uint32_t rbase = (uint32_t)__shfl(rincl, oi, 64) - (uint32_t)__shfl(Rs, oi, 64);
             uint32_t cell = 0, cols = 0;
             float x1 = 0.f, y1 = 0.f, z1 = 0.f, r1 = 0.f, reg = 0.f, x2 = 0.f, y2 = 0.f, zi = 0.f, ri = 0.f, th = 0.f;
+            uint32_t cum[5] = {0u, 0u, 0u, 0u, 0u}, off[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // the row's column ranges
             if (act) {
                 const uint2 en = e.s.ohc_list[(size_t)(h0 + ohl) * kOhcStride + (rr - rbase)];
                 cell = en.x;
@@ -908,9 +909,13 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 x2 = S.hx[ohl], y2 = S.hy[ohl], zi = S.hz[ohl], ri = S.hr[ohl], th = S.theta[ohl];
                 cols = S.cols[ohl];
                 n_eval += cols;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) cum[q] = S.cum[q][ohl];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) off[q] = S.off[q][ohl];
             }
             const uint32_t cmax = wave_max(cols);
-            uint32_t cnt = 0, q = 0;
+            uint32_t cnt = 0;
             uint4 nb = make_uint4(~0u, ~0u, ~0u, ~0u);
             for (uint32_t c0 = 0; c0 < cmax; c0 += kConnCols) {
                 uint32_t t[kConnCols];
@@ -918,14 +923,14 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 bool al[kConnCols];
 #pragma unroll
                 for (int k = 0; k < kConnCols; ++k) {
+                    // the range of column c without a branch: cum is non-decreasing and c < cum[5];
+                    // a lane past its columns loads cell 0, which is never used
                     const uint32_t c = c0 + (uint32_t)k;
-                    t[k] = 0;
-                    o3[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (c < cols) {
-                        while (c >= S.cum[q][ohl]) ++q;
-                        t[k] = S.a[q][ohl] + c - (q ? S.cum[q - 1][ohl] : 0u);
-                        o3[k] = e.s.cell_oxyzr[t[k]];
-                    }
+                    uint32_t base = off[0];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) base = (c >= cum[q]) ? off[q + 1] : base;
+                    t[k] = (c < cols) ? base + c : 0u;
+                    o3[k] = e.s.cell_oxyzr[t[k]];
                 }
                 uint32_t nq = 0, qi[kConnCols];
 #pragma unroll
