@@ -590,14 +590,50 @@ __global__ void __launch_bounds__(256) k_doublet_scan(const EvDesc *__restrict__
 // outer) record, the pair, the outer hit's coordinates, cleared flags and
 // neighbour count, and an entry in the outer hit's list.  Doublets beyond
 // max_num_of_doublets are counted but not stored, like the serial loop.
+//
+// The cells of a tile share outer hits (neighbouring inner hits see the same
+// phi window; about 2.5 cells per distinct outer hit in a chunk of 192 cells
+// at PU50), so the list cursors are advanced once per distinct outer hit of a
+// chunk: the chunk's outer hits are counted in an LDS hash table, one global
+// atomic per table entry reserves that many list entries, and the cells take
+// consecutive entries of the reservation.  That cuts the cursor atomics, and
+// the entries of one outer hit are written by one instruction to adjacent
+// addresses.
+constexpr int kFillU = 3;                    // rounds of 64 cells per chunk
+constexpr uint32_t kFillHash = 256;          // > 64 * kFillU distinct keys, so an insert always ends
+constexpr uint32_t kFillEmpty = 0xFFFFFFFFu;
+static_assert(kFillHash > 64u * kFillU && (kFillHash & (kFillHash - 1u)) == 0u, "hash table size");
+
+struct FillHash {
+    uint32_t key[kFillHash], cnt[kFillHash], base[kFillHash];
+};
+
+// table slot of key k (inserted if absent); the table never fills, so the
+// linear probe ends
+__device__ __forceinline__ uint32_t fill_hash_slot(FillHash &H, uint32_t k) {
+    uint32_t s = (k * 2654435761u) >> 24;  // 8 bits
+    for (;;) {
+        const uint32_t old = atomicCAS(&H.key[s], kFillEmpty, k);
+        if (old == kFillEmpty || old == k) return s;
+        s = (s + 1u) & (kFillHash - 1u);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ uint32_t lid[4][64];
     __shared__ unsigned long long lmsk[4][64];
     __shared__ uint4 lpos[4][64];
+    __shared__ FillHash hash_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    FillHash &H = hash_all[w];
+#pragma unroll
+    for (uint32_t v = 0; v < kFillHash / 64u; ++v) {
+        H.key[64u * v + lane] = kFillEmpty;
+        H.cnt[64u * v + lane] = 0u;
+    }
     const uint32_t n_slots = e.n_slots;
     const uint32_t n_tiles = (n_slots + 63u) / 64u;
     const uint32_t cap = cfg.p.max_num_of_doublets;
@@ -653,11 +689,11 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         const uint32_t fexcl = fincl - nm;
         const uint32_t ftotal = __shfl(fincl, 63, 64);
         const uint32_t jw = t * 64u;
-        constexpr int kFillU = 3;
         for (uint32_t base = 0; base < ftotal; base += 64u * kFillU) {
-            // kFillU rounds at a time: loads, then cell writes and list cursors, then list entries
+            // a chunk of kFillU rounds: loads; cell writes and the outer hits into the
+            // table; one cursor reservation per table entry; list entries
             bool act[kFillU];
-            uint32_t cid[kFillU], pos[kFillU], js[kFillU], oh[kFillU], i1[kFillU], r[kFillU];
+            uint32_t cid[kFillU], pos[kFillU], js[kFillU], oh[kFillU], i1[kFillU], hs[kFillU];
             int pp[kFillU];
             uint4 q[kFillU];
             float2 xy[kFillU];
@@ -686,11 +722,24 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
             for (int u = 0; u < kFillU; ++u)
                 if (act[u]) {
                     write_cell(js[u], pp[u], q[u], xy[u], cid[u], oh[u], i1[u]);
-                    r[u] = atomicAdd(&e.s.ohc_cursor[oh[u]], 1u);
+                    hs[u] = fill_hash_slot(H, oh[u]);
+                    atomicAdd(&H.cnt[hs[u]], 1u);
                 }
+            wave_sync();
+#pragma unroll
+            for (uint32_t v = 0; v < kFillHash / 64u; ++v) {
+                const uint32_t s = 64u * v + lane, k = H.key[s];
+                if (k != kFillEmpty) H.base[s] = atomicAdd(&e.s.ohc_cursor[k], H.cnt[s]);
+            }
+            wave_sync();
+            // the reservation's entries in any order (decrements bring the counts back to zero)
 #pragma unroll
             for (int u = 0; u < kFillU; ++u)
-                if (act[u]) list_add(oh[u], i1[u], cid[u], r[u]);
+                if (act[u]) list_add(oh[u], i1[u], cid[u], H.base[hs[u]] + atomicSub(&H.cnt[hs[u]], 1u) - 1u);
+            wave_sync();
+#pragma unroll
+            for (uint32_t v = 0; v < kFillHash / 64u; ++v) H.key[64u * v + lane] = kFillEmpty;
+            wave_sync();
         }
         if (j < n_slots && (sp.w & 0xFFFFu) > 64u) {  // candidates beyond a slot's first 64: serial, rare
             SlotCtx s;
