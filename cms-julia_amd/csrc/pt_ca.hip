@@ -669,7 +669,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         };
         auto list_add = [&](uint32_t oh, uint32_t i1, uint32_t cid, uint32_t r) {
             if (r < kOhcStride) {
-                e.s.ohc_list[(size_t)oh * kOhcStride + r] = make_uint2(cid, i1 - 1u);
+                e.s.ohc_list[ohc_index(oh, r)] = make_uint2(cid, i1 - 1u);
             } else if (r == kOhcStride) {
                 ovf |= PT_OVF_OUTER_HIT_CELLS;
                 const uint32_t k = atomicAdd(&e.s.misc[kMiscOhcOvf], 1u);
@@ -771,7 +771,6 @@ __global__ void __launch_bounds__(64) k_ohc_rebuild(const EvDesc *__restrict__ e
     const uint32_t n_cells = e.cnt->n_cells;
     for (uint32_t q = 0; q < n; ++q) {
         const uint32_t h = e.s.ohc_ovf[q];
-        uint2 *L = e.s.ohc_list + (size_t)h * kOhcStride;
         uint32_t r = 0;
         for (uint32_t c0 = 0; c0 < n_cells && r < kOhcStride; c0 += 64u) {
             const uint32_t c = c0 + lane_id();
@@ -779,7 +778,7 @@ __global__ void __launch_bounds__(64) k_ohc_rebuild(const EvDesc *__restrict__ e
             const bool match = c < n_cells && (io >> 16) == h + 1u;
             const uint64_t bal = __ballot(match);
             const uint32_t rank = r + (uint32_t)__popcll(bal & lanes_below());
-            if (match && rank < kOhcStride) L[rank] = make_uint2(c, (io & 0xFFFFu) - 1u);
+            if (match && rank < kOhcStride) e.s.ohc_list[ohc_index(h, rank)] = make_uint2(c, (io & 0xFFFFu) - 1u);
             r += (uint32_t)__popcll(bal);
         }
     }
@@ -950,7 +949,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
             float x1 = 0.f, y1 = 0.f, z1 = 0.f, r1 = 0.f, reg = 0.f, x2 = 0.f, y2 = 0.f, zi = 0.f, ri = 0.f, th = 0.f;
             uint32_t cum[5] = {0u, 0u, 0u, 0u, 0u}, off[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // the row's column ranges
             if (act) {
-                const uint2 en = e.s.ohc_list[(size_t)(h0 + ohl) * kOhcStride + (rr - rbase)];
+                const uint2 en = e.s.ohc_list[ohc_index(h0 + ohl, rr - rbase)];
                 cell = en.x;
                 const float4 p1 = e.s.hxyzr[en.y];
                 x1 = p1.x, y1 = p1.y, z1 = p1.z, r1 = p1.w;
@@ -1179,7 +1178,7 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                     uint2 en = make_uint2(0u, 0u);
                     bool used = false;
                     if (rr < rows) {
-                        en = e.s.ohc_list[(size_t)(h0 + (uint32_t)o) * kOhcStride + (rr - s0)];
+                        en = e.s.ohc_list[ohc_index(h0 + (uint32_t)o, rr - s0)];
                         used = (e.s.cell_flags[en.x] & kUsed) != 0;
                     }
                     const uint64_t um = __ballot(used);

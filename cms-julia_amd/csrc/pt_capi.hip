@@ -93,7 +93,7 @@ size_t carve(char *base, pt::Scratch *s) {
     t.tup_where = (uint2 *)take((size_t)kMaxCells * 8);
     t.tup_buf = (uint4 *)take((size_t)kTupBuf * 16);
     t.ohc_cursor = (uint32_t *)take(kMaxHits * 4);
-    t.ohc_list = (uint2 *)take((size_t)kMaxHits * kOhcStride * 8);
+    t.ohc_list = (uint2 *)take(kOhcSize * 8);
     t.ohc_ovf = (uint32_t *)take(kOhcOvfCap * 4);
     t.trk_ovf = (uint32_t *)take(kTrkOvfCap * 4);
     t.tuple_cells = (uint32_t *)take(kMaxTuples * 4 * 4);
@@ -738,14 +738,14 @@ int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
     if (d->ohc_off && d->ohc) {
         std::vector<uint32_t> cur(nh);
         if (nh) HIP_OK(hipMemcpy(cur.data(), s.ohc_cursor, (size_t)nh * 4, hipMemcpyDeviceToHost));
-        std::vector<uint2> lst((size_t)nh * pt::kOhcStride);
+        std::vector<uint2> lst(pt::kOhcSize);
         if (nh) HIP_OK(hipMemcpy(lst.data(), s.ohc_list, lst.size() * sizeof(uint2), hipMemcpyDeviceToHost));
         uint32_t pos = 0;
         d->ohc_off[0] = 0;
         for (uint32_t h = 0; h < nh; ++h) {
             const uint32_t m = std::min(cur[h], pt::kOhcStride);
             std::vector<uint32_t> v(m);
-            for (uint32_t q = 0; q < m; ++q) v[q] = lst[(size_t)h * pt::kOhcStride + q].x;
+            for (uint32_t q = 0; q < m; ++q) v[q] = lst[pt::ohc_index(h, q)].x;
             std::sort(v.begin(), v.end());
             for (uint32_t q = 0; q < v.size() && pos < d->cap_ohc; ++q) d->ohc[pos++] = v[q] + 1;
             d->ohc_off[h + 1] = pos;

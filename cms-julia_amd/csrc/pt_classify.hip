@@ -57,22 +57,47 @@ constexpr uint32_t kDoneLds = 8192u;
 // block barriers between them: every phase reads what the previous one wrote,
 // and all of it belongs to the block's own event.
 
+// The phases are chains of dependent loads (tuple -> cells -> cell fields ->
+// lists); each is written so that the loads of one step are issued together
+// (all cells of a tuple, all hits of a tuple, up to four list entries at a
+// time), which keeps the number of memory round trips per item small.
+
+// the cells of stored tuple t that carry >= 2 tuples and have a shared-cell
+// slot (cells past the first empty one are absent)
+__device__ __forceinline__ void shared_cells_of(const EvDesc &e, uint32_t t, uint32_t (&cs)[4], uint32_t (&sl)[4],
+                                                bool (&v)[4]) {
+    const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[t];
+    cs[0] = tc.x, cs[1] = tc.y, cs[2] = tc.z, cs[3] = tc.w;
+    bool run = true;
+    uint32_t nt[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        run = run && cs[q] != ~0u;
+        v[q] = run;
+        const uint32_t c = run ? cs[q] : 0u;
+        nt[q] = e.s.trk[c].x;
+        sl[q] = e.s.dc_slot[c];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = v[q] && nt[q] >= 2u && sl[q] != ~0u;
+}
+
 // Phase 1: every stored tuple appends its id to the lists of its shared cells
 // (list offsets come from the multiplicity scan, cursors start at zero).
 __device__ __forceinline__ void cl_lists(const EvDesc &e) {
     for (uint32_t t = threadIdx.x; t < n_stored(e); t += blockDim.x) {
-        const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[t];
-        const uint32_t cs[4] = {tc.x, tc.y, tc.z, tc.w};
+        uint32_t cs[4], sl[4], pos[4], off[4];
+        bool v[4];
+        shared_cells_of(e, t, cs, sl, v);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t cq = cs[q];
-            if (cq == ~0u) break;
-            if (e.s.trk[cq].x < 2u) continue;
-            const uint32_t slot = e.s.dc_slot[cq];
-            if (slot == ~0u) continue;
-            const uint32_t pos = atomicAdd(&e.s.dc_cur[slot], 1u);
-            e.s.dc_ent[e.s.dc_off[slot] + pos] = t;
-        }
+        for (int q = 0; q < 4; ++q)
+            if (v[q]) {
+                pos[q] = atomicAdd(&e.s.dc_cur[sl[q]], 1u);
+                off[q] = e.s.dc_off[sl[q]];
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (v[q]) e.s.dc_ent[off[q] + pos[q]] = t;
     }
 }
 
@@ -84,6 +109,7 @@ __device__ __forceinline__ void cl_prepare(const EvDesc &e) {
     for (uint32_t s = threadIdx.x; s < n_shared(e); s += blockDim.x) {
         uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
         const uint32_t n_all = e.s.dc_cur[s];
+        const uint32_t c = e.s.dc_cells[s];
         if (n_all > PT_MAX_CELL_TRACKS) {
             constexpr uint32_t K = PT_MAX_CELL_TRACKS;
             for (uint32_t a = 1; a < n_all; ++a) {
@@ -99,22 +125,33 @@ __device__ __forceinline__ void cl_prepare(const EvDesc &e) {
             }
         }
         const uint32_t n = min(n_all, PT_MAX_CELL_TRACKS);
-        const uint32_t c = e.s.dc_cells[s];
         uint32_t pr[4] = {~0u, ~0u, ~0u, ~0u};
         uint32_t np = 0;
         bool many = false;
-        for (uint32_t k = 0; k < n && !many; ++k) {
-            const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[L[k]];
-            const uint32_t cs[4] = {tc.x, tc.y, tc.z, tc.w};
+        // once `many` is set the outcome is fixed, so the tuples are taken two at
+        // a time: their cells' counts and slots are loaded together
+        for (uint32_t k0 = 0; k0 < n && !many; k0 += 2u) {
+            uint32_t cs[8], nt[8], sq[8];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t cq = cs[q];
-                if (cq == ~0u || cq >= c) continue;
-                if (e.s.trk[cq].x < 2u) continue;
-                const uint32_t sq = e.s.dc_slot[cq];
-                if (sq == ~0u || sq == pr[0] || sq == pr[1] || sq == pr[2] || sq == pr[3]) continue;
+            for (int j = 0; j < 2; ++j) {
+                const bool in = k0 + (uint32_t)j < n;
+                const uint4 tc = in ? reinterpret_cast<const uint4 *>(e.s.tuple_cells)[L[k0 + (uint32_t)j]]
+                                    : make_uint4(~0u, ~0u, ~0u, ~0u);
+                cs[4 * j] = tc.x, cs[4 * j + 1] = tc.y, cs[4 * j + 2] = tc.z, cs[4 * j + 3] = tc.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t cq = (cs[q] == ~0u || cs[q] >= c) ? 0u : cs[q];
+                nt[q] = e.s.trk[cq].x;
+                sq[q] = e.s.dc_slot[cq];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (cs[q] == ~0u || cs[q] >= c || nt[q] < 2u) continue;
+                const uint32_t x = sq[q];
+                if (x == ~0u || x == pr[0] || x == pr[1] || x == pr[2] || x == pr[3]) continue;
                 if (np < 4u) {
-                    pr[np] = sq;
+                    pr[np] = x;
                     ++np;
                 } else {
                     many = true;
@@ -159,15 +196,13 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
             } else {
                 const uint32_t c = e.s.dc_cells[s];
                 for (uint32_t k = 0; k < n && ready; ++k) {
-                    const uint4 tc = reinterpret_cast<const uint4 *>(e.s.tuple_cells)[L[k]];
-                    const uint32_t cs[4] = {tc.x, tc.y, tc.z, tc.w};
+                    uint32_t cs[4], sl[4];
+                    bool v[4];
+                    shared_cells_of(e, L[k], cs, sl, v);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const uint32_t cq = cs[q];
-                        if (cq == ~0u || cq >= c || e.s.trk[cq].x < 2u) continue;
-                        const uint32_t sq = e.s.dc_slot[cq];
-                        if (sq == ~0u) continue;
-                        const uint32_t d = done_of(sq);
+                        if (!v[q] || cs[q] >= c) continue;
+                        const uint32_t d = done_of(sl[q]);
                         if (d == 0u || d == round) ready = false;
                     }
                 }
@@ -178,20 +213,39 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
             }
             // the loop body for this cell: the loose tuple with the smallest
             // |tip| stays (ties: the smallest id, the first in the list), every
-            // other tuple that is not bad becomes dup
+            // other tuple that is not bad becomes dup; four entries at a time
             float mc = 10000.0f;
             uint32_t im = ~0u;
-            for (uint32_t k = 0; k < n; ++k) {
-                const uint32_t t = L[k];
-                const float sc = abs_tip(e, t);
-                if (e.quality[t] == PT_QUALITY_LOOSE && (sc < mc || (sc == mc && im != ~0u && t < im))) {
-                    mc = sc;
-                    im = t;
+            for (uint32_t k0 = 0; k0 < n; k0 += 4u) {
+                uint32_t t[4];
+                float sc[4];
+                int8_t ql[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t tt = t[j] == ~0u ? 0u : t[j];
+                    sc[j] = abs_tip(e, tt);
+                    ql[j] = e.quality[tt];
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (t[j] != ~0u && ql[j] == PT_QUALITY_LOOSE &&
+                        (sc[j] < mc || (sc[j] == mc && im != ~0u && t[j] < im))) {
+                        mc = sc[j];
+                        im = t[j];
+                    }
             }
-            for (uint32_t k = 0; k < n; ++k) {
-                const uint32_t t = L[k];
-                if (t != im && e.quality[t] != PT_QUALITY_BAD) e.quality[t] = PT_QUALITY_DUP;
+            for (uint32_t k0 = 0; k0 < n; k0 += 4u) {
+                uint32_t t[4];
+                int8_t ql[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ql[j] = e.quality[t[j] == ~0u ? 0u : t[j]];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (t[j] != ~0u && t[j] != im && ql[j] != PT_QUALITY_BAD) e.quality[t[j]] = PT_QUALITY_DUP;
             }
             if (s < kDoneLds)
                 s_done[s] = round;
@@ -208,21 +262,32 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
     }
 }
 
+// hits of a tuple looked at together (longer tuples take further rounds)
+constexpr uint32_t kHitsAtOnce = 6u;
+
 // Phase 4: per loose tuple and hit, the hit's loose-tuple count, their largest
 // hit count and the (|tip|, largest id) minimum; the association's entries
 // are counted against its capacity.  The per-hit fields start from k_reset.
 __device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
     for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
-        uint32_t nh = 0;
-        if (e.quality[t] == PT_QUALITY_LOOSE) {
-            const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
-            nh = b1 - b0;
-            const uint64_t key = ((uint64_t)__float_as_uint(abs_tip(e, t)) << 32) | (uint64_t)(0xFFFFFFFFu - t);
-            for (uint32_t p = b0; p < b1; ++p) {
-                const uint32_t h = e.hit_bins[p] - 1u;
-                atomicAdd(&e.s.h2t_cnt[h], 1u);
-                atomicMax(&e.s.h2t_max[h], nh);
-                atomicMin(reinterpret_cast<unsigned long long *>(&e.s.h2t_win[h]), (unsigned long long)key);
+        const bool loose = e.quality[t] == PT_QUALITY_LOOSE;
+        const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
+        const float tip = abs_tip(e, t);
+        const uint32_t nh = loose ? b1 - b0 : 0u;
+        if (loose) {
+            const uint64_t key = ((uint64_t)__float_as_uint(tip) << 32) | (uint64_t)(0xFFFFFFFFu - t);
+            for (uint32_t p0 = b0; p0 < b1; p0 += kHitsAtOnce) {
+                uint32_t h[kHitsAtOnce];
+#pragma unroll
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j) h[j] = (p0 + j < b1) ? e.hit_bins[p0 + j] - 1u : ~0u;
+#pragma unroll
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j)
+                    if (h[j] != ~0u) {
+                        atomicAdd(&e.s.h2t_cnt[h[j]], 1u);
+                        atomicMax(&e.s.h2t_max[h[j]], nh);
+                        atomicMin(reinterpret_cast<unsigned long long *>(&e.s.h2t_win[h[j]]),
+                                  (unsigned long long)key);
+                    }
             }
         }
         // entries of the wave: tuple sizes are below 8, sum them bit by bit
@@ -240,15 +305,26 @@ __device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
 __device__ __forceinline__ void cl_h2t_decide(const EvDesc &e) {
     for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
         bool keep = false;
-        if (e.quality[t] == PT_QUALITY_LOOSE) {
-            const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
+        const bool loose = e.quality[t] == PT_QUALITY_LOOSE;
+        const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
+        if (loose) {
             const uint32_t nh = b1 - b0;
             bool dup = false;
-            for (uint32_t p = b0; p < b1; ++p) {
-                const uint32_t h = e.hit_bins[p] - 1u;
-                if (e.s.h2t_cnt[h] < 2u) continue;
-                const uint32_t mx = e.s.h2t_max[h];
-                if (nh != mx || (mx <= 3u && (uint32_t)e.s.h2t_win[h] != 0xFFFFFFFFu - t)) dup = true;
+            for (uint32_t p0 = b0; p0 < b1; p0 += kHitsAtOnce) {
+                uint32_t h[kHitsAtOnce], cn[kHitsAtOnce], mx[kHitsAtOnce], win[kHitsAtOnce];
+#pragma unroll
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j) h[j] = (p0 + j < b1) ? e.hit_bins[p0 + j] - 1u : ~0u;
+#pragma unroll
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j) {
+                    const uint32_t hh = h[j] == ~0u ? 0u : h[j];
+                    cn[j] = e.s.h2t_cnt[hh];
+                    mx[j] = e.s.h2t_max[hh];
+                    win[j] = (uint32_t)e.s.h2t_win[hh];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j)
+                    if (h[j] != ~0u && cn[j] >= 2u && (nh != mx[j] || (mx[j] <= 3u && win[j] != 0xFFFFFFFFu - t)))
+                        dup = true;
             }
             if (dup)
                 e.quality[t] = PT_QUALITY_DUP;

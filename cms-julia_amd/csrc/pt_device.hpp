@@ -28,6 +28,9 @@ constexpr uint32_t kMaxSlots = 6u * PT_MAX_HITS;   // layer 0 is the inner layer
 constexpr uint32_t kMaxTuples = PT_MAX_TUPLES;
 constexpr uint32_t kNbrPool = 1024u * 1024u;   // neighbour entries beyond the 4 kept inline, per event
 constexpr uint32_t kOhcStride = PT_MAX_CELLS_PER_HIT;  // outer-hit list capacity per hit
+// Entry r (< kOhcStride) of outer hit h's cell list, in a buffer of kOhcSize entries.
+constexpr size_t kOhcSize = (size_t)kMaxHits * kOhcStride;
+__host__ __device__ __forceinline__ size_t ohc_index(uint32_t h, uint32_t r) { return (size_t)h * kOhcStride + r; }
 constexpr uint32_t kOhcOvfCap = kMaxCells / (PT_MAX_CELLS_PER_HIT + 1) + 1;  // hits that can exceed it
 constexpr uint32_t kTrkOvfCap = 4u * PT_MAX_TUPLES / (PT_MAX_CELL_TRACKS + 1) + 8u;
 constexpr uint32_t kSlotTile = 1024;              // slots per prefix-sum tile of the doublet scan
@@ -71,7 +74,7 @@ struct Scratch {
     PT_G uint2 *tup_where;       // [kMaxCells]    per start-list entry: (buffer offset, tuples) or ~0
     PT_G uint4 *tup_buf;         // [kTupBuf]      cells of the tuples kept by the count pass
     PT_G uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
-    PT_G uint2 *ohc_list;        // [kMaxHits*kOhcStride] (cell, inner hit) 0-based, unordered unless
+    PT_G uint2 *ohc_list;        // [kOhcSize] (cell, inner hit) 0-based, unordered unless
                             //                the hit overflowed (then the first 128 cells, ascending)
     PT_G uint32_t *ohc_ovf;      // [kOhcOvfCap]   hits with more than kOhcStride cells
     PT_G uint32_t *trk_ovf;      // [kTrkOvfCap]   cells whose tuple list needs the exact path
