@@ -1386,9 +1386,25 @@ __global__ void __launch_bounds__(256) k_dfs_count(const EvDesc *__restrict__ ev
             });
             v = ((uint64_t)nt << 32) | nh;
             tb.y = nt;
-            if (nt && nt <= kTupKeep) {
-                const uint32_t off = atomicAdd(&e.s.misc[kMiscTupBuf], nt);
-                if (off + nt <= kTupBuf) {
+            // buffer space for the lanes that keep their tuples: one atomic for the
+            // wavefront's active lanes, each lane's offset from the bits of the
+            // requests of the lanes below it (requests are 1..kTupKeep)
+            const uint32_t req = (nt && nt <= kTupKeep) ? nt : 0u;
+            uint32_t below = 0, total = 0;
+#pragma unroll
+            for (int bit = 0; bit < 3; ++bit) {
+                const uint64_t m = __ballot((req >> bit) & 1u);
+                below += (uint32_t)__popcll(m & lanes_below()) << bit;
+                total += (uint32_t)__popcll(m) << bit;
+            }
+            const uint64_t want = __ballot(req != 0u);
+            if (want) {
+                const int leader = __ffsll((long long)want) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&e.s.misc[kMiscTupBuf], total);
+                base = __shfl(base, leader, 64);
+                const uint32_t off = base + below;
+                if (req && off + nt <= kTupBuf) {
                     tb = make_uint2(off, nt);
 #pragma unroll
                     for (uint32_t k = 0; k < kTupKeep; ++k)
@@ -1458,67 +1474,97 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
         const uint64_t pre = e.s.tup[c];
         uint32_t it = (uint32_t)(pre >> 32);  // 0-based id of this cell's first tuple
         uint32_t hp = (uint32_t)pre;
+        // per cell of a stored tuple: the tuple count before this one (its
+        // atomic issued with the others of the tuple) decides the lists
+        auto after_count = [&](uint32_t cq, uint32_t before) {
+            if (before == 0u) ++lists;
+            if (before == 1u) {  // the cell now carries two tuples: a shared cell
+                const uint32_t sl = atomicAdd(&e.s.misc[kMiscDc], 1u);
+                if (sl < kDcCap) {
+                    e.s.dc_cells[sl] = cq;
+                    e.s.dc_slot[cq] = sl;
+                } else {
+                    e.s.dc_slot[cq] = ~0u;
+                    err |= PT_INT_OVF_LIST;
+                }
+            }
+            if (before == PT_MAX_CELL_TRACKS) {  // list capped: exact path
+                ovf |= PT_OVF_CELL_TRACKS;
+                const uint32_t k = atomicAdd(&e.s.misc[kMiscTrkOvf], 1u);
+                if (k < kTrkOvfCap)
+                    e.s.trk_ovf[k] = cq;
+                else
+                    err |= PT_INT_OVF_LIST;
+            }
+        };
+        // the tuple's container entries (bulk_fill: 1-based tuple id it), cells and quality
+        auto store = [&](const uint32_t *cells, const uint32_t *ios, uint32_t len) {
+            e.hit_off[it - 1] = hp + 1u;
+            uint4 tc = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                if (q < len) {
+                    e.hit_bins[hp + q] = (uint16_t)(ios[q] & 0xFFFFu);
+                    if (q + 1 == len) e.hit_bins[hp + len] = (uint16_t)(ios[q] >> 16);
+                    if (q == 0) tc.x = cells[q];
+                    if (q == 1) tc.y = cells[q];
+                    if (q == 2) tc.z = cells[q];
+                    if (q == 3) tc.w = cells[q];
+                }
+            }
+            reinterpret_cast<PT_G uint4 *>(e.s.tuple_cells)[it - 1] = tc;
+            e.quality[it - 1] = PT_QUALITY_BAD;
+        };
         auto emit = [&](const uint32_t *cells, const uint32_t *ios, uint32_t len) {
             ++it;  // 1-based tuple id (bulk_fill)
             if (it <= kMaxTuples) {
-                e.hit_off[it - 1] = hp + 1u;
-                uint4 tc = make_uint4(~0u, ~0u, ~0u, ~0u);
+                uint32_t before[4];
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) {
+                for (uint32_t q = 0; q < 4; ++q)
                     if (q < len) {
-                        const uint32_t cq = cells[q];
-                        e.hit_bins[hp + q] = (uint16_t)(ios[q] & 0xFFFFu);
-                        if (q + 1 == len) e.hit_bins[hp + len] = (uint16_t)(ios[q] >> 16);
-                        const uint32_t before = atomicAdd(&e.s.trk[cq].x, 1u);
-                        atomicMax(&e.s.trk[cq].y, len + 1u);
-                        if (before == 0u) ++lists;
-                        if (before == 1u) {  // the cell now carries two tuples: a shared cell
-                            const uint32_t sl = atomicAdd(&e.s.misc[kMiscDc], 1u);
-                            if (sl < kDcCap) {
-                                e.s.dc_cells[sl] = cq;
-                                e.s.dc_slot[cq] = sl;
-                            } else {
-                                e.s.dc_slot[cq] = ~0u;
-                                err |= PT_INT_OVF_LIST;
-                            }
-                        }
-                        if (before == PT_MAX_CELL_TRACKS) {  // list capped: exact path
-                            ovf |= PT_OVF_CELL_TRACKS;
-                            const uint32_t k = atomicAdd(&e.s.misc[kMiscTrkOvf], 1u);
-                            if (k < kTrkOvfCap)
-                                e.s.trk_ovf[k] = cq;
-                            else
-                                err |= PT_INT_OVF_LIST;
-                        }
-                        if (q == 0) tc.x = cq;
-                        if (q == 1) tc.y = cq;
-                        if (q == 2) tc.z = cq;
-                        if (q == 3) tc.w = cq;
+                        before[q] = atomicAdd(&e.s.trk[cells[q]].x, 1u);
+                        atomicMax(&e.s.trk[cells[q]].y, len + 1u);
                     }
-                }
-                reinterpret_cast<PT_G uint4 *>(e.s.tuple_cells)[it - 1] = tc;
-                e.quality[it - 1] = PT_QUALITY_BAD;
+                store(cells, ios, len);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (q < len) after_count(cells[q], before[q]);
             }
             hp += len + 1;
         };
         if (tb.x != ~0u) {
-            // the kept tuples: cells, then their hits, all loads in flight together
+            // the kept tuples: cells, then their hits, all loads in flight together;
+            // then the counters of all their cells
             uint4 t4[kTupKeep];
-            uint32_t io[kTupKeep][4];
+            uint32_t io[kTupKeep][4], cc[kTupKeep][4], len[kTupKeep], before[kTupKeep][4];
 #pragma unroll
             for (uint32_t k = 0; k < kTupKeep; ++k) t4[k] = (k < tb.y) ? e.s.tup_buf[tb.x + k] : make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
             for (uint32_t k = 0; k < kTupKeep; ++k) {
-                const uint32_t cc[4] = {t4[k].x, t4[k].y, t4[k].z, t4[k].w};
+                cc[k][0] = t4[k].x, cc[k][1] = t4[k].y, cc[k][2] = t4[k].z, cc[k][3] = t4[k].w;
+                len[k] = (cc[k][2] == ~0u) ? 2u : (cc[k][3] == ~0u) ? 3u : 4u;
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) io[k][q] = (cc[q] != ~0u) ? e.s.cell_io[cc[q]] : 0u;
+                for (uint32_t q = 0; q < 4; ++q) io[k][q] = (cc[k][q] != ~0u) ? e.s.cell_io[cc[k][q]] : 0u;
             }
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k)
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (k < tb.y && it + k < kMaxTuples && q < len[k]) {
+                        before[k][q] = atomicAdd(&e.s.trk[cc[k][q]].x, 1u);
+                        atomicMax(&e.s.trk[cc[k][q]].y, len[k] + 1u);
+                    }
 #pragma unroll
             for (uint32_t k = 0; k < kTupKeep; ++k) {
                 if (k >= tb.y) break;
-                const uint32_t cc[4] = {t4[k].x, t4[k].y, t4[k].z, t4[k].w};
-                const uint32_t len = (cc[2] == ~0u) ? 2u : (cc[3] == ~0u) ? 3u : 4u;
-                emit(cc, io[k], len);
+                ++it;
+                if (it <= kMaxTuples) {
+                    store(cc[k], io[k], len[k]);
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q)
+                        if (q < len[k]) after_count(cc[k][q], before[k][q]);
+                }
+                hp += len[k] + 1;
             }
         } else {
             const DNode root = load_node(e, c);
