@@ -859,20 +859,29 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 // Per hit h, the rows are the cells ending on h that kernel_connect visits (the
 // first 128 of h's list) and the columns the cells starting on h, in id order
 // (one contiguous id range per layer pair).  A wavefront takes kConnHits hits
-// per ticket, sorts them by column count (so that rows of similar length share
-// a round), and gives every lane one row: the lane walks the row's columns in
-// id order, kConnCols at a time, and keeps the passing ones in order - the
-// serial push! order of the row cell's neighbour list.  The alignment cut
-// runs in every lane; the dca cut (about 2% of the pairs get there) runs on
-// the survivors of a whole round of columns, compacted through LDS.
+// per ticket and sorts them by column count, descending (so that rows of
+// similar length share a round).  Consecutive sorted hits form groups whose
+// columns fit the LDS staging area; a hit with more columns is a group of its
+// own and is staged in chunks.  A group's columns - cell id and outer-hit
+// coordinates - are loaded once into LDS, in id order per hit; then every
+// lane takes one row and walks its hit's staged columns kConnCols at a time,
+// keeping the passing ones in order - the serial push! order of the row
+// cell's neighbour list.  The alignment cut runs in every lane; the dca cut
+// (about 2% of the pairs get there) runs on the survivors of a whole round of
+// columns, compacted through LDS.
 constexpr uint32_t kConnHits = 32;
 constexpr int kConnCols = 4;
+constexpr uint32_t kConnStage = 192;  // columns staged at a time
 
 struct ConnLds {
     float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
-    uint32_t cols[kConnHits], off[6][kConnHits], cum[6][kConnHits];  // column c of range q: cell off[q] + c
-    // pairs that passed the alignment cut, waiting for the dca cut
-    float qx1[64], qy1[64], qx2[64], qy2[64], qx3[64], qy3[64], qreg[64];
+    uint32_t off[6][kConnHits], cum[6][kConnHits];  // hit lane's column c of range q: cell off[q] + c
+    uint32_t ccell[kConnStage];                     // staged columns: cell id, outer-hit x, y, z, r
+    float4 cxyzr[kConnStage];
+    // the row batch's rows (per lane), and the pairs waiting for the dca cut:
+    // row lane | staged column << 8
+    float rx1[64], ry1[64], rreg[64];
+    uint32_t rhl[64], q[64];
 };
 
 __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
@@ -918,7 +927,6 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 }
             }
             if (!C) R = 0;
-            S.cols[lane] = C;
         }
         // sort the hits by column count, descending (bitonic over the wavefront)
         uint32_t key = R ? ((C << 8) | (63u - lane)) : 0u;  // ties: lower lane first
@@ -933,125 +941,159 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 key = take_max ? max(key, other) : min(key, other);
             }
         }
-        const uint32_t hl = 63u - (key & 0xFFu);           // hit lane of the i-th sorted hit
+        const uint32_t hl = 63u - (key & 0xFFu);  // hit lane of the i-th sorted hit
         const uint32_t Rsrc = (uint32_t)__shfl(R, (int)hl, 64);
         const uint32_t Rs = key ? Rsrc : 0u;
-        const uint32_t rincl = wave_incl_scan(Rs);
-        const uint32_t rows = __shfl(rincl, 63, 64);
+        const uint32_t Cs = key >> 8;  // columns of the i-th sorted hit (0: no rows)
+        const uint32_t cincl = wave_incl_scan(Cs);
         wave_sync();
-        for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
-            const uint32_t rr = r0 + lane;
-            const bool act = rr < rows;
-            const int oi = owner_of(rincl, rr);
-            const uint32_t ohl = (uint32_t)__shfl(hl, oi, 64);
-            const uint32_t rbase = (uint32_t)__shfl(rincl, oi, 64) - (uint32_t)__shfl(Rs, oi, 64);
-            uint32_t cell = 0, cols = 0;
-            float x1 = 0.f, y1 = 0.f, z1 = 0.f, r1 = 0.f, reg = 0.f, x2 = 0.f, y2 = 0.f, zi = 0.f, ri = 0.f, th = 0.f;
-            uint32_t cum[5] = {0u, 0u, 0u, 0u, 0u}, off[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // the row's column ranges
-            if (act) {
-                const uint2 en = e.s.ohc_list[ohc_index(h0 + ohl, rr - rbase)];
-                cell = en.x;
-                const float4 p1 = e.s.hxyzr[en.y];
-                x1 = p1.x, y1 = p1.y, z1 = p1.z, r1 = p1.w;
-                reg = region_of(cfg, e.s.hdbl[en.y].z >> 16);
-                x2 = S.hx[ohl], y2 = S.hy[ohl], zi = S.hz[ohl], ri = S.hr[ohl], th = S.theta[ohl];
-                cols = S.cols[ohl];
-                n_eval += cols;
+        for (uint32_t first = 0; first < 64u;) {
+            if ((uint32_t)__shfl(Cs, (int)first, 64) == 0u) break;  // the rest have no rows
+            // the group: sorted hits first..last whose columns fit the staging area
+            const uint32_t cbefore = first ? (uint32_t)__shfl(cincl, (int)first - 1, 64) : 0u;
+            const uint64_t fits = __ballot(lane >= first && Cs != 0u && cincl - cbefore <= kConnStage);
+            const uint32_t last = fits ? 63u - (uint32_t)__clzll(fits) : first;
+            const uint32_t T = (uint32_t)__shfl(cincl, (int)last, 64) - cbefore;
+            const bool in_group = lane >= first && lane <= last;
+            const uint32_t gincl = in_group ? cincl - cbefore : (lane < first ? 0u : T);
+            const uint32_t cb = gincl - (in_group ? Cs : 0u);  // group column of the hit's first column
+            // stage group columns [s0, s0 + n) at slots 0..n-1
+            auto stage = [&](uint32_t s0, uint32_t n) {
+                wave_sync();  // earlier reads of the staging area are done
+                for (uint32_t g0 = 0; g0 < n; g0 += 64u) {
+                    const uint32_t g = g0 + lane, gc = s0 + g;
+                    const int oi = owner_of(gincl, gc);
+                    const uint32_t ohl = (uint32_t)__shfl(hl, oi, 64);
+                    const uint32_t c = gc - (uint32_t)__shfl(cb, oi, 64);  // column of the hit
+                    if (g < n) {
+                        uint32_t base = S.off[0][ohl];
 #pragma unroll
-                for (int q = 0; q < 5; ++q) cum[q] = S.cum[q][ohl];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) off[q] = S.off[q][ohl];
-            }
-            const uint32_t cmax = wave_max(cols);
-            uint32_t cnt = 0;
-            uint4 nb = make_uint4(~0u, ~0u, ~0u, ~0u);
-            for (uint32_t c0 = 0; c0 < cmax; c0 += kConnCols) {
-                uint32_t t[kConnCols];
-                float4 o3[kConnCols];
-                bool al[kConnCols];
-#pragma unroll
-                for (int k = 0; k < kConnCols; ++k) {
-                    // the range of column c without a branch: cum is non-decreasing and c < cum[5];
-                    // a lane past its columns loads cell 0, which is never used
-                    const uint32_t c = c0 + (uint32_t)k;
-                    uint32_t base = off[0];
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) base = (c >= cum[q]) ? off[q + 1] : base;
-                    t[k] = (c < cols) ? base + c : 0u;
-                    o3[k] = e.s.cell_oxyzr[t[k]];
+                        for (int q = 0; q < 5; ++q) base = (c >= S.cum[q][ohl]) ? S.off[q + 1][ohl] : base;
+                        const uint32_t t = base + c;
+                        S.ccell[g] = t;
+                        S.cxyzr[g] = e.s.cell_oxyzr[t];
+                    }
                 }
-                uint32_t nq = 0, qi[kConnCols];
-#pragma unroll
-                for (int k = 0; k < kConnCols; ++k) {
-                    al[k] = c0 + (uint32_t)k < cols && are_aligned(r1, z1, ri, zi, o3[k].w, o3[k].z, cfg.p.pt_min, th);
-                    const uint64_t am = __ballot(al[k]);
-                    qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
-                    nq += (uint32_t)__popcll(am);
+                wave_sync();
+            };
+            const bool one_chunk = T <= kConnStage;
+            if (one_chunk) stage(0u, T);
+            const uint32_t rv = in_group ? Rs : 0u;
+            const uint32_t rincl = wave_incl_scan(rv);
+            const uint32_t rows = __shfl(rincl, 63, 64);
+            for (uint32_t r0 = 0; r0 < rows; r0 += 64u) {
+                const uint32_t rr = r0 + lane;
+                const bool act = rr < rows;
+                const int oi = owner_of(rincl, rr);
+                const uint32_t ohl = (uint32_t)__shfl(hl, oi, 64);
+                const uint32_t rbase = (uint32_t)__shfl(rincl, oi, 64) - (uint32_t)__shfl(rv, oi, 64);
+                // (cross-lane reads outside conditionals: a lane outside the batch still
+                // supplies its value)
+                const uint32_t cols_o = (uint32_t)__shfl(Cs, oi, 64);
+                const uint32_t cfirst = (uint32_t)__shfl(cb, oi, 64);
+                const uint32_t cols = act ? cols_o : 0u;
+                uint32_t cell = 0;
+                float x1 = 0.f, y1 = 0.f, z1 = 0.f, r1 = 0.f, reg = 0.f, zi = 0.f, ri = 0.f, th = 0.f;
+                if (act) {
+                    const uint2 en = e.s.ohc_list[ohc_index(h0 + ohl, rr - rbase)];
+                    cell = en.x;
+                    const float4 p1 = e.s.hxyzr[en.y];
+                    x1 = p1.x, y1 = p1.y, z1 = p1.z, r1 = p1.w;
+                    reg = region_of(cfg, e.s.hdbl[en.y].z >> 16);
+                    zi = S.hz[ohl], ri = S.hr[ohl], th = S.theta[ohl];
+                    n_eval += cols;
                 }
-                uint32_t pm = 0;  // this lane's passing columns of the round, bit k
-                if (nq) {
-                    // survivors into LDS in (k, lane) order, the dca cut 64 at a time
-                    for (uint32_t qb = 0; qb < nq; qb += 64u) {
+                S.rx1[lane] = x1;
+                S.ry1[lane] = y1;
+                S.rreg[lane] = reg;
+                S.rhl[lane] = ohl;
+                uint32_t cnt = 0;
+                uint4 nb = make_uint4(~0u, ~0u, ~0u, ~0u);
+                for (uint32_t cc0 = 0; cc0 < T; cc0 += kConnStage) {
+                    if (!one_chunk) stage(cc0, min(T - cc0, kConnStage));
+                    // this row's columns in the chunk, as staging slots [lo, lo + span)
+                    const uint32_t a = max(cfirst, cc0), z = min(cfirst + cols, cc0 + kConnStage);
+                    const uint32_t span = (act && z > a) ? z - a : 0u;
+                    const uint32_t lo = a - cc0;
+                    const uint32_t smax = wave_max(span);
+                    for (uint32_t j0 = 0; j0 < smax; j0 += kConnCols) {
+                        uint32_t sl[kConnCols];
+                        float4 o3[kConnCols];
+                        bool al[kConnCols];
+                        uint32_t nq = 0, qi[kConnCols];
 #pragma unroll
-                        for (int k = 0; k < kConnCols; ++k)
-                            if (al[k] && qi[k] >= qb && qi[k] < qb + 64u) {
-                                const uint32_t s = qi[k] - qb;
-                                S.qx1[s] = x1;
-                                S.qy1[s] = y1;
-                                S.qreg[s] = reg;
-                                S.qx2[s] = x2;
-                                S.qy2[s] = y2;
-                                S.qx3[s] = o3[k].x;
-                                S.qy3[s] = o3[k].y;
+                        for (int k = 0; k < kConnCols; ++k) {
+                            const bool in = j0 + (uint32_t)k < span;
+                            sl[k] = in ? lo + j0 + (uint32_t)k : 0u;
+                            o3[k] = S.cxyzr[sl[k]];
+                            al[k] = in && are_aligned(r1, z1, ri, zi, o3[k].w, o3[k].z, cfg.p.pt_min, th);
+                            const uint64_t am = __ballot(al[k]);
+                            qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
+                            nq += (uint32_t)__popcll(am);
+                        }
+                        uint32_t pm = 0;  // this lane's passing columns of the round, bit k
+                        // survivors in (k, lane) order, the dca cut 64 at a time
+                        for (uint32_t qb = 0; qb < nq; qb += 64u) {
+#pragma unroll
+                            for (int k = 0; k < kConnCols; ++k)
+                                if (al[k] && qi[k] >= qb && qi[k] < qb + 64u) S.q[qi[k] - qb] = lane | (sl[k] << 8);
+                            wave_sync();
+                            bool ok = false;
+                            if (qb + lane < nq) {
+                                const uint32_t v = S.q[lane], rl = v & 63u;
+                                const uint32_t hh = S.rhl[rl];
+                                const float4 o = S.cxyzr[v >> 8];
+                                ok = dca_cut(S.rx1[rl], S.ry1[rl], S.hx[hh], S.hy[hh], o.x, o.y, S.rreg[rl],
+                                             cfg.p.hard_curv_cut);
                             }
-                        wave_sync();
-                        const bool ok = qb + lane < nq && dca_cut(S.qx1[lane], S.qy1[lane], S.qx2[lane], S.qy2[lane],
-                                                                  S.qx3[lane], S.qy3[lane], S.qreg[lane],
-                                                                  cfg.p.hard_curv_cut);
-                        const uint64_t okm = __ballot(ok);
+                            const uint64_t okm = __ballot(ok);
+#pragma unroll
+                            for (int k = 0; k < kConnCols; ++k)
+                                if (al[k] && qi[k] >= qb && qi[k] < qb + 64u && ((okm >> (qi[k] - qb)) & 1ull))
+                                    pm |= 1u << k;
+                            wave_sync();
+                        }
 #pragma unroll
                         for (int k = 0; k < kConnCols; ++k)
-                            if (al[k] && qi[k] >= qb && qi[k] < qb + 64u && ((okm >> (qi[k] - qb)) & 1ull))
-                                pm |= 1u << k;
-                        wave_sync();
+                            if ((pm >> k) & 1u) {
+                                const uint32_t t = S.ccell[sl[k]];
+                                e.s.cell_flags[t] = kUsed;
+                                e.s.trk[t] = make_uint2(0u, 0u);
+                                if (cnt == 0) nb.x = t;
+                                if (cnt == 1) nb.y = t;
+                                if (cnt == 2) nb.z = t;
+                                if (cnt == 3) nb.w = t;
+                                ++cnt;
+                            }
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < kConnCols; ++k)
-                    if ((pm >> k) & 1u) {
-                        e.s.cell_flags[t[k]] = kUsed;
-                        e.s.trk[t[k]] = make_uint2(0u, 0u);
-                        if (cnt == 0) nb.x = t[k];
-                        if (cnt == 1) nb.y = t[k];
-                        if (cnt == 2) nb.z = t[k];
-                        if (cnt == 3) nb.w = t[k];
-                        ++cnt;
-                    }
-            }
-            bool st = false;
-            if (cnt) {
-                const uint32_t nn = cnt < PT_MAX_NEIGHBORS ? cnt : PT_MAX_NEIGHBORS;
-                e.s.cell_flags[cell] = kUsed;
-                e.s.trk[cell] = make_uint2(0u, 0u);
-                e.s.cell_nn[cell] = (uint8_t)nn;
-                e.s.nbr4[cell] = nb;
-                n_conn += nn;
-                ++n_lists;
-                if (cnt > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
-                if (nn > 4u) e.s.ext_list[atomicAdd(&e.s.misc[kMiscExt], 1u)] = cell;
-                st = start_pair(cfg, e.s.cell_pair[cell]);
-            }
-            const uint64_t sb = __ballot(st);
-            if (sb) {
-                uint32_t sbase = 0;
-                if (lane == 0) sbase = atomicAdd(&e.s.misc[kMiscStart], (uint32_t)__popcll(sb));
-                sbase = __shfl(sbase, 0, 64);
-                if (st) {
-                    e.s.start_list[sbase + (uint32_t)__popcll(sb & lanes_below())] = cell;
-                    atomicOr(reinterpret_cast<unsigned long long *>(&e.s.start_bits[cell >> 6]),
-                             1ull << (cell & 63u));
+                bool st = false;
+                if (cnt) {
+                    const uint32_t nn = cnt < PT_MAX_NEIGHBORS ? cnt : PT_MAX_NEIGHBORS;
+                    e.s.cell_flags[cell] = kUsed;
+                    e.s.trk[cell] = make_uint2(0u, 0u);
+                    e.s.cell_nn[cell] = (uint8_t)nn;
+                    e.s.nbr4[cell] = nb;
+                    n_conn += nn;
+                    ++n_lists;
+                    if (cnt > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;
+                    if (nn > 4u) e.s.ext_list[atomicAdd(&e.s.misc[kMiscExt], 1u)] = cell;
+                    st = start_pair(cfg, e.s.cell_pair[cell]);
                 }
+                const uint64_t sb = __ballot(st);
+                if (sb) {
+                    uint32_t sbase = 0;
+                    if (lane == 0) sbase = atomicAdd(&e.s.misc[kMiscStart], (uint32_t)__popcll(sb));
+                    sbase = __shfl(sbase, 0, 64);
+                    if (st) {
+                        e.s.start_list[sbase + (uint32_t)__popcll(sb & lanes_below())] = cell;
+                        atomicOr(reinterpret_cast<unsigned long long *>(&e.s.start_bits[cell >> 6]),
+                                 1ull << (cell & 63u));
+                    }
+                }
+                wave_sync();  // this batch's reads of the row arrays are done
             }
+            first = last + 1u;
         }
     }
     wave_add_atomic(&e.cnt->n_connect_pairs, n_eval);
