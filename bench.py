@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--events", type=int, default=100)
     ap.add_argument("--preset", default="pu50")
     ap.add_argument("--streams", type=int, default=1, help="split each batch over this many streams")
-    ap.add_argument("--inflight", type=int, default=4,
+    ap.add_argument("--inflight", type=int, default=6,
                     help="batches in flight: each batch of --events events has its own context and stream, and "
                          "step k runs on context k %% inflight, so a batch's latency-bound kernels overlap the "
                          "next batch's")

@@ -200,14 +200,53 @@ struct SlotCtx {
     int32_t me_s;
     uint32_t pk;          // me_p (16 b) | pair << 16
     uint32_t a0, na, b0;  // candidates: positions [a0, a0+na) then [b0, ...)
+    // the pair's constants the candidate cuts use, so that the candidate loop
+    // reads them with the slot instead of indexing the pair tables per lane
+    uint32_t lo, hi;      // outer-layer hit ids (lo, hi], 1-based
+    float max_r;
+    uint32_t cut;         // phi cut (16 b) | inner layer << 16 | outer layer << 24
 };
 
 __device__ __forceinline__ int slot_pair(const SlotCtx &s) { return (int)(s.pk >> 16); }
 __device__ __forceinline__ int16_t slot_me_p(const SlotCtx &s) { return (int16_t)(s.pk & 0xFFFFu); }
 
+// The layer-pair constants of the block's event in LDS: a lane indexes them
+// by its own pair, which from the kernel arguments or the event descriptor
+// would be a chain of memory reads per lane.
+struct PairLds {
+    uint32_t slot_cum[20];                  // as EvDesc::slot_cum
+    uint32_t ls_inner[19], lo[19], hi[19];  // inner layer's first hit; outer layer's hits (lo, hi]
+    int32_t inner[19], outer[19], cut[19];
+    float min_z[19], max_z[19], max_r[19];
+    int32_t by_inner[10][6], n_by_inner[10];
+};
+// every thread of the block calls it (it ends with a barrier)
+__device__ __forceinline__ void load_pairs(PairLds &P, const EvDesc &e, const Config &cfg) {
+    const uint32_t t = threadIdx.x;
+    if (t < 20u) P.slot_cum[t] = e.slot_cum[t];
+    if (t < 19u) {
+        const int in = cfg.pairs.inner[t], out = cfg.pairs.outer[t];
+        P.inner[t] = in;
+        P.outer[t] = out;
+        P.cut[t] = cfg.pairs.phi_cut[t];
+        P.min_z[t] = cfg.pairs.min_z[t];
+        P.max_z[t] = cfg.pairs.max_z[t];
+        P.max_r[t] = cfg.pairs.max_r[t];
+        P.ls_inner[t] = e.layer_start[in];
+        P.lo[t] = e.layer_start[out];
+        P.hi[t] = e.layer_start[out + 1];
+    }
+    if (t < 60u) P.by_inner[t / 6u][t % 6u] = cfg.pairs.by_inner[t / 6u][t % 6u];
+    if (t < 10u) P.n_by_inner[t] = cfg.pairs.n_by_inner[t];
+    __syncthreads();
+}
+
+// the pair of inner slot j: slot_cum is non-decreasing (unused pairs hold
+// 0xFFFFFFFF), so the first pair with j < slot_cum is the count of the others
 __device__ __forceinline__ int pair_of_slot(const EvDesc &e, uint32_t j) {
     int p = 0;
-    while (j >= e.slot_cum[p]) ++p;
+#pragma unroll
+    for (int k = 0; k < 20; ++k) p += (j >= e.slot_cum[k]) ? 1 : 0;
     return p;
 }
 
@@ -219,16 +258,16 @@ struct SlotPrep {
     uint32_t hoff, kl, kh;  // window bins (kl == kh: none); hoff = 128 * outer layer
     bool ok;
 };
-__device__ __forceinline__ SlotPrep slot_prep(const EvDesc &e, const Config &cfg, int p, uint4 q, uint32_t &ref_bits,
-                                              SlotCtx &s) {
+__device__ __forceinline__ SlotPrep slot_prep(const EvDesc &e, const Config &cfg, const PairLds &T, int p, uint4 q,
+                                              uint32_t &ref_bits, SlotCtx &s) {
     const pt_params &P = cfg.p;
-    const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
+    const int inner = T.inner[p], outer = T.outer[p];
     SlotPrep r{0u, 0u, 0u, false};
     const int16_t mi = (int16_t)(q.z >> 16);
     s.pk = (uint32_t)p << 16;
     if (mi > 2000) return r;
     s.me_z = __uint_as_float(q.x);
-    if (s.me_z < cfg.pairs.min_z[p] || s.me_z > cfg.pairs.max_z[p]) return r;
+    if (s.me_z < T.min_z[p] || s.me_z > T.max_z[p]) return r;
     int me_s = -1;
     if (P.do_cluster_cut) {
         if (inner == 0 && !(mi < 96)) ref_bits |= PT_REF_BAD_MODULE;
@@ -241,7 +280,11 @@ __device__ __forceinline__ SlotPrep slot_prep(const EvDesc &e, const Config &cfg
     const int16_t me_p = (int16_t)(q.z & 0xFFFFu);
     s.me_r = __uint_as_float(q.y);
     s.pk = (uint32_t)(uint16_t)me_p | ((uint32_t)p << 16);
-    const int16_t cut = cfg.pairs.phi_cut[p];
+    const int16_t cut = (int16_t)T.cut[p];
+    s.lo = T.lo[p];
+    s.hi = T.hi[p];
+    s.max_r = T.max_r[p];
+    s.cut = (uint32_t)(uint16_t)cut | ((uint32_t)inner << 16) | ((uint32_t)outer << 24);
     const int kl = hist_bin((int16_t)(me_p - cut));
     int kh = hist_bin((int16_t)(me_p + cut)) + 1;
     if (kh == 129) kh = 1;
@@ -292,11 +335,11 @@ __device__ __forceinline__ uint32_t slot_window(const EvDesc &e, const SlotPrep 
 }
 
 // returns the number of candidates (0 when the inner hit fails its cuts)
-__device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cfg, uint32_t j, uint32_t &ref_bits,
-                                               SlotCtx &s) {
+__device__ __forceinline__ uint32_t slot_setup(const EvDesc &e, const Config &cfg, const PairLds &T, uint32_t j,
+                                               uint32_t &ref_bits, SlotCtx &s) {
     const int p = pair_of_slot(e, j);
-    const uint32_t i0 = ((p == 0) ? j : j - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]];
-    const SlotPrep r = slot_prep(e, cfg, p, e.s.hdbl[i0], ref_bits, s);
+    const uint32_t i0 = ((p == 0) ? j : j - T.slot_cum[p - 1]) + T.ls_inner[p];
+    const SlotPrep r = slot_prep(e, cfg, T, p, e.s.hdbl[i0], ref_bits, s);
     if (!r.ok) return 0;
     uint32_t o[4];
     slot_offsets(e, r, o);
@@ -317,10 +360,9 @@ __device__ __forceinline__ uint32_t cand_pos(const SlotCtx &s, uint32_t k) { ret
 __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, const SlotCtx &s, uint4 q,
                                           uint32_t &ref_bits) {
     const pt_params &P = cfg.p;
-    const int p = slot_pair(s);
-    const int inner = cfg.pairs.inner[p], outer = cfg.pairs.outer[p];
+    const int inner = (int)((s.cut >> 16) & 0xFFu), outer = (int)(s.cut >> 24);
     const uint32_t oi = q.w >> 16;
-    const bool in_layer = oi > e.layer_start[outer] && oi <= e.layer_start[outer + 1];
+    const bool in_layer = oi > s.lo && oi <= s.hi;
     ref_bits |= in_layer ? 0u : PT_REF_HIST_LAYER;
     bool ok = in_layer || (oi != 0 && oi <= e.n_hits);
     ok = ok && (int16_t)(q.z >> 16) <= 2000;
@@ -328,9 +370,9 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
     if (P.do_z0_cut) {
         // gpu_pixel_doublets.jl:69-74
         const float dr = ro - s.me_r;
-        ok = ok && !(dr > cfg.pairs.max_r[p] || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr);
+        ok = ok && !(dr > s.max_r || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr);
     }
-    const int16_t cut = cfg.pairs.phi_cut[p];
+    const int16_t cut = (int16_t)(s.cut & 0xFFFFu);
     int16_t i_dphi = (int16_t)((int16_t)(q.z & 0xFFFFu) - slot_me_p(s));
     i_dphi = (int16_t)(i_dphi < 0 ? -i_dphi : i_dphi);  // Int16 abs wraps at -32768
     ok = ok && !(i_dphi > cut);
@@ -446,6 +488,8 @@ __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict_
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t n_slots = e.n_slots;
     const uint32_t n_tiles = (n_slots + 63u) / 64u;
@@ -457,7 +501,7 @@ __global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict_
         const uint32_t j0 = t * 64u, j = j0 + lane;
         wave_sync();  // the previous tile's reads of ctx are done
         SlotCtx s = {};
-        const uint32_t n = (j < n_slots) ? slot_setup(e, cfg, j, ref, s) : 0u;
+        const uint32_t n = (j < n_slots) ? slot_setup(e, cfg, PT, j, ref, s) : 0u;
         const uint32_t incl = wave_incl_scan(n);
         const uint32_t excl = incl - n;
         const uint32_t total = __shfl(incl, 63, 64);
@@ -627,6 +671,8 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     FillHash &H = hash_all[w];
 #pragma unroll
@@ -659,7 +705,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         // writes cell cid from its outer-hit record q, xy at histogram position; returns
         // the outer hit (0-based) and inner hit (1-based) for the list entry
         auto write_cell = [&](uint32_t js, int p, uint4 q, float2 xy, uint32_t cid, uint32_t &oh, uint32_t &i1) {
-            i1 = ((p == 0) ? js : js - e.slot_cum[p - 1]) + e.layer_start[cfg.pairs.inner[p]] + 1u;
+            i1 = ((p == 0) ? js : js - PT.slot_cum[p - 1]) + PT.ls_inner[p] + 1u;
             oh = (q.w >> 16) - 1u;
             e.s.cell_io[cid] = i1 | ((oh + 1u) << 16);
             e.s.cell_flags[cid] = 0;
@@ -743,7 +789,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         }
         if (j < n_slots && (sp.w & 0xFFFFu) > 64u) {  // candidates beyond a slot's first 64: serial, rare
             SlotCtx s;
-            const uint32_t n = slot_setup(e, cfg, j, ref, s);
+            const uint32_t n = slot_setup(e, cfg, PT, j, ref, s);
             uint32_t cid = id + nm;
             for (uint32_t k = 64; k < n && cid < cap; ++k) {
                 const uint32_t pos = cand_pos(s, k);
@@ -840,10 +886,10 @@ __device__ __forceinline__ float region_of(const Config &cfg, uint32_t det_inner
 }
 
 // candidate cells of the pair by_inner[L][q] starting on hit h (lpos = h's index in L)
-__device__ __forceinline__ void pair_range(const EvDesc &e, const Config &cfg, int L, uint32_t lpos, int q,
+__device__ __forceinline__ void pair_range(const EvDesc &e, const PairLds &T, int L, uint32_t lpos, int q,
                                            uint32_t n_cells, uint32_t &a, uint32_t &b) {
-    const int p = cfg.pairs.by_inner[L][q];
-    const uint32_t j = ((p == 0) ? 0u : e.slot_cum[p - 1]) + lpos;
+    const int p = T.by_inner[L][q];
+    const uint32_t j = ((p == 0) ? 0u : T.slot_cum[p - 1]) + lpos;
     a = e.s.slot_cnt[j];
     b = e.s.slot_cnt[j + 1];
     if (b > n_cells) b = n_cells;
@@ -889,6 +935,8 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     ConnLds &S = lds_all[w];
     const uint32_t n_cells = e.cnt->n_cells, n_hits = e.n_hits;
@@ -903,13 +951,13 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
             R = min(e.s.ohc_cursor[h], kOhcStride);
             if (R) {
                 const int L = layer_of(e, h);
-                const int np = cfg.pairs.n_by_inner[L];
+                const int np = PT.n_by_inner[L];
                 const uint32_t lpos = h - e.layer_start[L];
                 uint32_t ra[6], rb[6];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     ra[q] = rb[q] = 0;
-                    if (q < np) pair_range(e, cfg, L, lpos, q, n_cells, ra[q], rb[q]);
+                    if (q < np) pair_range(e, PT, L, lpos, q, n_cells, ra[q], rb[q]);
                 }
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
@@ -1104,7 +1152,8 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
 
 // Cells with more than 4 neighbours (rare): walk the candidates again in id
 // order and put neighbours 5..64 in the per-event pool.  Items i0, i0+stride, ...
-__device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config &cfg, uint32_t i0, uint32_t stride,
+__device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config &cfg, const PairLds &T, uint32_t i0,
+                                                  uint32_t stride,
                                                   uint32_t &err) {
     const uint32_t n_ext = e.s.misc[kMiscExt];
     for (uint32_t i = i0; i < n_ext; i += stride) {
@@ -1121,7 +1170,7 @@ __device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config 
         const uint32_t io = e.s.cell_io[o];
         const uint32_t h0 = (io >> 16) - 1u, i1 = (io & 0xFFFFu) - 1u;
         const int L = layer_of(e, h0);
-        const int np = cfg.pairs.n_by_inner[L];
+        const int np = T.n_by_inner[L];
         const uint32_t lpos = h0 - e.layer_start[L];
         const float4 p1 = e.s.hxyzr[i1], p2 = e.s.hxyzr[h0];
         const ConnCtx c{p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w, region_of(cfg, e.s.hdbl[i1].z >> 16),
@@ -1129,7 +1178,7 @@ __device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config 
         uint32_t k = 0;
         for (int q = 0; q < np && k < nkeep; ++q) {
             uint32_t a, bq;
-            pair_range(e, cfg, L, lpos, q, n_cells, a, bq);
+            pair_range(e, T, L, lpos, q, n_cells, a, bq);
             for (uint32_t t = a; t < bq && k < nkeep; ++t) {
                 if (!conn_pass(c, cfg, e.s.cell_oxyzr[t])) continue;
                 if (k >= 4u) e.s.nbr_pool[base + k - 4u] = t;
@@ -1145,7 +1194,9 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     uint32_t err = 0;
-    connect_ext_items(e, cfg, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
+    connect_ext_items(e, cfg, PT, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
     wave_or_atomic(&e.cnt->internal_error, err);
 }
 
@@ -1186,9 +1237,11 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
     {  // the rare neighbour lists longer than 4 first (independent of the fishbone)
         uint32_t err = 0;
-        connect_ext_items(e, cfg, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
+        connect_ext_items(e, cfg, PT, b * blockDim.x + threadIdx.x, grid.per_event * blockDim.x, err);
         wave_or_atomic(&e.cnt->internal_error, err);
     }
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
