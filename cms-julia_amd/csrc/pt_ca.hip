@@ -481,7 +481,7 @@ struct alignas(16) SlotItem {
     uint32_t excl;  // the slot's first item in the tile
 };
 
-__global__ void __launch_bounds__(256) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+__global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ SlotItem ctx[4][64];
     __shared__ uint4 omap[4][kCountWin / 4];
     __shared__ uint64_t pbits[4][kCountWin / 64 + 1];
@@ -1414,21 +1414,14 @@ __device__ __forceinline__ void dfs_level(const EvDesc &e, const DNode &node, ui
             if (nn > 1) c1 = load_node(e, node.nb.y);
             if (nn > 2) c2 = load_node(e, node.nb.z);
             if (nn > 3) c3 = load_node(e, node.nb.w);
-            const uint32_t n4 = nn < 4u ? nn : 4u;
-            for (uint32_t i = 0; i < n4; ++i) {
-                const DNode c = pick4(i, c0, c1, c2, c3);
+            // one loop over all neighbours (5..64 from the pool, rare), so that each
+            // depth is instantiated once and the kernel's code stays small
+            const uint32_t ext = (nn > 4u) ? e.s.nbr_ext[node.id] : 0u;
+            for (uint32_t i = 0; i < nn; ++i) {
+                const DNode c = (i < 4u) ? pick4(i, c0, c1, c2, c3) : load_node(e, e.s.nbr_pool[ext + i - 4u]);
                 if (c.killed) continue;
                 child = true;
                 dfs_level<D + 1>(e, c, cells, ios, min_doublets, ref_bits, leaf);
-            }
-            if (nn > 4u) {  // neighbours 5..64 from the pool (rare)
-                const uint32_t ext = e.s.nbr_ext[node.id];
-                for (uint32_t i = 4; i < nn; ++i) {
-                    const DNode c = load_node(e, e.s.nbr_pool[ext + i - 4u]);
-                    if (c.killed) continue;
-                    child = true;
-                    dfs_level<D + 1>(e, c, cells, ios, min_doublets, ref_bits, leaf);
-                }
             }
         } else {
             // a live fifth doublet: the reference's @assert fails
