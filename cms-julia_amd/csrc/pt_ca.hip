@@ -1564,10 +1564,11 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
         uint32_t hp = (uint32_t)pre;
         // per cell of a stored tuple: the tuple count before this one (its
         // atomic issued with the others of the tuple) decides the lists
-        auto after_count = [&](uint32_t cq, uint32_t before) {
+        // (sl_pre: the shared-cell slot when reserved beforehand, else ~0u)
+        auto after_count = [&](uint32_t cq, uint32_t before, uint32_t sl_pre) {
             if (before == 0u) ++lists;
             if (before == 1u) {  // the cell now carries two tuples: a shared cell
-                const uint32_t sl = atomicAdd(&e.s.misc[kMiscDc], 1u);
+                const uint32_t sl = (sl_pre != ~0u) ? sl_pre : atomicAdd(&e.s.misc[kMiscDc], 1u);
                 if (sl < kDcCap) {
                     e.s.dc_cells[sl] = cq;
                     e.s.dc_slot[cq] = sl;
@@ -1616,7 +1617,7 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
                 store(cells, ios, len);
 #pragma unroll
                 for (uint32_t q = 0; q < 4; ++q)
-                    if (q < len) after_count(cells[q], before[q]);
+                    if (q < len) after_count(cells[q], before[q], ~0u);
             }
             hp += len + 1;
         };
@@ -1642,6 +1643,29 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
                         before[k][q] = atomicAdd(&e.s.trk[cc[k][q]].x, 1u);
                         atomicMax(&e.s.trk[cc[k][q]].y, len[k] + 1u);
                     }
+            // shared-cell slots for the cells that now carry two tuples: one atomic
+            // for the wavefront's lanes on this path (at most 16 per lane)
+            uint32_t n1 = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kTupKeep; ++k)
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (k < tb.y && it + k < kMaxTuples && q < len[k] && before[k][q] == 1u) ++n1;
+            uint32_t below = 0, total = 0;
+#pragma unroll
+            for (int bit = 0; bit < 5; ++bit) {
+                const uint64_t m = __ballot((n1 >> bit) & 1u);
+                below += (uint32_t)__popcll(m & lanes_below()) << bit;
+                total += (uint32_t)__popcll(m) << bit;
+            }
+            const uint64_t want = __ballot(n1 != 0u);
+            uint32_t sl_next = 0;
+            if (want) {
+                const int leader = __ffsll((long long)want) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&e.s.misc[kMiscDc], total);
+                sl_next = (uint32_t)__shfl(base, leader, 64) + below;
+            }
 #pragma unroll
             for (uint32_t k = 0; k < kTupKeep; ++k) {
                 if (k >= tb.y) break;
@@ -1650,7 +1674,11 @@ __global__ void __launch_bounds__(256) k_dfs_emit(const EvDesc *__restrict__ evs
                     store(cc[k], io[k], len[k]);
 #pragma unroll
                     for (uint32_t q = 0; q < 4; ++q)
-                        if (q < len[k]) after_count(cc[k][q], before[k][q]);
+                        if (q < len[k]) {
+                            const bool two = before[k][q] == 1u;
+                            after_count(cc[k][q], before[k][q], two ? sl_next : ~0u);
+                            sl_next += two ? 1u : 0u;
+                        }
                 }
                 hp += len[k] + 1;
             }
