@@ -349,20 +349,23 @@ def main():
     ctx, batch = ctxs[0], batches[0]
     for _ in range(args.warmup):
         for b in batches:
-            b.run_async(out_zeroed=True)
+            b.run_async(out_zeroed=False)
     for c in ctxs:
         c.sync()
     counters = [ctx.counters(i) for i in range(len(events))]
     cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0] if k != "n_by_len"}
 
     # timed region: production mode (no timing marks); step k is one pass of the
-    # stage over the batch on context k % inflight
+    # stage over the batch on context k % inflight.  The output TrackSOAs are
+    # reused without being cleared in between, so the stage clears state and
+    # covariance itself (out_zeroed=False), as the reference's TrackSOA() does
+    # inside make_tuples
     for c in ctxs:
         c.set_streams(args.streams)
         c.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        batches[k % n_fl].run_async(out_zeroed=True)
+        batches[k % n_fl].run_async(out_zeroed=False)
     for c in ctxs:
         c.sync()
     wall = time.perf_counter() - t0
@@ -378,7 +381,7 @@ def main():
     per_kernel = {}
     n_inst = max(3, args.steps // 2)
     for _ in range(n_inst):
-        batch.run_async(out_zeroed=True)
+        batch.run_async(out_zeroed=False)
         for k, v in ctx.stage_times().items():
             per_kernel[k] = per_kernel.get(k, 0.0) + v
     ctx.enable_timing(False)
@@ -404,11 +407,11 @@ def main():
     dev_hist = mixed_stage = None
     if args.only == "all":
         hb = ca.DeviceBatch(ctx, events, device_hist=True)
-        hb.run_async(out_zeroed=True)
+        hb.run_async(out_zeroed=False)
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            hb.run_async(out_zeroed=True)
+            hb.run_async(out_zeroed=False)
         ctx.sync()
         w_h = time.perf_counter() - t0
         hb.free()
@@ -418,11 +421,11 @@ def main():
         # the same stage with the mixed-precision fit (one batch at a time)
         from pixeltrack import _abi as A
         ctx.set_fit_precision(A.FIT_MIXED)
-        batch.run_async(out_zeroed=True)
+        batch.run_async(out_zeroed=False)
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            batch.run_async(out_zeroed=True)
+            batch.run_async(out_zeroed=False)
         ctx.sync()
         w_m = time.perf_counter() - t0
         ctx.set_fit_precision(A.FIT_FP64)

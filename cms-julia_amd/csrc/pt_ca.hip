@@ -394,6 +394,33 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
     return ok;
 }
 
+// state and covariance can be cleared with 16-byte stores
+__device__ __forceinline__ bool zero_vec(const EvDesc &e) {
+    return ((reinterpret_cast<uintptr_t>(e.state) | reinterpret_cast<uintptr_t>(e.cov)) & 15u) == 0u;
+}
+// Clearing state and covariance (20 floats per tuple row, 2.5 MB per event) is
+// pure store bandwidth; the doublet count's wavefronts take it on by ticket
+// once its tiles are gone, so it fills that latency-bound kernel's tail
+// instead of holding every CU in a pass of its own.
+constexpr uint32_t kZeroChunk = 512;  // float4 per ticket
+__device__ __forceinline__ void zero_chunks(const EvDesc &e) {
+    constexpr uint32_t n4 = 5u * kMaxTuples;  // float4: state, then covariance
+    constexpr uint32_t n_state4 = 5u * kMaxTuples / 4u;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (;;) {
+        const uint32_t c = take_ticket(&e.s.misc[kMiscTkZero]);
+        if (c * kZeroChunk >= n4) break;
+#pragma unroll
+        for (uint32_t v = 0; v < kZeroChunk / 64u; ++v) {
+            const uint32_t k = c * kZeroChunk + 64u * v + (threadIdx.x & 63u);
+            if (k < n_state4)
+                reinterpret_cast<PT_G float4 *>(e.state)[k] = z;
+            else if (k < n4)
+                reinterpret_cast<PT_G float4 *>(e.cov)[k - n_state4] = z;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ K0: reset per event
 // Clears the counters, list cursors and per-tile totals; packs the per-hit
 // fields the CA reads into 16-byte records, once in hit order and once in
@@ -403,8 +430,10 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
 __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, EvGrid grid, int zero_state) {
     constexpr uint32_t kHead = 32u;
     constexpr uint32_t kBits = (kMaxCells + 63u) / 64u;
+    // state and covariance are cleared here only when they are not both 16-byte
+    // aligned; otherwise the doublet count clears them (zero_chunks)
     PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kSlotTile - 1) / kSlotTile + kDfsAgg + kBits +
-                            (zero_state ? 20u * kMaxTuples : 0u)) {
+                            ((zero_state && !zero_vec(e)) ? 20u * kMaxTuples : 0u)) {
         uint32_t k = i;
         if (k < kHead) {
             if (k < sizeof(pt_counters) / 4) reinterpret_cast<uint32_t *>(e.cnt)[k] = 0u;
@@ -481,7 +510,8 @@ struct alignas(16) SlotItem {
     uint32_t excl;  // the slot's first item in the tile
 };
 
-__global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+__global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg,
+                                                         int zero_state) {
     __shared__ SlotItem ctx[4][64];
     __shared__ uint4 omap[4][kCountWin / 4];
     __shared__ uint64_t pbits[4][kCountWin / 64 + 1];
@@ -584,6 +614,7 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
         const uint32_t sum = wave_sum(cnt);
         if (lane == 0 && sum) atomicAdd(&e.s.dbl_agg[j0 / kSlotTile], sum);
     }
+    if (zero_state && zero_vec(e)) zero_chunks(e);
     wave_or_atomic(&e.cnt->ref_assert, ref);
 }
 
@@ -1928,7 +1959,7 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
     mark("k_reset");
     const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events);
-    k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg);
+    k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg, zero_state);
     mark("k_doublet_count");
     const EvGrid g_dscan = ev_grid(max_slots / 4u, n_events);
     k_doublet_scan<<<ev_grid_dim(g_dscan), 256, 0, st>>>(d_evs, g_dscan, cfg);

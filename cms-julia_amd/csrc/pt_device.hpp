@@ -108,6 +108,7 @@ constexpr uint8_t kUsed = 1, kKilled = 2;
 enum Misc : int {
     kMiscStart = 0,       // entries of start_list
     kMiscExt = 1,         // entries of ext_list
+    kMiscTkZero = 2,      // next chunk of state/covariance to clear (doublet count)
     kMiscOhcOvf = 3,      // entries of ohc_ovf
     kMiscTrkOvf = 4,      // entries of trk_ovf
     kMiscPool = 5,        // neighbour pool entries used
