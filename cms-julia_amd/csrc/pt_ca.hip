@@ -406,7 +406,10 @@ constexpr uint32_t kZeroChunk = 512;  // float4 per ticket
 __device__ __forceinline__ void zero_chunks(const EvDesc &e) {
     constexpr uint32_t n4 = 5u * kMaxTuples;  // float4: state, then covariance
     constexpr uint32_t n_state4 = 5u * kMaxTuples / 4u;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    // non-temporal stores: the cleared lines are not kept in the caches the CA
+    // kernels work from
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v zv = {0.f, 0.f, 0.f, 0.f};
     for (;;) {
         const uint32_t c = take_ticket(&e.s.misc[kMiscTkZero]);
         if (c * kZeroChunk >= n4) break;
@@ -414,9 +417,9 @@ __device__ __forceinline__ void zero_chunks(const EvDesc &e) {
         for (uint32_t v = 0; v < kZeroChunk / 64u; ++v) {
             const uint32_t k = c * kZeroChunk + 64u * v + (threadIdx.x & 63u);
             if (k < n_state4)
-                reinterpret_cast<PT_G float4 *>(e.state)[k] = z;
+                __builtin_nontemporal_store(zv, &reinterpret_cast<PT_G f4v *>(e.state)[k]);
             else if (k < n4)
-                reinterpret_cast<PT_G float4 *>(e.cov)[k - n_state4] = z;
+                __builtin_nontemporal_store(zv, &reinterpret_cast<PT_G f4v *>(e.cov)[k - n_state4]);
         }
     }
 }
