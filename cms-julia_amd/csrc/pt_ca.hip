@@ -684,6 +684,8 @@ static_assert(kFillHash > 64u * kFillU && (kFillHash & (kFillHash - 1u)) == 0u, 
 
 struct FillHash {
     uint32_t key[kFillHash], cnt[kFillHash], base[kFillHash];
+    uint32_t pre[kFillHash];   // the slot's first entry in the chunk's entries grouped by slot
+    uint2 ent[64u * kFillU];   // those entries: (cell, slot | inner hit << 8)
 };
 
 // table slot of key k (inserted if absent); the table never fills, so the
@@ -806,16 +808,43 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
                     atomicAdd(&H.cnt[hs[u]], 1u);
                 }
             wave_sync();
+            // per table slot (lane l owns slots 4l..4l+3): the reservation, and the
+            // slot's first position when the chunk's entries are grouped by slot
+            static_assert(kFillHash == 256u, "four table slots per lane");
+            const uint4 k4 = reinterpret_cast<const uint4 *>(H.key)[lane];
+            const uint4 c4 = reinterpret_cast<const uint4 *>(H.cnt)[lane];
+            const uint32_t kk[4] = {k4.x, k4.y, k4.z, k4.w}, cc[4] = {c4.x, c4.y, c4.z, c4.w};
+            uint32_t bb[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-            for (uint32_t v = 0; v < kFillHash / 64u; ++v) {
-                const uint32_t s = 64u * v + lane, k = H.key[s];
-                if (k != kFillEmpty) H.base[s] = atomicAdd(&e.s.ohc_cursor[k], H.cnt[s]);
-            }
+            for (int v = 0; v < 4; ++v)
+                if (kk[v] != kFillEmpty) bb[v] = atomicAdd(&e.s.ohc_cursor[kk[v]], cc[v]);
+            const uint32_t lsum = cc[0] + cc[1] + cc[2] + cc[3];
+            const uint32_t lincl = wave_incl_scan(lsum);
+            const uint32_t n_ent = (uint32_t)__shfl(lincl, 63, 64);
+            const uint32_t p0 = lincl - lsum;
+            reinterpret_cast<uint4 *>(H.base)[lane] = make_uint4(bb[0], bb[1], bb[2], bb[3]);
+            reinterpret_cast<uint4 *>(H.pre)[lane] =
+                make_uint4(p0, p0 + cc[0], p0 + cc[0] + cc[1], p0 + cc[0] + cc[1] + cc[2]);
             wave_sync();
-            // the reservation's entries in any order (decrements bring the counts back to zero)
+            // each entry takes a rank in its slot (the decrements bring the counts back
+            // to zero) and its place in the grouped order
 #pragma unroll
             for (int u = 0; u < kFillU; ++u)
-                if (act[u]) list_add(oh[u], i1[u], cid[u], H.base[hs[u]] + atomicSub(&H.cnt[hs[u]], 1u) - 1u);
+                if (act[u]) {
+                    const uint32_t r = atomicSub(&H.cnt[hs[u]], 1u) - 1u;
+                    H.ent[H.pre[hs[u]] + r] = make_uint2(cid[u], hs[u] | ((i1[u] - 1u) << 8));
+                }
+            wave_sync();
+            // list entries in the grouped order: lanes with the same outer hit write
+            // adjacent entries of its list
+            for (uint32_t g0 = 0; g0 < n_ent; g0 += 64u) {
+                const uint32_t g = g0 + lane;
+                if (g < n_ent) {
+                    const uint2 en = H.ent[g];
+                    const uint32_t sl = en.y & 0xFFu;
+                    list_add(H.key[sl], (en.y >> 8) + 1u, en.x, H.base[sl] + (g - H.pre[sl]));
+                }
+            }
             wave_sync();
 #pragma unroll
             for (uint32_t v = 0; v < kFillHash / 64u; ++v) H.key[64u * v + lane] = kFillEmpty;
