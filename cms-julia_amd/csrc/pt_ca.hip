@@ -448,6 +448,7 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
         k -= kHead;
         if (k < e.n_hits) {
             e.s.ohc_cursor[k] = 0u;
+            e.s.fish_cnt[k] = 0u;
             e.s.h2t_cnt[k] = 0u;
             e.s.h2t_max[k] = 0u;
             e.s.h2t_win[k] = ~0ull;
@@ -985,7 +986,7 @@ constexpr uint32_t kConnStage = 192;  // columns staged at a time
 struct ConnLds {
     float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
     uint32_t off[6][kConnHits], cum[6][kConnHits];  // hit lane's column c of range q: cell off[q] + c
-    uint32_t ccell[kConnStage];                     // staged columns: cell id, outer-hit x, y, z, r
+    uint32_t coh[kConnStage];                       // staged columns: outer hit, its x, y, z, r
     float4 cxyzr[kConnStage];
     // the row batch's rows (per lane), and the pairs waiting for the dca cut:
     // row lane | staged column << 8
@@ -1081,7 +1082,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
 #pragma unroll
                         for (int q = 0; q < 5; ++q) base = (c >= S.cum[q][ohl]) ? S.off[q + 1][ohl] : base;
                         const uint32_t t = base + c;
-                        S.ccell[g] = t;
+                        S.coh[g] = (e.s.cell_io[t] >> 16) - 1u;
                         S.cxyzr[g] = e.s.cell_oxyzr[t];
                     }
                 }
@@ -1167,8 +1168,16 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
 #pragma unroll
                         for (int k = 0; k < kConnCols; ++k)
                             if ((pm >> k) & 1u) {
-                                const uint32_t t = S.ccell[sl[k]];
+                                // the column's cell id again from the hit's ranges
+                                const uint32_t c = sl[k] + cc0 - cfirst;
+                                uint32_t base = S.off[0][ohl];
+#pragma unroll
+                                for (int q = 0; q < 5; ++q) base = (c >= S.cum[q][ohl]) ? S.off[q + 1][ohl] : base;
+                                const uint32_t t = base + c;
                                 e.s.cell_flags[t] = kUsed;
+                                // marks per outer hit (a cell can be counted more than once):
+                                // the fishbone only looks at hits with two or more
+                                atomicAdd(&e.s.fish_cnt[S.coh[sl[k]]], 1u);
                                 e.s.trk[t] = make_uint2(0u, 0u);
                                 if (cnt == 0) nb.x = t;
                                 if (cnt == 1) nb.y = t;
@@ -1182,6 +1191,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 if (cnt) {
                     const uint32_t nn = cnt < PT_MAX_NEIGHBORS ? cnt : PT_MAX_NEIGHBORS;
                     e.s.cell_flags[cell] = kUsed;
+                    atomicAdd(&e.s.fish_cnt[h0 + ohl], 1u);
                     e.s.trk[cell] = make_uint2(0u, 0u);
                     e.s.cell_nn[cell] = (uint8_t)nn;
                     e.s.nbr4[cell] = nb;
@@ -1263,6 +1273,32 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
     wave_or_atomic(&e.cnt->internal_error, err);
 }
 
+// The fishbone's work list: the hits with two or more marks from connect (a
+// hit with fewer has at most one used cell ending on it, so nothing to
+// compare), in hit order; one block per event, 16 hits per thread per round.
+__global__ void __launch_bounds__(256) k_fish_list(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ uint32_t lsum[4];
+    uint32_t ev, b;
+    if (!ev_block(grid, ev, b)) return;
+    const EvDesc &e = evs[ev];
+    const uint32_t n = e.n_hits;
+    uint32_t base = 0;
+    for (uint32_t h0 = 0; h0 < n; h0 += 256u * 16u) {
+        const uint32_t hb = h0 + 16u * threadIdx.x;
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 16u; ++j)
+            if (hb + j < n && e.s.fish_cnt[hb + j] >= 2u) m |= 1u << j;
+        uint32_t tot;
+        uint32_t pos = base + block_excl_scan4((uint32_t)__popc(m), lsum, tot);
+#pragma unroll
+        for (uint32_t j = 0; j < 16u; ++j)
+            if ((m >> j) & 1u) e.s.fish_hits[pos++] = hb + j;
+        base += tot;
+    }
+    if (threadIdx.x == 0) e.s.misc[kMiscFish] = base;
+}
+
 // ------------------------------------------------------------------ fishbone (gpu_fishbone.jl:4-62)
 // The serial loop, per hit, over the ascending cell list L (first 128):
 //   for ic, for jc > ic, both used: if collinear and on different modules:
@@ -1309,13 +1345,15 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
     }
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     FishLds &S = lds_all[w];
-    const uint32_t n_hits = e.n_hits;
+    // the hits on which connect used at least two cells (each listed once)
+    const uint32_t n_fh = min(e.s.misc[kMiscFish], e.n_hits);
     uint32_t killed = 0;
     for (;;) {
-        const uint32_t h0 = take_ticket(&e.s.misc[kMiscTkFish]) * kHitTile;
-        if (h0 >= n_hits) break;
-        const uint32_t h = h0 + lane;
-        const uint32_t cur = (lane < kHitTile && h < n_hits) ? e.s.ohc_cursor[h] : 0u;
+        const uint32_t f0 = take_ticket(&e.s.misc[kMiscTkFish]) * kHitTile;
+        if (f0 >= n_fh) break;
+        const bool mine = lane < kHitTile && f0 + lane < n_fh;
+        const uint32_t h = mine ? e.s.fish_hits[f0 + lane] : 0u;
+        const uint32_t cur = mine ? e.s.ohc_cursor[h] : 0u;
         const uint32_t R = (cur >= 2u) ? (cur < kOhcStride ? cur : kOhcStride) : 0u;
         for (uint32_t first = 0; first < kHitTile;) {
             const uint32_t v = (lane >= first) ? R : 0u;
@@ -1333,17 +1371,18 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                     const uint32_t rr = 64u * u + lane;
                     const int o = owner_of(rincl, rr);
                     const uint32_t s0 = (uint32_t)__shfl(rexcl, o, 64);
+                    const uint32_t ho = (uint32_t)__shfl(h, o, 64);  // the row's hit
                     uint2 en = make_uint2(0u, 0u);
                     bool used = false;
                     if (rr < rows) {
-                        en = e.s.ohc_list[ohc_index(h0 + (uint32_t)o, rr - s0)];
+                        en = e.s.ohc_list[ohc_index(ho, rr - s0)];
                         used = (e.s.cell_flags[en.x] & kUsed) != 0;
                     }
                     const uint64_t um = __ballot(used);
                     const uint32_t pos = n_used + (uint32_t)__popcll(um & lanes_below());
                     S.upre[rr] = pos;
                     if (used) {
-                        const float4 po = e.s.hxyzr[h0 + (uint32_t)o];
+                        const float4 po = e.s.hxyzr[ho];
                         const float4 pi = e.s.hxyzr[en.y];
                         const float dx = pi.x - po.x, dy = pi.y - po.y, dz = pi.z - po.z;
                         S.id[pos] = en.x;
@@ -2005,6 +2044,8 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
     mark("k_connect");
     if (cfg.p.early_fish_bone) {
+        k_fish_list<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one);
+        mark("k_fish_list");
         const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events);
         k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish, cfg);
     } else {

@@ -93,6 +93,8 @@ size_t carve(char *base, pt::Scratch *s) {
     t.tup_where = (uint2 *)take((size_t)kMaxCells * 8);
     t.tup_buf = (uint4 *)take((size_t)kTupBuf * 16);
     t.ohc_cursor = (uint32_t *)take(kMaxHits * 4);
+    t.fish_cnt = (uint32_t *)take(kMaxHits * 4);
+    t.fish_hits = (uint32_t *)take(kMaxHits * 4);
     t.ohc_list = (uint2 *)take(kOhcSize * 8);
     t.ohc_ovf = (uint32_t *)take(kOhcOvfCap * 4);
     t.trk_ovf = (uint32_t *)take(kTrkOvfCap * 4);

@@ -74,6 +74,8 @@ struct Scratch {
     PT_G uint2 *tup_where;       // [kMaxCells]    per start-list entry: (buffer offset, tuples) or ~0
     PT_G uint4 *tup_buf;         // [kTupBuf]      cells of the tuples kept by the count pass
     PT_G uint32_t *ohc_cursor;   // [kMaxHits]     cells whose outer hit is h (all, not capped)
+    PT_G uint32_t *fish_cnt;     // [kMaxHits]     cells ending on h that connect used
+    PT_G uint32_t *fish_hits;    // [kMaxHits]     hits with at least two of them (fishbone work)
     PT_G uint2 *ohc_list;        // [kOhcSize] (cell, inner hit) 0-based, unordered unless
                             //                the hit overflowed (then the first 128 cells, ascending)
     PT_G uint32_t *ohc_ovf;      // [kOhcOvfCap]   hits with more than kOhcStride cells
@@ -121,6 +123,7 @@ enum Misc : int {
     kMiscTupBuf = 12,     // entries of tup_buf
     kMiscDc = 13,         // shared cells (classification)
     kMiscH2tEnt = 14,     // hit -> loose-tuple entries (classification)
+    kMiscFish = 15,       // entries of fish_hits
 };
 
 // Per-event descriptor: inputs, outputs and scratch.  Built on the host,
