@@ -986,8 +986,8 @@ constexpr uint32_t kConnStage = 192;  // columns staged at a time
 struct ConnLds {
     float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
     uint32_t off[6][kConnHits], cum[6][kConnHits];  // hit lane's column c of range q: cell off[q] + c
-    uint32_t coh[kConnStage];                       // staged columns: outer hit, its x, y, z, r
-    float4 cxyzr[kConnStage];
+    uint32_t coh[kConnStage];                       // staged columns: outer hit, its (z, r) and (x, y)
+    float2 czr[kConnStage], cxy[kConnStage];
     // the row batch's rows (per lane), and the pairs waiting for the dca cut:
     // row lane | staged column << 8
     float rx1[64], ry1[64], rreg[64];
@@ -1083,7 +1083,9 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                         for (int q = 0; q < 5; ++q) base = (c >= S.cum[q][ohl]) ? S.off[q + 1][ohl] : base;
                         const uint32_t t = base + c;
                         S.coh[g] = (e.s.cell_io[t] >> 16) - 1u;
-                        S.cxyzr[g] = e.s.cell_oxyzr[t];
+                        const float4 o = e.s.cell_oxyzr[t];
+                        S.czr[g] = make_float2(o.z, o.w);
+                        S.cxy[g] = make_float2(o.x, o.y);
                     }
                 }
                 wave_sync();
@@ -1130,15 +1132,15 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                     const uint32_t smax = wave_max(span);
                     for (uint32_t j0 = 0; j0 < smax; j0 += kConnCols) {
                         uint32_t sl[kConnCols];
-                        float4 o3[kConnCols];
+                        float2 o3[kConnCols];  // (z, r) of the column's outer hit
                         bool al[kConnCols];
                         uint32_t nq = 0, qi[kConnCols];
 #pragma unroll
                         for (int k = 0; k < kConnCols; ++k) {
                             const bool in = j0 + (uint32_t)k < span;
                             sl[k] = in ? lo + j0 + (uint32_t)k : 0u;
-                            o3[k] = S.cxyzr[sl[k]];
-                            al[k] = in && are_aligned(r1, z1, ri, zi, o3[k].w, o3[k].z, cfg.p.pt_min, th);
+                            o3[k] = S.czr[sl[k]];
+                            al[k] = in && are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
                             const uint64_t am = __ballot(al[k]);
                             qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
                             nq += (uint32_t)__popcll(am);
@@ -1154,7 +1156,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                             if (qb + lane < nq) {
                                 const uint32_t v = S.q[lane], rl = v & 63u;
                                 const uint32_t hh = S.rhl[rl];
-                                const float4 o = S.cxyzr[v >> 8];
+                                const float2 o = S.cxy[v >> 8];
                                 ok = dca_cut(S.rx1[rl], S.ry1[rl], S.hx[hh], S.hy[hh], o.x, o.y, S.rreg[rl],
                                              cfg.p.hard_curv_cut);
                             }

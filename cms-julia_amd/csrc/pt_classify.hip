@@ -162,12 +162,77 @@ __device__ __forceinline__ void cl_prepare(const EvDesc &e) {
     }
 }
 
+// The loop body of kernel_fast_duplicate_remover for one shared cell, with
+// list L of n tuples whose first four ids and |tip| are given (tips do not
+// change while it runs; qualities do, so they are read now): the loose tuple
+// with the smallest |tip| stays (ties: the smallest id, the first in the
+// list), every other tuple that is not bad becomes dup.
+__device__ __forceinline__ void cl_resolve(const EvDesc &e, const uint32_t *L, uint32_t n, const uint32_t (&t4)[4],
+                                           const float (&tip4)[4]) {
+    int8_t q4[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q4[j] = e.quality[t4[j] == ~0u ? 0u : t4[j]];
+    float mc = 10000.0f;
+    uint32_t im = ~0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (t4[j] != ~0u && q4[j] == PT_QUALITY_LOOSE && (tip4[j] < mc || (tip4[j] == mc && im != ~0u && t4[j] < im))) {
+            mc = tip4[j];
+            im = t4[j];
+        }
+    for (uint32_t k0 = 4; k0 < n; k0 += 4u) {  // lists longer than four (rare)
+        uint32_t t[4];
+        float sc[4];
+        int8_t ql[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t tt = t[j] == ~0u ? 0u : t[j];
+            sc[j] = abs_tip(e, tt);
+            ql[j] = e.quality[tt];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (t[j] != ~0u && ql[j] == PT_QUALITY_LOOSE && (sc[j] < mc || (sc[j] == mc && im != ~0u && t[j] < im))) {
+                mc = sc[j];
+                im = t[j];
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (t4[j] != ~0u && t4[j] != im && q4[j] != PT_QUALITY_BAD) e.quality[t4[j]] = PT_QUALITY_DUP;
+    for (uint32_t k0 = 4; k0 < n; k0 += 4u) {
+        uint32_t t[4];
+        int8_t ql[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ql[j] = e.quality[t[j] == ~0u ? 0u : t[j]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (t[j] != ~0u && t[j] != im && ql[j] != PT_QUALITY_BAD) e.quality[t[j]] = PT_QUALITY_DUP;
+    }
+}
+
+// the first four tuple ids of a list and their |tip|
+__device__ __forceinline__ void cl_head(const EvDesc &e, const uint32_t *L, uint32_t n, uint32_t (&t4)[4],
+                                        float (&tip4)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t4[j] = ((uint32_t)j < n) ? L[j] : ~0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tip4[j] = abs_tip(e, t4[j] == ~0u ? 0u : t4[j]);
+}
+
 // Phase 3: kernel_fast_duplicate_remover in rounds; the round that resolved a
 // shared cell is its stamp (0: not yet), in LDS for the first kDoneLds cells.
+// A thread's first kRoundCache cells keep their list head, tips and
+// predecessors in registers across the rounds.
 // The count of cells still pending after round r is s_pending[r % 3]: thread
 // 0 clears the counter of round r + 1 during round r, when every wave has
 // passed the barrier that follows its read of that counter in round r - 2, so
 // no wave can read a counter that was cleared under it.
+constexpr uint32_t kRoundCache = 2;
 __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uint32_t *s_pending) {
     const uint32_t nd = n_shared(e);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
@@ -175,78 +240,77 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
     if (tid == 0) s_pending[1] = 0u;
     // the stamps beyond kDoneLds were cleared by the multiplicity kernel
     auto done_of = [&](uint32_t s) -> uint32_t { return s < kDoneLds ? s_done[s] : e.s.dc_done[s]; };
+    // predecessors ready: each resolved in an earlier round
+    auto ready_of = [&](uint32_t s, const uint4 &pred, const uint32_t *L, uint32_t n, uint32_t round) {
+        bool ready = true;
+        if (pred.x != kPredMany) {
+            const uint32_t ps[4] = {pred.x, pred.y, pred.z, pred.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ps[q] == ~0u) continue;
+                const uint32_t d = done_of(ps[q]);
+                if (d == 0u || d == round) ready = false;
+            }
+        } else {
+            const uint32_t c = e.s.dc_cells[s];
+            for (uint32_t k = 0; k < n && ready; ++k) {
+                uint32_t cs[4], sl[4];
+                bool v[4];
+                shared_cells_of(e, L[k], cs, sl, v);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (!v[q] || cs[q] >= c) continue;
+                    const uint32_t d = done_of(sl[q]);
+                    if (d == 0u || d == round) ready = false;
+                }
+            }
+        }
+        return ready;
+    };
+    uint32_t c_n[kRoundCache], c_t[kRoundCache][4];
+    const uint32_t *c_L[kRoundCache];
+    uint4 c_pred[kRoundCache];
+    float c_tip[kRoundCache][4];
+#pragma unroll
+    for (uint32_t k = 0; k < kRoundCache; ++k) {
+        const uint32_t s = tid + k * nthr;
+        const bool in = s < nd;
+        c_L[k] = e.s.dc_ent + (in ? e.s.dc_off[s] : 0u);
+        c_n[k] = in ? min(e.s.dc_cur[s], PT_MAX_CELL_TRACKS) : 0u;
+        c_pred[k] = in ? e.s.dc_pred[s] : make_uint4(~0u, ~0u, ~0u, ~0u);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRoundCache; ++k) cl_head(e, c_L[k], c_n[k], c_t[k], c_tip[k]);
     __syncthreads();
     for (uint32_t round = 1;; ++round) {
         if (tid == 0) s_pending[(round + 1u) % 3u] = 0u;
         uint32_t pending = 0;
-        for (uint32_t s = tid; s < nd; s += nthr) {
-            if (done_of(s)) continue;
-            const uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
-            const uint32_t n = min(e.s.dc_cur[s], PT_MAX_CELL_TRACKS);
-            const uint4 pred = e.s.dc_pred[s];
-            bool ready = true;
-            if (pred.x != kPredMany) {
-                const uint32_t ps[4] = {pred.x, pred.y, pred.z, pred.w};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (ps[q] == ~0u) continue;
-                    const uint32_t d = done_of(ps[q]);
-                    if (d == 0u || d == round) ready = false;
-                }
-            } else {
-                const uint32_t c = e.s.dc_cells[s];
-                for (uint32_t k = 0; k < n && ready; ++k) {
-                    uint32_t cs[4], sl[4];
-                    bool v[4];
-                    shared_cells_of(e, L[k], cs, sl, v);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (!v[q] || cs[q] >= c) continue;
-                        const uint32_t d = done_of(sl[q]);
-                        if (d == 0u || d == round) ready = false;
-                    }
-                }
-            }
-            if (!ready) {
+        for (uint32_t k = 0; k < kRoundCache; ++k) {
+            const uint32_t s = tid + k * nthr;
+            if (s >= nd || done_of(s)) continue;
+            if (!ready_of(s, c_pred[k], c_L[k], c_n[k], round)) {
                 ++pending;
                 continue;
             }
-            // the loop body for this cell: the loose tuple with the smallest
-            // |tip| stays (ties: the smallest id, the first in the list), every
-            // other tuple that is not bad becomes dup; four entries at a time
-            float mc = 10000.0f;
-            uint32_t im = ~0u;
-            for (uint32_t k0 = 0; k0 < n; k0 += 4u) {
-                uint32_t t[4];
-                float sc[4];
-                int8_t ql[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t tt = t[j] == ~0u ? 0u : t[j];
-                    sc[j] = abs_tip(e, tt);
-                    ql[j] = e.quality[tt];
-                }
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (t[j] != ~0u && ql[j] == PT_QUALITY_LOOSE &&
-                        (sc[j] < mc || (sc[j] == mc && im != ~0u && t[j] < im))) {
-                        mc = sc[j];
-                        im = t[j];
-                    }
+            cl_resolve(e, c_L[k], c_n[k], c_t[k], c_tip[k]);
+            if (s < kDoneLds)
+                s_done[s] = round;
+            else
+                e.s.dc_done[s] = round;
+        }
+        for (uint32_t s = tid + kRoundCache * nthr; s < nd; s += nthr) {
+            if (done_of(s)) continue;
+            const uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
+            const uint32_t n = min(e.s.dc_cur[s], PT_MAX_CELL_TRACKS);
+            if (!ready_of(s, e.s.dc_pred[s], L, n, round)) {
+                ++pending;
+                continue;
             }
-            for (uint32_t k0 = 0; k0 < n; k0 += 4u) {
-                uint32_t t[4];
-                int8_t ql[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) t[j] = (k0 + (uint32_t)j < n) ? L[k0 + (uint32_t)j] : ~0u;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) ql[j] = e.quality[t[j] == ~0u ? 0u : t[j]];
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (t[j] != ~0u && t[j] != im && ql[j] != PT_QUALITY_BAD) e.quality[t[j]] = PT_QUALITY_DUP;
-            }
+            uint32_t t4[4];
+            float tip4[4];
+            cl_head(e, L, n, t4, tip4);
+            cl_resolve(e, L, n, t4, tip4);
             if (s < kDoneLds)
                 s_done[s] = round;
             else
