@@ -163,6 +163,47 @@ def test_device_batch_matches_oracle(frames):
     ctx.close()
 
 
+@pytest.mark.parametrize("offset", [0, 4])
+def test_device_entry_clears_dirty_outputs(frames, offset):
+    """With out_zeroed = 0 the device entry point clears state and covariance of
+    a reused TrackSOA itself: 16-byte aligned buffers are cleared in the doublet
+    count's tail, others (offset 4) in the reset kernel.  The result must equal a
+    run into cleared buffers, bit for bit."""
+    import ctypes as C
+    events = [G.generate_event("pu50", seed=s) for s in (15, 16)]
+    ctx = ca.Context(max_events=len(events), frames=frames)
+    clean = ca.DeviceBatch(ctx, events)
+    dirty = ca.DeviceBatch(ctx, events)
+    extra = []
+    try:
+        print(f"[parity] device entry, dirty state/covariance, offset {offset}", flush=True)
+        clean.run_async(out_zeroed=True)
+        ctx.sync()
+        ref = [clean.download(i) for i in range(len(events))]
+        for i in range(len(events)):
+            o = dirty.outs[i]
+            for name, count in (("state", 5 * A.MAX_TUPLES), ("covariance", 15 * A.MAX_TUPLES)):
+                p = ctx.malloc(4 * count + 16)
+                extra.append(p)
+                ctx.memset(p, 0xFF, 4 * count + 16)  # NaN everywhere
+                setattr(o, name, C.cast(C.c_void_p(p + offset), type(getattr(o, name))))
+        for _ in range(2):  # the second pass clears what the first one wrote
+            dirty.run_async(out_zeroed=False)
+            ctx.sync()
+            for i in range(len(events)):
+                got = dirty.download(i)
+                m = min(ctx.counters(i)["n_tuples"], A.MAX_TUPLES)  # quality is defined for stored rows
+                np.testing.assert_array_equal(got.quality[:m], ref[i].quality[:m], err_msg=i)
+                for k in ("state", "covariance", "hit_off"):
+                    np.testing.assert_array_equal(getattr(got, k), getattr(ref[i], k), err_msg=(i, k))
+    finally:
+        for p in extra:
+            ctx.free(p)
+        clean.free()
+        dirty.free()
+        ctx.close()
+
+
 def test_fit_only_fp64_matches_oracle():
     tr = G.fit_tracks(20000, seed=7)
     ctx = ca.Context(max_events=1)
