@@ -364,24 +364,25 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
     const uint32_t oi = q.w >> 16;
     const bool in_layer = oi > s.lo && oi <= s.hi;
     ref_bits |= in_layer ? 0u : PT_REF_HIST_LAYER;
-    bool ok = in_layer || (oi != 0 && oi <= e.n_hits);
-    ok = ok && (int16_t)(q.z >> 16) <= 2000;
+    // (bitwise & and |: every operand is cheap and safe to evaluate, so no branches)
+    bool ok = in_layer | ((oi != 0) & (oi <= e.n_hits));
+    ok &= (int16_t)(q.z >> 16) <= 2000;
     const float zo = __uint_as_float(q.x), ro = __uint_as_float(q.y);
     if (P.do_z0_cut) {
         // gpu_pixel_doublets.jl:69-74
         const float dr = ro - s.me_r;
-        ok = ok && !(dr > s.max_r || dr < 0.0f || fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr);
+        ok &= !((dr > s.max_r) | (dr < 0.0f) | (fabsf(s.me_z * ro - s.me_r * zo) > 12.0f * dr));
     }
     const int16_t cut = (int16_t)(s.cut & 0xFFFFu);
     int16_t i_dphi = (int16_t)((int16_t)(q.z & 0xFFFFu) - slot_me_p(s));
     i_dphi = (int16_t)(i_dphi < 0 ? -i_dphi : i_dphi);  // Int16 abs wraps at -32768
-    ok = ok && !(i_dphi > cut);
+    ok &= !(i_dphi > cut);
     if (P.do_cluster_cut) {
         // gpu_pixel_doublets.jl:79-88
         const int me_s = s.me_s;
         if (outer < 4) {
             const int so = (int16_t)(q.w & 0xFFFFu);
-            ok = ok && !(me_s > 0 && so > 0 && abs(so - me_s) > ((inner == 0) ? 28 : 20));
+            ok &= !((me_s > 0) & (so > 0) & (abs(so - me_s) > ((inner == 0) ? 28 : 20)));
         } else if (inner < 4 && me_s > 0 && ok) {
             const float qq = fabsf((s.me_z - zo) / (s.me_r - ro)) * (float)(8 * 0.0285 / 0.015);
             const double t = trunc((double)qq + 0.5);
@@ -390,7 +391,7 @@ __device__ __forceinline__ bool cand_pass(const EvDesc &e, const Config &cfg, co
             ok = !bad && !(llabs((long long)me_s - (long long)(bad ? 0.0 : t)) > 20);
         }
     }
-    if (P.do_pt_cut) ok = ok && !pt_cut(ro, i_dphi, s.me_r);
+    if (P.do_pt_cut) ok &= !pt_cut(ro, i_dphi, s.me_r);
     return ok;
 }
 
@@ -591,7 +592,12 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
                 }
 #pragma unroll
                 for (int u = 0; u < kEvalU; ++u) {
-                    const bool ok = base + 64u * u + lane < wn && cand_pass(e, cfg, c[u].s, q[u], ref);
+                    // every lane evaluates the cuts (no branch); only real items count
+                    const bool valid = base + 64u * u + lane < wn;
+                    uint32_t r = 0;
+                    const bool pass = cand_pass(e, cfg, c[u].s, q[u], r);
+                    ref |= valid ? r : 0u;
+                    const bool ok = valid & pass;
                     const uint64_t bal = __ballot(ok);
                     if (lane == 0) pbits[w][(base >> 6) + u] = bal;
                 }
@@ -1135,12 +1141,18 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                         float2 o3[kConnCols];  // (z, r) of the column's outer hit
                         bool al[kConnCols];
                         uint32_t nq = 0, qi[kConnCols];
+                        // the round's reads first, then the cuts, then the ballots
 #pragma unroll
                         for (int k = 0; k < kConnCols; ++k) {
-                            const bool in = j0 + (uint32_t)k < span;
-                            sl[k] = in ? lo + j0 + (uint32_t)k : 0u;
+                            sl[k] = (j0 + (uint32_t)k < span) ? lo + j0 + (uint32_t)k : 0u;
                             o3[k] = S.czr[sl[k]];
-                            al[k] = in && are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
+                        }
+#pragma unroll
+                        for (int k = 0; k < kConnCols; ++k)
+                            al[k] = (j0 + (uint32_t)k < span) &
+                                    are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
+#pragma unroll
+                        for (int k = 0; k < kConnCols; ++k) {
                             const uint64_t am = __ballot(al[k]);
                             qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
                             nq += (uint32_t)__popcll(am);
