@@ -985,7 +985,7 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 // cell's neighbour list.  The alignment cut runs in every lane; the dca cut
 // (about 2% of the pairs get there) runs on the survivors of a whole round of
 // columns, compacted through LDS.
-constexpr uint32_t kConnHits = 32;
+constexpr uint32_t kConnHits = 24;
 constexpr int kConnCols = 4;
 constexpr uint32_t kConnStage = 192;  // columns staged at a time
 
