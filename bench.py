@@ -279,7 +279,7 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=100)
     ap.add_argument("--preset", default="pu50")
