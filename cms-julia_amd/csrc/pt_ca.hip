@@ -751,9 +751,8 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
             i1 = ((p == 0) ? js : js - PT.slot_cum[p - 1]) + PT.ls_inner[p] + 1u;
             oh = (q.w >> 16) - 1u;
             e.s.cell_io[cid] = i1 | ((oh + 1u) << 16);
-            e.s.cell_flags[cid] = 0;
             e.s.cell_pair[cid] = (uint8_t)p;
-            e.s.cell_nn[cid] = 0;
+            *reinterpret_cast<PT_G uint16_t *>(e.s.cell_nn + 2u * cid) = 0;  // neighbours and flags
             e.s.cell_oxyzr[cid] = make_float4(xy.x, xy.y, __uint_as_float(q.x), __uint_as_float(q.y));
         };
         auto list_add = [&](uint32_t oh, uint32_t i1, uint32_t cid, uint32_t r) {
@@ -1188,7 +1187,7 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
 #pragma unroll
                                 for (int q = 0; q < 5; ++q) base = (c >= S.cum[q][ohl]) ? S.off[q + 1][ohl] : base;
                                 const uint32_t t = base + c;
-                                e.s.cell_flags[t] = kUsed;
+                                e.s.cell_flags[2u * t] = kUsed;
                                 // marks per outer hit (a cell can be counted more than once):
                                 // the fishbone only looks at hits with two or more
                                 atomicAdd(&e.s.fish_cnt[S.coh[sl[k]]], 1u);
@@ -1204,10 +1203,10 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                 bool st = false;
                 if (cnt) {
                     const uint32_t nn = cnt < PT_MAX_NEIGHBORS ? cnt : PT_MAX_NEIGHBORS;
-                    e.s.cell_flags[cell] = kUsed;
+                    e.s.cell_flags[2u * cell] = kUsed;
                     atomicAdd(&e.s.fish_cnt[h0 + ohl], 1u);
                     e.s.trk[cell] = make_uint2(0u, 0u);
-                    e.s.cell_nn[cell] = (uint8_t)nn;
+                    e.s.cell_nn[2u * cell] = (uint8_t)nn;
                     e.s.nbr4[cell] = nb;
                     n_conn += nn;
                     ++n_lists;
@@ -1245,11 +1244,11 @@ __device__ __forceinline__ void connect_ext_items(const EvDesc &e, const Config 
     const uint32_t n_ext = e.s.misc[kMiscExt];
     for (uint32_t i = i0; i < n_ext; i += stride) {
         const uint32_t o = e.s.ext_list[i];
-        const uint32_t nkeep = e.s.cell_nn[o];
+        const uint32_t nkeep = e.s.cell_nn[2u * o];
         const uint32_t base = atomicAdd(&e.s.misc[kMiscPool], nkeep - 4u);
         if (base + (nkeep - 4u) > kNbrPool) {
             err |= PT_INT_NBR_POOL;
-            e.s.cell_nn[o] = 4;
+            e.s.cell_nn[2u * o] = 4;
             continue;
         }
         e.s.nbr_ext[o] = base;
@@ -1390,7 +1389,7 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                     bool used = false;
                     if (rr < rows) {
                         en = e.s.ohc_list[ohc_index(ho, rr - s0)];
-                        used = (e.s.cell_flags[en.x] & kUsed) != 0;
+                        used = (e.s.cell_flags[2u * en.x] & kUsed) != 0;
                     }
                     const uint64_t um = __ballot(used);
                     const uint32_t pos = n_used + (uint32_t)__popcll(um & lanes_below());
@@ -1466,7 +1465,7 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
                             }
                         }
                         if (kill) {
-                            e.s.cell_flags[myid] = kUsed | kKilled;
+                            e.s.cell_flags[2u * myid] = kUsed | kKilled;
                             ++killed;
                         }
                     }
@@ -1495,8 +1494,9 @@ __device__ __forceinline__ DNode load_node(const EvDesc &e, uint32_t c) {
     DNode n;
     n.id = c;
     n.io = e.s.cell_io[c];
-    n.nn = e.s.cell_nn[c];
-    n.killed = (e.s.cell_flags[c] & kKilled) != 0;
+    const uint32_t fn = *reinterpret_cast<const PT_G uint16_t *>(e.s.cell_nn + 2u * c);  // one load for both
+    n.nn = fn & 0xFFu;
+    n.killed = ((fn >> 8) & kKilled) != 0;
     n.nb = e.s.nbr4[c];
     return n;
 }
@@ -1544,7 +1544,7 @@ __device__ __forceinline__ void dfs_level(const EvDesc &e, const DNode &node, ui
             const uint32_t ext = (nn > 4u) ? e.s.nbr_ext[node.id] : 0u;
             for (uint32_t i = 0; i < nn; ++i) {
                 const uint32_t n = (i < 4u) ? comp4(node.nb, i) : e.s.nbr_pool[ext + i - 4u];
-                if (e.s.cell_flags[n] & kKilled) continue;
+                if (e.s.cell_flags[2u * n] & kKilled) continue;
                 child = true;
                 ref_bits |= PT_REF_DEPTH;
             }

@@ -77,9 +77,9 @@ size_t carve(char *base, pt::Scratch *s) {
     t.slot_pos = (uint4 *)take((size_t)kMaxSlots * 16);
     t.dbl_agg = (uint32_t *)take(kDblAgg * 4);
     t.cell_io = (uint32_t *)take(kMaxCells * 4);
-    t.cell_flags = (uint8_t *)take(kMaxCells);
     t.cell_pair = (uint8_t *)take(kMaxCells);
-    t.cell_nn = (uint8_t *)take(kMaxCells);
+    t.cell_nn = (uint8_t *)take(2 * (size_t)kMaxCells);  // (neighbours, flags) per cell
+    t.cell_flags = t.cell_nn + 1;
     t.cell_oxyzr = (float4 *)take((size_t)kMaxCells * 16);
     t.nbr4 = (uint4 *)take((size_t)kMaxCells * 16);
     t.nbr_ext = (uint32_t *)take(kMaxCells * 4);
@@ -721,11 +721,12 @@ int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
     d->n_cells = nc;
     if (nc > d->cap_cells) return fail(PT_ERR_CAPACITY, "debug capacity %u < %u cells", d->cap_cells, nc);
     std::vector<uint32_t> io(nc);
-    std::vector<uint8_t> fl(nc);
+    std::vector<uint8_t> fl(nc), fn(2 * (size_t)nc);  // (neighbours, flags) pairs
     if (nc) {
         HIP_OK(hipMemcpy(io.data(), s.cell_io, (size_t)nc * 4, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(fl.data(), s.cell_flags, nc, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(fn.data(), s.cell_nn, fn.size(), hipMemcpyDeviceToHost));
     }
+    for (uint32_t i = 0; i < nc; ++i) fl[i] = fn[2 * (size_t)i + 1];
     std::vector<uint8_t> pr(nc);
     if (nc) HIP_OK(hipMemcpy(pr.data(), s.cell_pair, nc, hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < nc; ++i) {
@@ -759,7 +760,9 @@ int pt_get_ca_debug(pt_ctx *c, uint32_t event, pt_ca_debug *d) {
         std::vector<uint4> v4(nc);
         std::vector<uint32_t> ext(nc), pool(pt::kNbrPool);
         if (nc) {
-            HIP_OK(hipMemcpy(nn.data(), s.cell_nn, nc, hipMemcpyDeviceToHost));
+            std::vector<uint8_t> pairs(2 * (size_t)nc);
+            HIP_OK(hipMemcpy(pairs.data(), s.cell_nn, pairs.size(), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < nc; ++i) nn[i] = pairs[2 * (size_t)i];
             HIP_OK(hipMemcpy(v4.data(), s.nbr4, (size_t)nc * 16, hipMemcpyDeviceToHost));
             HIP_OK(hipMemcpy(ext.data(), s.nbr_ext, (size_t)nc * 4, hipMemcpyDeviceToHost));
             HIP_OK(hipMemcpy(pool.data(), s.nbr_pool, pool.size() * 4, hipMemcpyDeviceToHost));

@@ -58,9 +58,10 @@ struct Scratch {
     PT_G uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
     PT_G uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kSlotTile slots
     PT_G uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
-    PT_G uint8_t *cell_flags;    // [kMaxCells]    kUsed (connect), kKilled (fishbone)
+    PT_G uint8_t *cell_flags;    // [2*kMaxCells]  byte 2c+1 of the pair array: kUsed (connect), kKilled (fishbone)
     PT_G uint8_t *cell_pair;     // [kMaxCells]    layer pair
-    PT_G uint8_t *cell_nn;       // [kMaxCells]    stored outer neighbours (<= 64)
+    PT_G uint8_t *cell_nn;       // [2*kMaxCells]  byte 2c: stored outer neighbours (<= 64); cell_flags = cell_nn + 1,
+                            //                so one 16-bit load reads both (index 2c)
     PT_G float4 *cell_oxyzr;     // [kMaxCells]    (x, y, z, r) of the outer hit
     PT_G uint4 *nbr4;            // [kMaxCells]    first 4 neighbours (0-based cell ids, ascending)
     PT_G uint32_t *nbr_ext;      // [kMaxCells]    pool offset of neighbours 5..64
