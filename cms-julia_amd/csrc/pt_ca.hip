@@ -450,8 +450,6 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
         if (k < e.n_hits) {
             e.s.ohc_cursor[k] = 0u;
             e.s.fish_cnt[k] = 0u;
-            e.s.h2t_cnt[k] = 0u;
-            e.s.h2t_max[k] = 0u;
             e.s.h2t_win[k] = ~0ull;
             const float z = e.zg[k], r = e.rg[k];
             e.s.hxyzr[k] = make_float4(e.xg[k], e.yg[k], z, r);

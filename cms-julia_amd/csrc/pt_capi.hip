@@ -101,8 +101,6 @@ size_t carve(char *base, pt::Scratch *s) {
     t.tuple_cells = (uint32_t *)take(kMaxTuples * 4 * 4);
     t.tm_off = (uint32_t *)take(16 * 4);
     t.tm_bins = (uint16_t *)take(kMaxTuples * 2);
-    t.h2t_cnt = (uint32_t *)take(kMaxHits * 4);
-    t.h2t_max = (uint32_t *)take(kMaxHits * 4);
     t.h2t_win = (uint64_t *)take((size_t)kMaxHits * 8);
     t.misc = (uint32_t *)take(16 * 4);
     t.hxyzr = (float4 *)take(kMaxHits * 16);

@@ -17,11 +17,14 @@
 //     ready, so the rounds end; on PU50 events there are about four;
 //   * the triplet cleaner only ever writes dup, and its tests (!= bad) do not
 //     change while it runs (every tuple on a hit's list was loose), so its
-//     decision for a tuple depends only on, per hit of the tuple: the number
-//     of loose tuples on the hit, their largest hit count and the one with
-//     the smallest |tip| (ties: the first of the hit's list, which holds the
-//     tuples in descending id order by fill_direct, i.e. the largest id).
-//     These three are atomic reductions per hit; the decision is per tuple.
+//     decision for a tuple depends only on, per hit of the tuple: the largest
+//     hit count of the loose tuples on the hit and the one with the smallest
+//     |tip| (ties: the first of the hit's list, which holds the tuples in
+//     descending id order by fill_direct, i.e. the largest id).  Both come
+//     from one 64-bit atomic minimum per hit; the decision is per tuple.  (The
+//     reference skips hits with a single loose tuple; for those the largest
+//     count is the tuple's own and it is the smallest |tip|, so the test
+//     below passes them too.)
 #include <hip/hip_runtime.h>
 
 #include "pt_device.hpp"
@@ -339,19 +342,19 @@ __device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
         const float tip = abs_tip(e, t);
         const uint32_t nh = loose ? b1 - b0 : 0u;
         if (loose) {
-            const uint64_t key = ((uint64_t)__float_as_uint(tip) << 32) | (uint64_t)(0xFFFFFFFFu - t);
+            // smallest key: the largest hit count, then the smallest |tip|, then the
+            // largest id (t < 32768 = kMaxTuples, |tip| bits < 2^31)
+            const uint64_t key = ((uint64_t)(15u - nh) << 60) | ((uint64_t)__float_as_uint(tip) << 15) |
+                                 (uint64_t)(0x7FFFu - t);
             for (uint32_t p0 = b0; p0 < b1; p0 += kHitsAtOnce) {
                 uint32_t h[kHitsAtOnce];
 #pragma unroll
                 for (uint32_t j = 0; j < kHitsAtOnce; ++j) h[j] = (p0 + j < b1) ? e.hit_bins[p0 + j] - 1u : ~0u;
 #pragma unroll
                 for (uint32_t j = 0; j < kHitsAtOnce; ++j)
-                    if (h[j] != ~0u) {
-                        atomicAdd(&e.s.h2t_cnt[h[j]], 1u);
-                        atomicMax(&e.s.h2t_max[h[j]], nh);
+                    if (h[j] != ~0u)
                         atomicMin(reinterpret_cast<unsigned long long *>(&e.s.h2t_win[h[j]]),
                                   (unsigned long long)key);
-                    }
             }
         }
         // entries of the wave: tuple sizes are below 8, sum them bit by bit
@@ -375,20 +378,21 @@ __device__ __forceinline__ void cl_h2t_decide(const EvDesc &e) {
             const uint32_t nh = b1 - b0;
             bool dup = false;
             for (uint32_t p0 = b0; p0 < b1; p0 += kHitsAtOnce) {
-                uint32_t h[kHitsAtOnce], cn[kHitsAtOnce], mx[kHitsAtOnce], win[kHitsAtOnce];
+                uint32_t h[kHitsAtOnce];
+                uint64_t kw[kHitsAtOnce];
 #pragma unroll
                 for (uint32_t j = 0; j < kHitsAtOnce; ++j) h[j] = (p0 + j < b1) ? e.hit_bins[p0 + j] - 1u : ~0u;
 #pragma unroll
                 for (uint32_t j = 0; j < kHitsAtOnce; ++j) {
                     const uint32_t hh = h[j] == ~0u ? 0u : h[j];
-                    cn[j] = e.s.h2t_cnt[hh];
-                    mx[j] = e.s.h2t_max[hh];
-                    win[j] = (uint32_t)e.s.h2t_win[hh];
+                    kw[j] = e.s.h2t_win[hh];
                 }
 #pragma unroll
-                for (uint32_t j = 0; j < kHitsAtOnce; ++j)
-                    if (h[j] != ~0u && cn[j] >= 2u && (nh != mx[j] || (mx[j] <= 3u && win[j] != 0xFFFFFFFFu - t)))
-                        dup = true;
+                for (uint32_t j = 0; j < kHitsAtOnce; ++j) {
+                    const uint32_t mx = 15u - (uint32_t)(kw[j] >> 60);
+                    const uint32_t wt = 0x7FFFu - (uint32_t)(kw[j] & 0x7FFFu);
+                    if (h[j] != ~0u && (nh != mx || (mx <= 3u && wt != t))) dup = true;
+                }
             }
             if (dup)
                 e.quality[t] = PT_QUALITY_DUP;

@@ -84,9 +84,8 @@ struct Scratch {
     PT_G uint32_t *tuple_cells;  // [kMaxTuples*4] cells of each stored tuple
     PT_G uint32_t *tm_off;       // [16]           tuple multiplicity (9 used)
     PT_G uint16_t *tm_bins;      // [kMaxTuples]
-    PT_G uint32_t *h2t_cnt;      // [kMaxHits]     loose tuples on the hit (classification)
-    PT_G uint32_t *h2t_max;      // [kMaxHits]     their largest hit count
-    PT_G uint64_t *h2t_win;      // [kMaxHits]     min over them of (|tip| bits << 32 | ~tuple id)
+    PT_G uint64_t *h2t_win;      // [kMaxHits]     min over the loose tuples on the hit of
+                            //                (15 - hits) << 60 | |tip| bits << 15 | (32767 - tuple id)
     PT_G uint32_t *misc;         // [16]           internal scalars (see Misc)
     PT_G float4 *hxyzr;          // [kMaxHits]     packed (x, y, z, r) per hit
     PT_G uint4 *hdbl;            // [kMaxHits]     packed (z, r, iphi | det << 16, ysize) per hit
