@@ -975,13 +975,15 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 // per ticket and sorts them by column count, descending (so that rows of
 // similar length share a round).  Consecutive sorted hits form groups whose
 // columns fit the LDS staging area; a hit with more columns is a group of its
-// own and is staged in chunks.  A group's columns - cell id and outer-hit
-// coordinates - are loaded once into LDS, in id order per hit; then every
-// lane takes one row and walks its hit's staged columns kConnCols at a time,
-// keeping the passing ones in order - the serial push! order of the row
-// cell's neighbour list.  The alignment cut runs in every lane; the dca cut
-// (about 2% of the pairs get there) runs on the survivors of a whole round of
-// columns, compacted through LDS.
+// own and is staged in chunks.  A group's columns - the outer hit and its
+// coordinates - are loaded once into LDS, in id order per hit (a column's cell
+// id is its hit's range offset plus its index, recomputed when it passes);
+// then every lane takes one row and walks its hit's staged columns kConnCols
+// at a time, keeping the passing ones in order - the serial push! order of the
+// row cell's neighbour list.  The alignment cut runs in every lane; the dca
+// cut (about 2% of the pairs get there) runs on the survivors of a whole round
+// of columns, compacted through LDS.  Each mark of a used cell is counted per
+// outer hit for the fishbone's work list.
 constexpr uint32_t kConnHits = 24;
 constexpr int kConnCols = 4;
 constexpr uint32_t kConnStage = 192;  // columns staged at a time
