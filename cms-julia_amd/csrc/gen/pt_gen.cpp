@@ -246,6 +246,12 @@ int ptg_frames(float *rot, uint32_t n_modules) {
     return PT_OK;
 }
 
+int ptg_iphi(uint32_t n, const float *x, const float *y, int16_t *iphi) {
+    if (n && (!x || !y || !iphi)) return PT_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i) iphi[i] = unsafe_atan2s(y[i], x[i]);
+    return PT_OK;
+}
+
 int ptg_build_phi_hist(const int16_t *iphi, const uint32_t *layer_start, uint32_t *off, uint16_t *bins) {
     const uint32_t nh = 10;
     for (uint32_t b = 0; b < PT_HIST_NOFF; ++b) off[b] = 0;

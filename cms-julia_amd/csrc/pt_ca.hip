@@ -1924,7 +1924,13 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
             uint32_t o = 0;
             if (e.s.misc[kMiscNbrLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_NEIGHBOR_LISTS;
             if (e.s.misc[kMiscTrkLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_TRACK_LISTS;
-            if (o) atomicOr(&e.cnt->overflow, o);
+            if (o) {
+                atomicOr(&e.cnt->overflow, o);
+                // the reference's later cells then share its last list column
+                // (extend!, simple_matrix.jl:13-19); these lists are kept per
+                // cell here, so the event's results are not the reference's
+                atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
+            }
         }
         __syncthreads();
     }

@@ -242,6 +242,15 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         pt_params_default(&prm);
     if (prm.use_riemann_fit) return fail(PT_ERR_UNSUPPORTED, "the Riemann fit is not wired in the reference");
     if (!prm.fit_5_as_4) return fail(PT_ERR_UNSUPPORTED, "fit_5_as_4 = false throws in the reference");
+    // late fishbone: fish_bone(..., check_track = true) calls empty() on the
+    // UInt32 column id of a cell (gpu_fishbone.jl:29), a MethodError
+    if (prm.late_fish_bone)
+        return fail(PT_ERR_UNSUPPORTED, "late_fish_bone = true throws in the reference (gpu_fishbone.jl:29)");
+    // do_stats: kernel_mark_used((...)) names no defined function and passes a
+    // single tuple (ca_hit_ntuplet_generator_kernels.jl:182-184)
+    if (prm.do_stats)
+        return fail(PT_ERR_UNSUPPORTED,
+                    "do_stats = true throws in the reference (ca_hit_ntuplet_generator_kernels.jl:182-184)");
     if (prm.min_hits_per_ntuplet < 3 || prm.min_hits_per_ntuplet > 5)
         return fail(PT_ERR_ARG, "min_hits_per_ntuplet must be 3..5");
     if (prm.max_num_of_doublets == 0 || prm.max_num_of_doublets > PT_MAX_DOUBLETS)

@@ -44,6 +44,7 @@ REF = {
     "TOO_MANY_HITS": 0x001, "HIST_LAYER": 0x002, "SIZE_CUT_INF": 0x004, "DEPTH": 0x008, "NO_CELLS": 0x010,
     "FIT_NAN": 0x020, "FIT_CHARGE": 0x040, "SQRT_DOMAIN": 0x080, "BAD_MODULE": 0x100, "MULT": 0x200,
 }
+INT = {"LOOKBACK": 0x001, "NBR_POOL": 0x002, "OVF_LIST": 0x004, "ROUNDS": 0x008, "LIST_SHARE": 0x010}
 QUALITY_BAD, QUALITY_DUP, QUALITY_LOOSE = 0, 1, 2
 FIT_FP64, FIT_MIXED, FIT_ADJOINT = 0, 1, 2
 SCHED_PLAIN, SCHED_PER_LENGTH, SCHED_PERSISTENT = 0, 1, 2
@@ -195,7 +196,7 @@ HIP_SYMBOLS = [
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
-    "ptg_generate_fit_tracks",
+    "ptg_generate_fit_tracks", "ptg_iphi",
 ]
 
 _hip = None
@@ -282,6 +283,7 @@ def gen_lib():
         lib.ptg_build_phi_hist.argtypes = [c_i16p, c_u32p, c_u32p, c_u16p]
         lib.ptg_generate_fit_tracks.argtypes = [C.POINTER(PtgFitConfig), c_f32p, c_f32p, c_u8p, c_f32p, c_i16p,
                                                 c_f32p]
+        lib.ptg_iphi.argtypes = [C.c_uint32, c_f32p, c_f32p, c_i16p]
         _gen = lib
     return _gen
 

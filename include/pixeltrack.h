@@ -13,7 +13,8 @@
  *
  * Status: every function returns PT_OK (0) or a negative PT_ERR_* code;
  * pt_last_error() gives a message.  Capacity overflows are NOT errors: they
- * truncate exactly like the reference and set bits in pt_counters.overflow.
+ * truncate exactly like the reference and set bits in pt_counters.overflow
+ * (one exception: list-column exhaustion, PT_INT_LIST_SHARE below).
  * Inputs on which the reference would stop with an AssertionError /
  * InexactError / DomainError are reported in pt_counters.ref_assert.
  */
@@ -80,6 +81,9 @@ extern "C" {
 #define PT_INT_NBR_POOL        0x002u /* neighbour pool (lists longer than 4) full   */
 #define PT_INT_OVF_LIST        0x004u /* internal overflow list full                 */
 #define PT_INT_ROUNDS          0x008u /* duplicate-remover rounds exceeded the shared cells */
+#define PT_INT_LIST_SHARE      0x010u /* > 65535 cells need neighbour or tuple lists: the reference's
+                                         extend! then hands the later cells its last column, shared
+                                         (simple_matrix.jl:13-19); lists stay per cell here */
 
 /* Quality enum (track_quality.jl:129-136) */
 #define PT_QUALITY_BAD         0
@@ -96,9 +100,9 @@ typedef struct pt_params {
     uint8_t  fit_5_as_4;                       /* 1 (0 is unsupported: the reference throws)       */
     uint8_t  include_jumping_forward_doublets; /* 1      */
     uint8_t  early_fish_bone;                  /* 1      */
-    uint8_t  late_fish_bone;                   /* 0      */
+    uint8_t  late_fish_bone;                   /* 0 (1 is unsupported: the reference throws)       */
     uint8_t  ideal_conditions;                 /* 1      */
-    uint8_t  do_stats;                         /* 0      */
+    uint8_t  do_stats;                         /* 0 (1 is unsupported: the reference throws)       */
     uint8_t  do_cluster_cut;                   /* 1      */
     uint8_t  do_z0_cut;                        /* 1      */
     uint8_t  do_pt_cut;                        /* 1      */

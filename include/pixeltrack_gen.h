@@ -58,6 +58,10 @@ void ptg_free_event(ptg_event *e);
  * local y, normal in global coordinates), as DetParams.frame.rot. */
 int  ptg_frames(float *rot, uint32_t n_modules);
 
+/* iphi[i] = unsafe_atan2s(y[i], x[i], 7) (approx_atan2.jl:164-184), for hits
+ * that callers place themselves. */
+int  ptg_iphi(uint32_t n, const float *x, const float *y, int16_t *iphi);
+
 /* Phi histogram of a hit list (fill_many_from_vector semantics). */
 int  ptg_build_phi_hist(const int16_t *iphi, const uint32_t *layer_start, uint32_t *off, uint16_t *bins);
 

@@ -698,6 +698,12 @@ void oracle_params_default(pt_params *p) {
 int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *frames, double b_field,
                        pt_track_soa *out, pt_counters *cnt, pt_ca_debug *dbg, int stop_after) {
     if (!P || !hv || !out) return PT_ERR_ARG;
+    // switches on which the reference throws before producing tracks:
+    // late_fish_bone (empty() on a UInt32, gpu_fishbone.jl:29), do_stats
+    // (kernel_mark_used is not defined, ca_hit_ntuplet_generator_kernels.jl:182-184),
+    // fit_5_as_4 = false (broken_line_fit_on_gpu.jl:190-199), use_riemann_fit
+    // (launch_riemann_kernels! only prints, helix_fit_on_gpu.jl:94-96)
+    if (P->late_fish_bone || P->do_stats || !P->fit_5_as_4 || P->use_riemann_fit) return PT_ERR_UNSUPPORTED;
     // per-thread scratch reused across events, like CACHED_KERNELS (a TaskLocalValue,
     // ca_hit_ntuplet_generator.jl:9) reset by resetCAHitNTupletGeneratorKernels
     thread_local State st_tls;

@@ -94,3 +94,46 @@ def test_stage_with_mixed_fit(frames):
             d = np.minimum(d, 2 * np.pi - d)
         err = np.max(d / np.maximum(np.abs(ref.state[k, fitted]), sig[k]))
         assert err <= 1e-4, (k, err)
+
+
+def test_pipeline_replays_of_pu200_events(frames):
+    """Regression test of the duplicate remover's round loop: eight PU200
+    events (every one passes the doublet cap) in two groups of four replayed
+    three times with eager batches in between; every replay equals the eager
+    batch and the oracle (CA, quality and counters bit-exact)."""
+    events = [G.generate_event("pu200", seed=s) for s in range(1001, 1009)]
+    ctx = ca.Context(max_events=8, frames=frames)
+    eager = ca.DeviceBatch(ctx, events)
+    piped = ca.DeviceBatch(ctx, events)
+    try:
+        print("[pipeline] eager batch of 8 pu200 events", flush=True)
+        eager.run_async(out_zeroed=True)
+        ctx.sync()
+        ref = [eager.download(i) for i in range(8)]
+        ref_cnt = [ctx.counters(i) for i in range(8)]
+        for i, h in enumerate(events):
+            o, ocnt, _ = O.make_tuples(h, frames, stop_after=0, debug=False)
+            for k in COUNTER_KEYS + ("n_loose",):
+                assert ref_cnt[i][k] == ocnt[k], (i, k, ref_cnt[i][k], ocnt[k])
+            m = min(ocnt["n_tuples"], A.MAX_TUPLES)
+            np.testing.assert_array_equal(ref[i].quality[:m], o.quality[:m])
+            np.testing.assert_array_equal(ref[i].hit_off, o.hit_off)
+        pl = ca.Pipeline(ctx, group_size=4, n_groups=2)
+        for rep in range(3):
+            print(f"[pipeline] pu200 replay {rep}", flush=True)
+            pl.submit(0, piped, 0)
+            pl.submit(1, piped, 4)
+            pl.wait(0)
+            pl.wait(1)
+            for i in range(8):
+                bad = same_tracks(piped.download(i), ref[i], ref_cnt[i])
+                assert bad is None, (rep, i, bad)
+                cnt = pl.counters(i // 4, i % 4)
+                for k in COUNTER_KEYS:
+                    assert cnt[k] == ref_cnt[i][k], (rep, i, k)
+            eager.run_async(out_zeroed=True)
+            ctx.sync()
+    finally:
+        eager.free()
+        piped.free()
+        ctx.close()

@@ -1,0 +1,120 @@
+"""GPU parity under the reference's algorithm switches, and at the list-column
+limit of the CA.
+
+Switches (Params, ca_hit_ntuplet_generator_kernels.jl:31-87), each against the
+oracle run with the same switch, bit-exact for the CA and the quality, FP64
+fit at the stage tolerance:
+  early_fish_bone = 0              no fishbone before the ntuplets (:178)
+  min_hits_per_ntuplet = 4 / 5     13 layer pairs and the p < 3 start filter
+                                   (:152-160, ..._impl.jl:123-125)
+  include_jumping_forward_doublets = 0   15 layer pairs (:152-156)
+  do_cluster_cut / do_z0_cut / do_pt_cut = 0   (gpu_pixel_doublets.jl:229-245,326,338)
+  ideal_conditions = 0             the outer-ladder test of the cluster cut (:233)
+
+List columns: the reference keeps at most 65,535 neighbour lists
+(CellNeighborsVector(64, 65536) with column 1 the empty sentinel); extend!
+hands every later cell the last column (simple_matrix.jl:13-19), so those
+cells share one list.  This library keeps the lists per cell and reports the
+case as an internal error; below the limit everything is bit-exact.
+"""
+import numpy as np
+import pytest
+
+import crafted as K
+import oracle_lib as O
+from pixeltrack import _abi as A
+from pixeltrack import ca, generator as G
+from test_gpu_parity import FIT_ATOL, FIT_RTOL, stage_errors
+
+pytestmark = pytest.mark.gpu
+
+SWITCHES = [dict(early_fish_bone=0), dict(min_hits_per_ntuplet=4), dict(min_hits_per_ntuplet=5),
+            dict(include_jumping_forward_doublets=0), dict(do_cluster_cut=0), dict(do_z0_cut=0), dict(do_pt_cut=0),
+            dict(ideal_conditions=0)]
+EVENTS = [("single_event", 1), ("pu50", 1)]
+
+
+def sw_id(kw):
+    return ",".join(f"{k}={v}" for k, v in kw.items())
+
+
+@pytest.mark.parametrize("kw", SWITCHES, ids=sw_id)
+@pytest.mark.parametrize("preset,seed", EVENTS)
+def test_stage_parity_with_switch(frames, preset, seed, kw):
+    hits = G.generate_event(preset, seed=seed)
+    print(f"[switch] {sw_id(kw)} {preset} seed={seed} hits={hits.n_hits}", flush=True)
+    ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=0, p=O.params(**kw))
+    gen = ca.CAHitNtupletGeneratorOnGPU(frames, params=ca.default_params(**kw))
+    try:
+        out = gen.make_tuples(hits)
+        gcnt = gen.last_counters
+        gdbg = gen.ctx.ca_debug(0, hits.n_hits)
+    finally:
+        gen.ctx.close()
+    errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
+    assert not errors, "\n".join(errors)
+    fitted = np.nonzero(ref.state[2] != 0)[0]
+    assert fitted.size == rcnt["n_fitted"] > 0
+    np.testing.assert_allclose(out.state[:, fitted], ref.state[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+    np.testing.assert_allclose(out.covariance[:, fitted], ref.covariance[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
+
+
+def list_owner_keys(nbr_lists):
+    """Cells with outer neighbours in the order the reference's connect loop
+    first extends a list for them: by the first (smallest) cell that connects
+    to them, then by their own id (connect walks `this` cells in id order and
+    each outer-hit list in ascending cell order)."""
+    owners = [i for i, l in enumerate(nbr_lists) if len(l)]
+    return sorted(owners, key=lambda o: (int(nbr_lists[o][0]), o))
+
+
+# collimated bundles: 60 azimuths x 3 pseudorapidities x 10 copies give 54,000
+# cells with outer neighbours, 80 azimuths give 72,000
+BUNDLES = {"below_list_limit": 60, "above_list_limit": 80}
+
+
+@pytest.mark.parametrize("case", sorted(BUNDLES))
+def test_neighbour_list_columns_at_the_limit(frames, case):
+    hits = K.make_event(K.bundles(BUNDLES[case], (0.0, 0.3, -0.3), 10))
+    print(f"[switch] bundles {case}: {hits.n_hits} hits", flush=True)
+    ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=0)
+    rl = rdbg.lists("nbr")
+    n_owned = sum(1 for l in rl if len(l))
+    gen = ca.CAHitNtupletGeneratorOnGPU(frames)
+    try:
+        if case == "below_list_limit":
+            assert n_owned < A.MAX_ACTIVE_LISTS - 1 and not rcnt["overflow"] & A.OVF["NEIGHBOR_LISTS"]
+            out = gen.make_tuples(hits)
+            gcnt = gen.last_counters
+            gdbg = gen.ctx.ca_debug(0, hits.n_hits)
+            errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
+            assert not errors, "\n".join(errors)
+            return
+        assert rcnt["overflow"] & A.OVF["NEIGHBOR_LISTS"]
+        with pytest.raises(RuntimeError, match="internal limit"):
+            gen.make_tuples(hits)
+        gcnt = gen.ctx.counters(0)
+        gdbg = gen.ctx.ca_debug(0, hits.n_hits)
+    finally:
+        gen.ctx.close()
+    assert gcnt["internal_error"] & A.INT["LIST_SHARE"]
+    assert gcnt["overflow"] & A.OVF["NEIGHBOR_LISTS"]
+    # doublets, outer-hit lists and used flags do not depend on the columns
+    rc, gc = rdbg.cells(), gdbg.cells()
+    for k in ("inner", "outer", "pair", "used"):
+        np.testing.assert_array_equal(gc[k], rc[k], err_msg=k)
+    rh, gh = rdbg.lists("ohc"), gdbg.lists("ohc")
+    assert all(np.array_equal(a, b) for a, b in zip(rh, gh))
+    # the per-cell lists of this library give the reference's columns: the
+    # first 65,534 list owners keep their own, the rest share the last one,
+    # which holds the first 64 of their connections in connect's order
+    gl = gdbg.lists("nbr")
+    order = list_owner_keys(gl)
+    assert len(order) > A.MAX_ACTIVE_LISTS - 1
+    own, shared = order[:A.MAX_ACTIVE_LISTS - 2], order[A.MAX_ACTIVE_LISTS - 2:]
+    for o in own:
+        np.testing.assert_array_equal(rl[o], gl[o])
+    pushes = sorted((int(t), o) for o in shared for t in gl[o])
+    column = np.array([t for t, _ in pushes[:A.MAX_NEIGHBORS]], np.uint32)
+    for o in shared:
+        np.testing.assert_array_equal(rl[o], column)
