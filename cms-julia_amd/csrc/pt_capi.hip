@@ -28,6 +28,7 @@ void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, co
 void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
 void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, hipStream_t st);
 void launch_load_tracks(const EvDesc *d_evs, uint32_t n_events, float pt_min, hipStream_t st);
+int noalloc_scratch(int device, uint32_t **counts, uint32_t **lists);
 }  // namespace pt
 
 namespace {
@@ -256,6 +257,9 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
     if (prm.max_num_of_doublets == 0 || prm.max_num_of_doublets > PT_MAX_DOUBLETS)
         return fail(PT_ERR_ARG, "max_num_of_doublets must be 1..%u", PT_MAX_DOUBLETS);
     HIP_OK(hipSetDevice(device));
+    // address of this device's fixed scratch (pt_fit_bl_sched_noalloc_async)
+    if (pt::noalloc_scratch(device, nullptr, nullptr) != 0)
+        return fail(PT_ERR_HIP, "hipGetSymbolAddress of the fixed fit scratch failed on device %d", device);
     pt_ctx *c = new pt_ctx();
     c->device = device;
     c->p = prm;
@@ -986,6 +990,36 @@ int pt_fit_bl_sched_async(pt_ctx *c, uint32_t n, const float *hits, const float 
     if (!c) return fail(PT_ERR_ARG, "null context");
     const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
     return fit_flat(c, n, hits, ge, nhit, b_field, precision, schedule, c->tuning[pi], state, cov, pt, eta, chi2);
+}
+
+int pt_fit_bl_sched_noalloc_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                                  double b_field, int precision, int schedule, float *state, float *cov, float *pt,
+                                  float *eta, float *chi2) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (n && (!hits || !ge || !nhit || !state || !cov || !pt || !eta || !chi2)) return fail(PT_ERR_ARG, "null pointer");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (schedule != PT_SCHED_PLAIN && schedule != PT_SCHED_PER_LENGTH && schedule != PT_SCHED_PERSISTENT)
+        return fail(PT_ERR_ARG, "unknown schedule");
+    if (n > PT_NOALLOC_MAX_TRACKS)
+        return fail(PT_ERR_CAPACITY, "%u tracks exceed the fixed scratch of the allocation-free fit (%u tracks)", n,
+                    (unsigned)PT_NOALLOC_MAX_TRACKS);
+    HIP_OK(hipSetDevice(c->device));
+    uint32_t *counts = nullptr, *lists = nullptr;
+    if (pt::noalloc_scratch(c->device, &counts, &lists) != 0)
+        return fail(PT_ERR_HIP, "hipGetSymbolAddress of the fixed fit scratch failed on device %d", c->device);
+    const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
+    const pt_fit_tuning &t = c->tuning[pi];
+    pt::FlatArgs a;
+    a.hits = hits, a.ge = ge, a.nhit = nhit, a.n = n, a.B = (double)(float)b_field;
+    a.state = state, a.cov = cov, a.pt = pt, a.eta = eta, a.chi2 = chi2;
+    pt::FitLaunch L;
+    L.schedule = schedule;
+    L.block = t.block, L.tile = t.tile, L.blocks = t.blocks;
+    L.lists = lists;
+    L.counts = counts;
+    pt::launch_fit_flat(a, precision, L, c->stream);
+    HIP_OK(hipGetLastError());
+    return PT_OK;
 }
 
 int pt_fit_set_tuning(pt_ctx *c, int precision, const pt_fit_tuning *t) {

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "pt_bl_adjoint.hpp"
 #include "pt_bl_fit.hpp"
@@ -420,6 +421,36 @@ void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hip
         launch_job(AdjJob<true>{a}, a.n, L, st);
     else
         launch_job(AdjJob<false>{a}, a.n, L, st);
+}
+
+// ------------------------------------------------------------------ fixed scratch
+// Scratch of the fit-only entry point that allocates nothing
+// (pt_fit_bl_sched_noalloc_async): kNoAllocCounters words of counters /
+// tickets, then the per-length id lists (2 per track).  One copy per device,
+// held until the process ends; every call on a device shares it.
+constexpr uint32_t kNoAllocCounters = 64u;  // 256 bytes, keeps the lists 256-byte aligned
+static __device__ __align__(256) uint32_t scratch[kNoAllocCounters + 2u * PT_NOALLOC_MAX_TRACKS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device` (the current device): looked up
+// once per device with hipGetSymbolAddress and kept.  The array's name is a
+// host-side stand-in and is never passed to a kernel.
+int noalloc_scratch(int device, uint32_t **counts, uint32_t **lists) {
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+        g_scratch_ptr[device] = (uint32_t *)p;
+    }
+    if (counts) *counts = g_scratch_ptr[device];
+    if (lists) *lists = g_scratch_ptr[device] + kNoAllocCounters;
+    return 0;
 }
 
 // Global covariances of pre-gathered hits from local sigmas and module ids

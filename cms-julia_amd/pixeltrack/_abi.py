@@ -29,6 +29,7 @@ N_LAYERS = 10
 HIST_NOFF = 1281
 N_MODULES = 1856
 N_STAGES = 48
+NOALLOC_MAX_TRACKS = 262144
 
 PT_OK = 0
 PT_ERR_ARG = -1
@@ -192,7 +193,7 @@ HIP_SYMBOLS = [
     "pt_fit_set_tuning", "pt_fit_get_tuning", "pt_fit_autotune", "pt_hit_errors_to_global_async",
     "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async", "pt_ctx_set_fit_precision", "pt_pipeline_init",
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
-    "pt_set_hits_layer_start", "pt_load_tracks_async",
+    "pt_set_hits_layer_start", "pt_load_tracks_async", "pt_fit_bl_sched_noalloc_async",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -243,6 +244,8 @@ def hip_lib():
         lib.pt_fit_bl_device_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
         lib.pt_fit_bl_sched_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
                                               vp, vp]
+        lib.pt_fit_bl_sched_noalloc_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp,
+                                                      vp, vp, vp, vp]
         lib.pt_fit_set_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
         lib.pt_fit_get_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
         lib.pt_fit_autotune.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp,

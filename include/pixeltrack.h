@@ -337,6 +337,19 @@ typedef struct pt_fit_tuning {
 int pt_fit_bl_sched_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
                           const uint8_t *nhit, double b_field, int precision, int schedule,
                           float *state, float *cov, float *pt, float *eta, float *chi2);
+/* Same as pt_fit_bl_sched_async, allocating nothing: no reference counterpart;
+ * for callers that cannot allocate device memory during the call (code under
+ * hipGraph capture, tight serving loops).  pt_fit_bl_sched_async grows its
+ * per-length lists with hipMalloc on demand; this variant takes its counters
+ * and lists from a fixed device array of the library (one per GPU, about
+ * 2 MiB, kept until the process ends), so n_tracks is at most
+ * PT_NOALLOC_MAX_TRACKS (PT_ERR_CAPACITY above).  All calls on one GPU share
+ * that array: calls that overlap on one GPU (other streams, other contexts,
+ * other threads) race and must be serialised by the caller. */
+#define PT_NOALLOC_MAX_TRACKS 262144u
+int pt_fit_bl_sched_noalloc_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
+                                  const uint8_t *nhit, double b_field, int precision, int schedule,
+                                  float *state, float *cov, float *pt, float *eta, float *chi2);
 /* Tuning of the persistent schedule per precision (PT_FIT_FP64, PT_FIT_MIXED
  * or PT_FIT_ADJOINT; default: 256 threads, 512-track tiles, 4 blocks per CU). */
 int pt_fit_set_tuning(pt_ctx *ctx, int precision, const pt_fit_tuning *t);
