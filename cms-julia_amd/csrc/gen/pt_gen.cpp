@@ -207,6 +207,114 @@ double sample_pt(const ptg_config &c, Rng &rng) {
 
 inline int hist_bin(int16_t t) { return (int)(((uint16_t)t) >> 9) + 1; }
 
+// ---------------------------------------------------------------- multiple scattering
+// A track between two deflections: position, direction (azimuth, cot(theta))
+// and transverse radius of curvature; q = +1 turns clockwise seen from +z.
+struct Flight {
+    double x, y, z, phi, cot, R;
+    int q;
+    void centre(double &cx, double &cy) const {
+        cx = x + q * R * std::sin(phi);
+        cy = y - q * R * std::cos(phi);
+    }
+};
+
+// Smallest positive transverse arc length from the current point to layer L
+// (barrel cylinder or disk plane) along the current helix; -1 if not reached.
+double arc_to_layer(const Flight &f, int L) {
+    if (L >= 4) {
+        const double zd = (L < 7) ? kDiskZ[L - 4] : -kDiskZ[L - 7];
+        if (f.cot == 0.0) return -1.0;
+        const double s = (zd - f.z) / f.cot;
+        return s > 1e-9 ? s : -1.0;
+    }
+    // |c + Rot(a)(p - c)|^2 = r^2 with a = -q s / R: cos(a + delta) = K
+    const double r = kBarrelR[L];
+    double cx, cy;
+    f.centre(cx, cy);
+    const double c2 = cx * cx + cy * cy, cn = std::sqrt(c2);
+    if (cn == 0.0) return -1.0;
+    const double K = (r * r - c2 - f.R * f.R) / (2.0 * cn * f.R);
+    if (K < -1.0 || K > 1.0) return -1.0;
+    const double delta = std::atan2(f.y - cy, f.x - cx) - std::atan2(cy, cx);
+    const double ac = std::acos(K);
+    double best = -1.0;
+    for (int sg = -1; sg <= 1; sg += 2)
+        for (int m = -3; m <= 3; ++m) {
+            const double s = -f.q * f.R * (sg * ac - delta + 2.0 * kPi * m);
+            if (s > 1e-9 && (best < 0.0 || s < best)) best = s;
+        }
+    return best;
+}
+
+void advance(Flight &f, double s) {
+    double cx, cy;
+    f.centre(cx, cy);
+    const double a = -f.q * s / f.R;
+    const double ux = f.x - cx, uy = f.y - cy, ca = std::cos(a), sa = std::sin(a);
+    f.x = cx + ca * ux - sa * uy;
+    f.y = cy + sa * ux + ca * uy;
+    f.z += s * f.cot;
+    f.phi += a;
+}
+
+// Highland's width of the projected scattering angle (PDG 2022, eq. 34.15;
+// beta = 1, unit charge) squared, for a path of `len` cm through the medium
+// the broken-line fit assumes: 1/X0 = 0.06/16 per cm and a geometry factor
+// 0.7 on the variance (broken_line.jl:46-79).
+double highland_var(double len, double p) {
+    const double x = len * (0.06 / 16.0);
+    const double f = 1.0 + 0.038 * std::log(x);
+    return 0.7 * (0.0136 / p) * (0.0136 / p) * x * f * f;
+}
+
+// Deflects the direction by Gaussian angles of variance v_bend (towards the
+// transverse normal of the direction) and v_polar (in the plane of the
+// direction and the z axis); |p| is kept.
+void kink(Flight &f, double p, double kappa, double v_bend, double v_polar, Rng &rng) {
+    const double st = 1.0 / std::sqrt(1.0 + f.cot * f.cot), ct = f.cot * st;
+    const double cp = std::cos(f.phi), sp = std::sin(f.phi);
+    const double d1 = v_bend > 0.0 ? std::sqrt(v_bend) * rng.normal() : 0.0;
+    const double d2 = v_polar > 0.0 ? std::sqrt(v_polar) * rng.normal() : 0.0;
+    const double tx = cp * st - d1 * sp + d2 * ct * cp;
+    const double ty = sp * st + d1 * cp + d2 * ct * sp;
+    const double tz = ct - d2 * st;
+    const double tn = std::sqrt(tx * tx + ty * ty + tz * tz), tr = std::sqrt(tx * tx + ty * ty);
+    f.phi = std::atan2(ty, tx);
+    f.cot = tz / tr;
+    f.R = p * (tr / tn) / kappa;
+}
+
+// Positions of the first n crossings of `layers` for a helix deflected as
+// described at ptg_fit_config.ms_scale; false if a layer is missed.
+bool scattered_hits(const Helix &h, double kappa, double ms_scale, const std::vector<RawHit> &layers, int n,
+                    Rng &rng, std::vector<RawHit> &out) {
+    const double p = h.pt * std::sqrt(1.0 + h.cot * h.cot);
+    Flight f{0.0, 0.0, h.z0, h.phi0, h.cot, h.R, h.q};
+    auto path = [](double s, const Flight &g) { return s * std::sqrt(1.0 + g.cot * g.cot); };
+    out.assign(layers.begin(), layers.begin() + n);
+    for (int i = 0; i < n; ++i) {
+        const double s = arc_to_layer(f, layers[i].layer);
+        if (s < 0.0) return false;
+        const double prev_len = path(s, f);
+        advance(f, s);
+        out[i].x = f.x, out[i].y = f.y, out[i].z = f.z;
+        if (i + 1 < n) {
+            // the material of each segment between hits acts at both of its
+            // ends: the first hit takes the next segment's share, an interior
+            // hit both (the next segment along the current direction)
+            Flight g = f;
+            const double sn = arc_to_layer(g, layers[i + 1].layer);
+            if (sn < 0.0) return false;
+            double v = highland_var(path(sn, g), p);
+            if (i > 0) v += highland_var(prev_len, p);
+            v *= ms_scale;
+            kink(f, p, kappa, v, v, rng);
+        }
+    }
+    return true;
+}
+
 template <class T> T *dup_array(const std::vector<T> &v) {
     T *p = (T *)std::malloc(std::max<size_t>(1, v.size()) * sizeof(T));
     if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
@@ -431,7 +539,13 @@ int ptg_generate_fit_tracks(const ptg_fit_config *c, float *hits, float *ge, uin
             h.R = h.pt / kappa;
             h.cot = std::sinh(h.eta);
             crossings(h, cross, (int)t);
-            if ((int)cross.size() >= want) break;
+            if ((int)cross.size() < want) continue;
+            if (c->ms_scale > 0.0) {
+                std::vector<RawHit> sc;
+                if (!scattered_hits(h, kappa, c->ms_scale, cross, want, rng, sc)) continue;
+                std::copy(sc.begin(), sc.end(), cross.begin());
+            }
+            break;
         }
         nhit[t] = (uint8_t)want;
         for (int k = 0; k < 5; ++k) {

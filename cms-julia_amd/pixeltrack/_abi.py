@@ -167,7 +167,7 @@ class PtgFitConfig(C.Structure):
         ("n_tracks", C.c_uint32), ("len_mix", C.c_double * 3), ("pt_min", C.c_double), ("pt_max", C.c_double),
         ("eta_max", C.c_double), ("z_sigma", C.c_double), ("b_tesla", C.c_double),
         ("srphi0", C.c_double), ("srphi1", C.c_double), ("sz0", C.c_double), ("sz1", C.c_double),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("ms_scale", C.c_double),
     ]
 
 

@@ -95,8 +95,11 @@ def build_phi_hist(iphi, layer_start):
 
 
 def fit_tracks(n_tracks, seed=7, len_mix=(0.30, 0.55, 0.15), pt_min=0.5, pt_max=10.0, eta_max=2.5, z_sigma=5.0,
-               b_tesla=3.8, srphi=(8e-4, 15e-4), sz=(15e-4, 40e-4)):
+               b_tesla=3.8, srphi=(8e-4, 15e-4), sz=(15e-4, 40e-4), ms_scale=0.0):
     """Pre-gathered fit-only tracks in the pt_fit_bl layout.
+
+    ms_scale > 0 deflects the tracks by multiple scattering in the material
+    the broken-line fit assumes (ms_scale times its Highland variance).
 
     Returns dict with hits [3,5,n], ge [6,5,n], nhit [n], sig [2,5,n], det [5,n], truth [5,n]."""
     lib = A.gen_lib()
@@ -108,6 +111,7 @@ def fit_tracks(n_tracks, seed=7, len_mix=(0.30, 0.55, 0.15), pt_min=0.5, pt_max=
     c.srphi0, c.srphi1 = srphi
     c.sz0, c.sz1 = sz
     c.seed = seed
+    c.ms_scale = ms_scale
     hits = np.zeros((3, 5, n_tracks), np.float32)
     ge = np.zeros((6, 5, n_tracks), np.float32)
     nhit = np.zeros(n_tracks, np.uint8)

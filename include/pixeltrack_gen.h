@@ -77,6 +77,18 @@ typedef struct ptg_fit_config {
     double   eta_max, z_sigma, b_tesla;
     double   srphi0, srphi1, sz0, sz1;
     uint64_t seed;
+    /* Multiple scattering, 0 = none.  Otherwise the track is propagated hit
+     * to hit and deflected by Gaussian kinks whose variance is ms_scale times
+     * the Highland angle (PDG, beta = 1, unit charge) of the material the
+     * broken-line fit assumes: a uniform medium with 1/X0 = 0.06/16 per cm of
+     * path and a geometry factor 0.7 (broken_line.jl:46-79).  The material of
+     * each path segment between two hits acts at both ends of the segment, in
+     * both planes that contain the direction: an interior hit takes both
+     * adjacent segments (the fit's VarBeta, broken_line.jl:200-206), the first
+     * hit the segment to the second hit (the term the fit adds to the slope
+     * and phi variances, broken_line.jl:477-479,619-622).  Truth stays the
+     * vertex helix. */
+    double   ms_scale;
 } ptg_fit_config;
 int ptg_generate_fit_tracks(const ptg_fit_config *c, float *hits, float *ge, uint8_t *nhit, float *sig,
                             int16_t *det, float *truth5);

@@ -19,6 +19,11 @@ void oracle_params_default(pt_params *p);
 int oracle_make_tuples(const pt_params *p, const pt_hits_view *hits, const float *frames, double b_field,
                        pt_track_soa *out, pt_counters *cnt, pt_ca_debug *dbg, int stop_after);
 
+/* Plants a mistake in fish_bone (gpu_fishbone.jl:43-56) for the tests that
+ * show the fishbone pins catch it: no `break` after killing the i-th cell, no
+ * inner-module comparison.  (0, 0) restores the restatement.  Tests only. */
+void oracle_plant_fishbone(int no_break, int no_module_check);
+
 /* fill_many_from_vector (hist_to_container.jl:265-270) for the 10-layer phi histogram */
 int oracle_build_phi_hist(const int16_t *iphi, uint32_t n_hits, const uint32_t *layer_start, uint32_t *off,
                           uint16_t *bins);
@@ -33,6 +38,12 @@ int oracle_load_tracks(const pt_track_soa *t, float pt_min, uint32_t *n, uint16_
  * cov15[5..19], pt, eta, chi2.  flags[t] = oracle fit flags. */
 int oracle_fit_bl(uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
                   double *out64, uint32_t *flags, int n_threads);
+
+/* Plants a mistake in mult_scatt (broken_line.jl:46-79) for the tests that
+ * show the scattering pins catch it: the result times `scale`, log10 in place
+ * of the natural log, no 20 GeV cap.  (1, 0, 0) restores the restatement.
+ * Process-wide; tests only. */
+void oracle_plant_ms(double scale, int log10_for_ln, int drop_cap);
 
 /* One track with every intermediate (for the known-answer checks).
  * hits: 3xN column-major doubles (x,y,z per hit), ge: 6xN column-major floats.

@@ -337,6 +337,13 @@ void kernel_connect(const pt_params &P, const Hits &hh, State &st) {
 }
 
 // gpu_fishbone.jl:4-62
+// Planted mistakes in fish_bone, set only by the tests that show the
+// fishbone pins catch them (default: none).
+struct FishPlant {
+    bool no_break = false, no_module_check = false;
+};
+FishPlant g_fish_plant;
+
 void fish_bone(const Hits &hh, State &st, uint32_t n_hits, bool check_track) {
     float dx[128], dy[128], dz[128], norms[128];
     uint16_t inner_det[128];
@@ -364,9 +371,11 @@ void fish_bone(const Hits &hh, State &st, uint32_t n_hits, bool check_track) {
         for (int ic = 0; ic < curr - 1; ++ic) {
             for (int jc = ic + 1; jc < curr; ++jc) {
                 float cos12 = dx[ic] * dx[jc] + dy[ic] * dy[jc] + dz[ic] * dz[jc];
-                if (inner_det[ic] != inner_det[jc] && cos12 * cos12 >= 0.99999f * norms[ic] * norms[jc]) {
+                const bool other_module = g_fish_plant.no_module_check || inner_det[ic] != inner_det[jc];
+                if (other_module && cos12 * cos12 >= 0.99999f * norms[ic] * norms[jc]) {
                     if (norms[ic] > norms[jc]) {
                         st.cells[cidx[ic] - 1].doublet_id = -1;
+                        if (g_fish_plant.no_break) continue;
                         break;
                     } else {
                         st.cells[cidx[jc] - 1].doublet_id = -1;
@@ -693,6 +702,11 @@ void oracle_params_default(pt_params *p) {
     p->quadruplet_max_tip = 0.5f;
     p->quadruplet_min_pt = 0.3f;
     p->quadruplet_max_zip = 12.0f;
+}
+
+void oracle_plant_fishbone(int no_break, int no_module_check) {
+    g_fish_plant.no_break = no_break != 0;
+    g_fish_plant.no_module_check = no_module_check != 0;
 }
 
 int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *frames, double b_field,

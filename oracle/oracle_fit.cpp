@@ -276,6 +276,12 @@ int oracle_fit_bl(uint32_t n, const float *hits, const float *ge, const uint8_t 
     return PT_OK;
 }
 
+void oracle_plant_ms(double scale, int log10_for_ln, int drop_cap) {
+    oracle::ms_plant().scale = scale;
+    oracle::ms_plant().log10 = log10_for_ln != 0;
+    oracle::ms_plant().no_cap = drop_cap != 0;
+}
+
 int oracle_bl_detail(int N, const double *hits, const float *ge, double B, double *out, int cap) {
     if (N == 3) return detail_impl<3>(hits, ge, B, out, cap);
     if (N == 4) return detail_impl<4>(hits, ge, B, out, cap);

@@ -65,11 +65,24 @@ template <class T> inline T atan2_ref(T y, T x) {
     return T(std::numeric_limits<double>::quiet_NaN());
 }
 
+// Planted mistakes in mult_scatt, set only by the tests that show the
+// scattering pins catch them (default: none).
+struct MsPlant {
+    double scale = 1.0;   // result multiplied by this
+    bool log10 = false;   // log10 in place of the natural log
+    bool no_cap = false;  // no 20 GeV cap on B*R
+};
+inline MsPlant &ms_plant() {
+    static MsPlant p;
+    return p;
+}
+
 // broken_line.jl:46-79
 template <class T> inline T mult_scatt(T length, double B, T R, int /*layer*/, T slope) {
     using std::abs;
     using std::log;
-    T pt2 = jl_min(T(20.0), B * R);
+    const MsPlant &plant = ms_plant();
+    T pt2 = plant.no_cap ? T(B * R) : jl_min(T(20.0), B * R);
     pt2 = pt2 * pt2;
     const double XXI_0 = 0.06 / 16.0;
     const double geometry_factor = 0.7;
@@ -78,9 +91,12 @@ template <class T> inline T mult_scatt(T length, double B, T R, int /*layer*/, T
     T slope_term = 1.0 + slope * slope;
     T abs_length_XXI_0 = abs(length) * XXI_0;
     T log_term = log(abs_length_XXI_0);
+    if (plant.log10) log_term = log_term / std::log(10.0);
     T sq = 1.0 + 0.038 * log_term;
     T sqrt_term = sq * sq;
-    return fact / (pt2 * slope_term) * abs_length_XXI_0 * sqrt_term;
+    T r = fact / (pt2 * slope_term) * abs_length_XXI_0 * sqrt_term;
+    if (plant.scale != 1.0) r = r * plant.scale;
+    return r;
 }
 
 // broken_line.jl:89-95 ; R = [a b; -b a]

@@ -37,6 +37,10 @@ def lib():
         L.oracle_bl_detail.argtypes = [C.c_int, vp, vp, C.c_double, vp, C.c_int]
         L.oracle_fit_opcount.argtypes = [C.c_int, vp, vp, C.c_double, vp]
         L.oracle_fit_grad.argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.oracle_plant_ms.argtypes = [C.c_double, C.c_int, C.c_int]
+        L.oracle_plant_ms.restype = None
+        L.oracle_plant_fishbone.argtypes = [C.c_int, C.c_int]
+        L.oracle_plant_fishbone.restype = None
         _lib = L
     return _lib
 

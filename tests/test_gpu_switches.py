@@ -118,3 +118,21 @@ def test_neighbour_list_columns_at_the_limit(frames, case):
     column = np.array([t for t, _ in pushes[:A.MAX_NEIGHBORS]], np.uint32)
     for o in shared:
         np.testing.assert_array_equal(rl[o], column)
+
+
+@pytest.mark.parametrize("same_module", [False, True], ids=["modules_differ", "same_module"])
+def test_stage_parity_on_crafted_fishbone_events(frames, same_module):
+    """The hand-checked fishbone cases of the oracle's tests (the `break` after
+    a kill, the inner-module test), through the whole GPU stage."""
+    from test_oracle_ca import fishbone_event
+    hits = fishbone_event(same_module)
+    print(f"[switch] crafted fishbone event same_module={same_module}", flush=True)
+    ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=0)
+    assert rcnt["n_killed"] >= 3
+    gen = ca.CAHitNtupletGeneratorOnGPU(frames)
+    try:
+        out = gen.make_tuples(hits)
+        errors = stage_errors(ref, rcnt, rdbg, out, gen.last_counters, gen.ctx.ca_debug(0, hits.n_hits))
+    finally:
+        gen.ctx.close()
+    assert not errors, "\n".join(errors)

@@ -324,6 +324,71 @@ def test_single_clean_helix_gives_one_quadruplet(frames):
     assert abs(pt / 2.0 - 1.0) < 0.01
 
 
+def fishbone_event(same_module):
+    """Three 100 GeV tracks from vertices along z that all cross B4 at one hit
+    h, with cot(theta) 0.100 (P2), 0.1018 (P1) and 0.1036 (P3): seen from h
+    their inner hits lie 1.8 mrad apart in polar angle (P1 between P2 and P3),
+    inside the fishbone's collinearity (cos^2 >= 0.99999, 3.16 mrad) for P1-P2
+    and P1-P3, outside it for P2-P3.  P1 and P2 have hits on B1-B4 (c1, c2 on
+    B3, c1 first in hit order), P3 on B1, B2 and B4.  The cells ending on h,
+    in list order: X1 = c1-h, X2 = c2-h (pair B3-B4), then Y1, Y2, Y3 = b1-h,
+    b2-h, b3-h (pair B2-B4); by length X2 < X1 < Y2 < Y1 < Y3.  Module ids
+    are all distinct, except c1 and c2 when same_module."""
+    import crafted as K
+    z_h, pt = 1.7, 100.0
+    R = pt / K.KAPPA_3P8T
+    s16 = 2 * R * np.arcsin(16.0 / (2 * R))
+    mods = {(0, 0): 10, (0, 1): 11, (0, 2): 12, (1, 0): 100, (1, 1): 101, (1, 2): 102, (2, 0): 330,
+            (2, 1): 330 if same_module else 331, (3, 0): 700}
+    hits = []
+    for k, cot in enumerate((0.1018, 0.100, 0.1036)):  # P1, P2, P3
+        z0 = z_h - s16 * cot
+        for L, r in enumerate(K.BARREL_R):
+            if (k == 2 and L == 2) or (k > 0 and L == 3):
+                continue  # P3 has no B3 hit; h is stored once
+            x, y, z = K.helix_point(r, pt, 0.5, cot, z0)
+            hits.append(dict(x=x, y=y, z=z, layer=L, track=k, det=mods[(L, k if L < 3 else 0)]))
+    return K.make_event(hits)
+
+
+def cells_on_h(h, dbg):
+    c = dbg.cells()
+    h_id = int(np.nonzero(h.det_ind == 700)[0][0]) + 1
+    named = {}
+    for name, layer, track in (("X1", 2, 0), ("X2", 2, 1), ("Y1", 1, 0), ("Y2", 1, 1), ("Y3", 1, 2)):
+        inner = int(np.nonzero((h.truth_track == track) & (np.arange(h.n_hits) >= h.layer_start[layer]) &
+                               (np.arange(h.n_hits) < h.layer_start[layer + 1]))[0][0]) + 1
+        idx = np.nonzero((c["inner"] == inner) & (c["outer"] == h_id))[0]
+        assert idx.size == 1, name
+        named[name] = int(idx[0])
+    return named, c
+
+
+@pytest.mark.parametrize("same_module,killed", [(False, {"X1", "Y1", "Y2"}), (True, {"Y1", "Y2", "Y3"})])
+def test_fishbone_order_break_and_module_check(frames, same_module, killed):
+    """Expected from gpu_fishbone.jl:43-56 by hand.  Different modules: X1 is
+    compared first, with X2, and killed (farther); the `break` ends X1's row,
+    so X1 never kills Y3 (without the break Y3 would die too); X2 kills Y1 and
+    Y2; Y1 then dies against Y2.  X1 and X2 on one module: that pair is
+    skipped, X1 survives and kills Y1, Y2 and Y3 (without the module check X1
+    would die and Y3 survive)."""
+    h = fishbone_event(same_module)
+    out, cnt, dbg = O.make_tuples(h, frames, stop_after=3)
+    named, c = cells_on_h(h, dbg)
+    assert all(c["used"][i] for i in named.values())
+    got = {n for n, i in named.items() if c["doublet_id"][i] < 0}
+    assert got == killed, got
+    # the same expectation fails for the plausible mistake this case exercises
+    O.lib().oracle_plant_fishbone(0 if same_module else 1, 1 if same_module else 0)
+    try:
+        _, _, dbg2 = O.make_tuples(h, frames, stop_after=3)
+    finally:
+        O.lib().oracle_plant_fishbone(0, 0)
+    c2 = dbg2.cells()
+    wrong = {n for n, i in named.items() if c2["doublet_id"][i] < 0}
+    assert wrong != killed, wrong
+
+
 def test_fishbone_kills_the_farther_collinear_doublet(frames):
     # two doublets ending on the same B3 hit, from B1 and B2 hits on one straight
     # line from the origin (different modules): the B1->B3 doublet is the farther

@@ -16,6 +16,12 @@ oracle produces for them is re-derived independently:
     those parameters (this pins the Karimaki translations, the signs and the
     output conventions);
   * smeared high-pT tracks have pulls of unit width and chi2/ndf near 1;
+  * multiple scattering: low-pT tracks that the generator deflects by the
+    Highland kinks of the medium the fit assumes (from the true momentum and
+    path lengths) have unit pulls and chi2/ndf within 5% of 1, and planted
+    mistakes in mult_scatt (VarBeta x1.2 or /1.2, log10 for ln) break that;
+    at 40-60 GeV with tiny hit errors chi2/ndf follows the fit's 20 GeV
+    momentum cap, which a planted missing cap breaks;
   * the forward-mode gradient used for the adjoint checks agrees with
     central finite differences of the oracle's own loss.
 The vectors are committed in tests/golden/bl_kat.json (tests/golden/make_bl_kat.py).
@@ -285,6 +291,104 @@ def test_pulls_and_chi2_of_smeared_high_pt_tracks():
         assert 0.85 < width < 1.2, (k, width)
         assert abs(np.median(pull)) < 0.05, (k, np.median(pull))
     assert 0.85 < np.mean(out[ok, 22]) < 1.1, np.mean(out[ok, 22])
+
+
+# ---------------------------------------------------------------- multiple scattering
+PULL_DIAG = {0: 0, 1: 5, 2: 9, 3: 12, 4: 14}  # cov15 index of each parameter's variance
+
+
+def fit_statistics(tr, out, flags):
+    """Robust pull widths (1.4826 x MAD) of (phi, tip, q/pT, cot, zip) against
+    the generating helix, and the mean chi2/ndf, over the tracks fitted with
+    the right charge."""
+    truth = tr["truth"].astype(np.float64)
+    ok = (flags == 0) & (np.sign(out[:, 2]) == np.sign(truth[2]))
+    widths = []
+    for k, ci in PULL_DIAG.items():
+        res = out[ok, k] - truth[k, ok]
+        if k == 0:
+            res = (res + np.pi) % (2 * np.pi) - np.pi
+        pull = res / np.sqrt(out[ok, 5 + ci])
+        widths.append(1.4826 * float(np.median(np.abs(pull - np.median(pull)))))
+    return ok.mean(), widths, float(np.mean(out[ok, 22]))
+
+
+def plant(scale=1.0, log10=False, no_cap=False):
+    O.lib().oracle_plant_ms(C.c_double(scale), int(log10), int(no_cap))
+
+
+@pytest.fixture(scope="module")
+def scattered_low_pt():
+    """0.5-2 GeV tracks of 3/4/5 hits over |eta| < 2.5 whose hits the generator
+    deflects by the scattering the fit models (Highland in the fit's assumed
+    medium, applied from the true momentum and path lengths): there the
+    kink terms dominate the 8-40 um hit errors."""
+    return G.fit_tracks(12000, seed=31, pt_min=0.5, pt_max=2.0, b_tesla=B_TESLA_MATCHED, ms_scale=1.0)
+
+
+def low_pt_failures(tr):
+    """The fit's scattering variances (VarBeta, broken_line.jl:200-206, and the
+    first-segment terms, :477-479,619-622) must describe the simulated
+    scattering: chi2/ndf within 5% of 1 and unit pulls.  The tip pull is a
+    little narrower than 1 (0.91 for triplets, 0.96 for quadruplets): the fit
+    evaluates the first segment's phi term at the segment's midpoint and
+    carries it to the origin from there, while the deflection happens at the
+    first hit, nearer to the origin; so its band is 0.85-1.1."""
+    out, flags = O.fit_bl(tr["hits"], tr["ge"], tr["nhit"], n_threads=8)
+    okf, w, chi2 = fit_statistics(tr, out, flags)
+    fails = []
+    if okf < 0.995:
+        fails.append(f"fitted fraction {okf:.4f}")
+    if abs(chi2 - 1.0) > 0.05:
+        fails.append(f"chi2/ndf {chi2:.4f}")
+    for k, name in enumerate(("phi", "tip", "q/pt", "cot", "zip")):
+        lo = 0.85 if name == "tip" else 0.9
+        if not lo < w[k] < 1.1:
+            fails.append(f"{name} pull width {w[k]:.3f}")
+    return fails, w, chi2
+
+
+def test_scattering_variances_give_unit_pulls_and_chi2(scattered_low_pt):
+    fails, w, chi2 = low_pt_failures(scattered_low_pt)
+    print("pull widths", [round(x, 3) for x in w], "chi2/ndf", round(chi2, 4))
+    assert not fails, fails
+
+
+@pytest.mark.parametrize("mistake", [dict(scale=1.2), dict(scale=1 / 1.2), dict(log10=True)],
+                         ids=["VarBeta x1.2", "VarBeta /1.2", "log10 for ln"])
+def test_scattering_pins_catch_a_planted_mistake(scattered_low_pt, mistake):
+    plant(**mistake)
+    try:
+        fails, _, chi2 = low_pt_failures(scattered_low_pt)
+    finally:
+        plant()
+    assert fails, f"a planted {mistake} went unnoticed (chi2/ndf {chi2:.4f})"
+
+
+def momentum_cap_check(tr):
+    """With hit errors of 0.05 um every residual is scattering.  The fit caps
+    B*R at 20 GeV in mult_scatt (broken_line.jl:50), so above 20 GeV it
+    assumes (pT/20)^2 times the simulated kink variances, and chi2/ndf must
+    come out at the mean of (20/pT)^2 (0.17 at 40-60 GeV), not at 1."""
+    out, flags = O.fit_bl(tr["hits"], tr["ge"], tr["nhit"], n_threads=8)
+    okf, _, chi2 = fit_statistics(tr, out, flags)
+    pt = 1.0 / np.abs(tr["truth"][2].astype(np.float64))
+    expect = float(np.mean(np.minimum(20.0, pt) ** 2 / pt ** 2))
+    return okf, chi2, expect
+
+
+def test_scattering_momentum_cap():
+    tr = G.fit_tracks(8000, seed=37, len_mix=(0.3, 0.7, 0.0), pt_min=40.0, pt_max=60.0, eta_max=1.2,
+                      b_tesla=B_TESLA_MATCHED, srphi=(5e-6, 5e-6), sz=(5e-6, 5e-6), ms_scale=1.0)
+    okf, chi2, expect = momentum_cap_check(tr)
+    assert okf > 0.995
+    assert abs(chi2 / expect - 1.0) < 0.07, (chi2, expect)
+    plant(no_cap=True)
+    try:
+        _, chi2_nocap, _ = momentum_cap_check(tr)
+    finally:
+        plant()
+    assert abs(chi2_nocap / expect - 1.0) > 0.5, (chi2_nocap, expect)
 
 
 # ---------------------------------------------------------------- gradient used by the adjoint checks
