@@ -175,6 +175,8 @@ struct pt_ctx {
     // the per-length lists (grown on demand) and the counters / ticket
     int n_cu = 256;
     pt_fit_tuning tuning[3] = {{256, 512, 1024}, {256, 512, 1024}, {256, 256, 1024}};
+    // schedule of pt_fit_bl_device_async (FP64, mixed) and of the adjoint
+    int schedule[3] = {PT_SCHED_PERSISTENT, PT_SCHED_PERSISTENT, PT_SCHED_PLAIN};
     uint32_t *fit_lists = nullptr;
     size_t fit_lists_cap = 0;  // track ids
     uint32_t *fit_counts = nullptr;
@@ -980,8 +982,8 @@ int pt_fit_bl_device_async(pt_ctx *c, uint32_t n, const float *hits, const float
                            float *chi2) {
     if (!c) return fail(PT_ERR_ARG, "null context");
     const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
-    return fit_flat(c, n, hits, ge, nhit, b_field, precision, PT_SCHED_PERSISTENT, c->tuning[pi], state, cov, pt,
-                    eta, chi2);
+    return fit_flat(c, n, hits, ge, nhit, b_field, precision, c->schedule[pi], c->tuning[pi], state, cov, pt, eta,
+                    chi2);
 }
 
 int pt_fit_bl_sched_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
@@ -1032,6 +1034,16 @@ int pt_fit_set_tuning(pt_ctx *c, int precision, const pt_fit_tuning *t) {
     return PT_OK;
 }
 
+int pt_fit_set_schedule(pt_ctx *c, int precision, int schedule) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+        return fail(PT_ERR_ARG, "unknown precision");
+    if (schedule != PT_SCHED_PLAIN && schedule != PT_SCHED_PER_LENGTH && schedule != PT_SCHED_PERSISTENT)
+        return fail(PT_ERR_ARG, "unknown schedule");
+    c->schedule[precision] = schedule;
+    return PT_OK;
+}
+
 int pt_fit_get_tuning(pt_ctx *c, int precision, pt_fit_tuning *t) {
     if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
     if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
@@ -1067,11 +1079,15 @@ static int fit_adjoint(pt_ctx *c, uint32_t n, const float *hits, const float *si
     a.state = state, a.chi2 = chi2, a.loss = loss, a.grad = grad;
     const pt_fit_tuning &t = c->tuning[PT_FIT_ADJOINT];
     pt::FitLaunch L;
-    // One thread per track in input order.  The persistent schedule's
-    // instantiation of this job gave wrong gradients for some tracks when
-    // built with -O3 (correct at -O1 and under this schedule), so it is not
-    // used for the adjoint.
-    L.schedule = PT_SCHED_PLAIN;
+    L.schedule = c->schedule[PT_FIT_ADJOINT];
+    if (L.schedule == PT_SCHED_PER_LENGTH && (size_t)2 * n > c->fit_lists_cap) {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        (void)hipFree(c->fit_lists);
+        c->fit_lists = nullptr;
+        c->fit_lists_cap = 0;
+        HIP_OK(hipMalloc((void **)&c->fit_lists, (size_t)2 * n * sizeof(uint32_t)));
+        c->fit_lists_cap = (size_t)2 * n;
+    }
     L.block = t.block, L.tile = t.tile, L.blocks = t.blocks;
     L.lists = c->fit_lists;
     L.counts = c->fit_counts;

@@ -194,6 +194,7 @@ HIP_SYMBOLS = [
     "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async", "pt_ctx_set_fit_precision", "pt_pipeline_init",
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
     "pt_set_hits_layer_start", "pt_load_tracks_async", "pt_fit_bl_sched_noalloc_async",
+    "pt_fit_set_schedule",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -246,6 +247,7 @@ def hip_lib():
                                               vp, vp]
         lib.pt_fit_bl_sched_noalloc_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp,
                                                       vp, vp, vp, vp]
+        lib.pt_fit_set_schedule.argtypes = [vp, C.c_int, C.c_int]
         lib.pt_fit_set_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
         lib.pt_fit_get_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
         lib.pt_fit_autotune.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp,

@@ -487,6 +487,11 @@ class AdjointBatch:
         self.ptrs = []
 
 
+def set_fit_schedule(ctx: Context, precision, schedule):
+    """Schedule of pt_fit_bl_device_async (FIT_FP64 / FIT_MIXED) or of the adjoint (FIT_ADJOINT)."""
+    A.check(ctx.lib.pt_fit_set_schedule(ctx.h, precision, schedule), "pt_fit_set_schedule")
+
+
 def set_fit_tuning(ctx: Context, precision, block, tile, blocks):
     t = A.PtFitTuning(block, tile, blocks)
     A.check(ctx.lib.pt_fit_set_tuning(ctx.h, precision, C.byref(t)), "pt_fit_set_tuning")

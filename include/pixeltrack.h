@@ -353,6 +353,12 @@ int pt_fit_bl_sched_noalloc_async(pt_ctx *ctx, uint32_t n_tracks, const float *h
 /* Tuning of the persistent schedule per precision (PT_FIT_FP64, PT_FIT_MIXED
  * or PT_FIT_ADJOINT; default: 256 threads, 512-track tiles, 4 blocks per CU). */
 int pt_fit_set_tuning(pt_ctx *ctx, int precision, const pt_fit_tuning *t);
+/* Schedule used by pt_fit_bl_device_async for PT_FIT_FP64 / PT_FIT_MIXED
+ * (default PT_SCHED_PERSISTENT) and by the adjoint entry points for
+ * PT_FIT_ADJOINT (default PT_SCHED_PLAIN: the fastest for batches of one
+ * length; PT_SCHED_PERSISTENT is faster for mixed lengths).  No reference
+ * counterpart. */
+int pt_fit_set_schedule(pt_ctx *ctx, int precision, int schedule);
 int pt_fit_get_tuning(pt_ctx *ctx, int precision, pt_fit_tuning *t);
 /* Times every candidate launch shape of the persistent schedule on the given
  * batch (device pointers; outputs are overwritten), keeps the fastest in the
