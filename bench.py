@@ -33,10 +33,45 @@ FP32_PEAK_TFLOPS = 157.3
 FIT_FLOPS = {3: 1468, 4: 2007}
 
 
-def kernel_bytes(name, c):
-    """Algorithmic HBM bytes of one launch over the batch: the bytes the kernel
-    must read and write at least once (unique inputs and outputs, no re-reads),
-    summed over the events.  c: dict of per-event counter arrays."""
+# The measurement model of the path: algorithmic HBM bytes per event of each
+# stage, from the per-event counters (H hits, D doublets, C neighbour entries,
+# T stored tuples, Th their hits), the bytes each stage must move at least once.
+STAGES = {
+    # stage: (kernels that implement it, bytes per event)
+    "doublets (F2)": (("k_doublet_count", "k_doublet_scan", "k_doublet_fill", "k_ohc_rebuild"),
+                      lambda H, D, C, T, Th: 16 * H + 32 * D + 5168),
+    "connect (F3)": (("k_connect",), lambda H, D, C, T, Th: 8 * H + 34 * D + 4 * C),
+    "fishbone (F4)": (("k_fish_list", "k_fishbone"), lambda H, D, C, T, Th: 12 * H + 32 * D),
+    "ntuplets, duplicates, multiplicity (F5-F8)": (
+        ("k_dfs_count", "k_dfs_scan", "k_dfs_emit", "k_tuples_post", "k_multiplicity"),
+        lambda H, D, C, T, Th: 28 * D + 4 * C + 6 * Th + 12 * T),
+}
+STAGE_OF_KERNEL = {k: st for st, (ks, _) in STAGES.items() for k in ks}
+BOX_CPU_SHARE = 16  # host CPUs granted to a one-GPU job on the measurement boxes
+
+
+def counts(c):
+    return (c["n_hits"], c["n_cells"], c["n_connections"], np.minimum(c["n_tuples"], 32768),
+            np.minimum(c["n_tuple_hits"], 163840))
+
+
+def stage_bytes(stage, c):
+    """Algorithmic bytes of one stage over the batch (sum over the events)."""
+    return int(np.sum(STAGES[stage][1](*counts(c))))
+
+
+def fit_bytes(c):
+    """24 N + 98 bytes per fitted track (N = 3, or 4 for 4- and 5-hit tuples,
+    the fifth hit's id read too), 24,576 fits per length at most."""
+    nb = c["n_by_len"]
+    f3, f4, f5 = (np.minimum(nb[:, k], 24576) for k in (3, 4, 5))
+    return int(np.sum(170 * f3 + 194 * f4 + 196 * f5))
+
+
+def kernel_model_bytes(name, c):
+    """A finer per-kernel byte model (what each kernel's own data layout reads
+    and writes once: records, lists, counters), summed over the events.
+    c: dict of per-event counter arrays."""
     H, S, D = c["n_hits"], c["n_inner_slots"], c["n_cells"]
     C, T = c["n_connections"], np.minimum(c["n_tuples"], 32768)
     Th = np.minimum(c["n_tuple_hits"], 163840)
@@ -89,13 +124,38 @@ def profiled_traffic(kernel, path=ROOT / "profiles" / "hbm_traffic_pu50_batch100
     return None
 
 
-def run_cpu_baseline(events, frames, budget_s=12.0, max_threads=16):
-    """Oracle (serial restatement of the reference stage, incl. classification)
-    on host threads, one event per thread at a time like the reference's
-    --numberOfStreams runner."""
+def cpu_info():
+    """The host the CPU baselines run on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"model": model, "host_logical_cpus": os.cpu_count(), "cpus_available_to_this_process": affinity,
+            "threads_used": max(1, min(affinity, BOX_CPU_SHARE))}
+
+
+def oracle():
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the checker, used here only as the CPU baseline
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    return O
+
+
+def run_cpu_baseline(events, frames, budget_s=12.0, label="PU50"):
+    """Oracle (serial restatement of the reference stage, incl. classification)
+    on host threads, one event per thread at a time like the reference's
+    --numberOfStreams runner (all cores the job is given: the measurement
+    boxes grant BOX_CPU_SHARE CPUs per GPU)."""
+    O = oracle()
+    info = cpu_info()
+    threads = info["threads_used"]
     # warm-up (excluded, main.jl:148-151)
     O.make_tuples(events[0], frames, stop_after=0, debug=False)
     # size the sample to about budget_s of CPU work
@@ -125,12 +185,25 @@ def run_cpu_baseline(events, frames, budget_s=12.0, max_threads=16):
         t.join()
     wall = time.perf_counter() - t0
     return dict(value=round(len(work) / wall, 2), unit="events/s", cores=threads, kind="port",
-                sample=f"{len(work)} PU50 events (the {len(events)} benchmark events cycled), oracle = serial C++ "
+                sample=f"{len(work)} {label} events (the {len(events)} benchmark events cycled), oracle = serial C++ "
                        f"restatement of the Julia stage incl. classification, {threads} threads, "
-                       f"{per_event * 1e3:.1f} ms/event single-thread")
+                       f"{per_event * 1e3:.1f} ms/event single-thread",
+                single_thread_events_per_s=round(1.0 / per_event, 2), host=info)
 
 
-def fit_only(ctx, n_tracks=1 << 20, steps=20):
+def timed_cpu(fn, min_s=2.0):
+    """Seconds per call of fn, repeated until min_s has passed (after one warm-up call)."""
+    fn()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_s:
+            return dt / n
+
+
+def fit_only(ctx, n_tracks=1 << 20, steps=20, cpu=False):
     """Fit-only workload: 1,048,576 pre-gathered tracks (seed 7, lengths
     3/4/5 in a 30/55/15% mix in random order, pT log-uniform 0.5-10 GeV,
     per-layer sigma_rphi 8-15 um and sigma_z 15-40 um), resident in HBM.
@@ -168,12 +241,20 @@ def fit_only(ctx, n_tracks=1 << 20, steps=20):
             out[f"{pname}_persistent_frac_of_peak"] = round(tf / peak, 4)
         out["persistent_vs_per_length_mixed"] = round(out["mixed_persistent_tracks_per_s"] /
                                                       out["mixed_per_length_tracks_per_s"], 3)
+        if cpu:
+            O, info = oracle(), cpu_info()
+            th = info["threads_used"]
+            dt = timed_cpu(lambda: O.fit_bl(tr["hits"], tr["ge"], tr["nhit"], n_threads=th))
+            out["cpu_baseline"] = {"value": round(n_tracks / dt, 1), "unit": "tracks/s", "cores": th, "kind": "port",
+                                   "sample": f"the same {n_tracks} tracks, FP64 oracle fit (restatement of "
+                                             f"launchBrokenLineKernelsOnCPU's fit), {th} threads"}
+            out["fp64_persistent_vs_cpu"] = round(out["fp64_persistent_tracks_per_s"] / (n_tracks / dt), 1)
     finally:
         fb.free()
     return out
 
 
-def adjoint_only(ctx, n_tracks=262144, steps=20):
+def adjoint_only(ctx, frames, n_tracks=262144, steps=20, cpu=False):
     """Adjoint workload: 262,144 quadruplets (seed 11, pT log-uniform 0.5-50
     GeV) resident in HBM.  tracks/s of the plain FP64 fit against the mixed
     fit + adjoint of L = chi2_line + chi2_circle + |params - truth|^2 with
@@ -217,13 +298,30 @@ def adjoint_only(ctx, n_tracks=262144, steps=20):
             out[f"{k}_ms"] = round(t * 1e3, 4)
         out["fused_adjoint_vs_fp64_fit"] = round(t_fp64 / t_fused, 3)
         out["rrule_vs_fp64_fit"] = round(t_fp64 / t_rrule, 3)
+        if cpu:
+            # FP64 oracle fit, its forward-mode (dual-number) loss gradient, and
+            # the central finite-difference gradient: 2 x 5 N + 1 fits per track
+            O, info = oracle(), cpu_info()
+            th = info["threads_used"]
+            n_s = 65536
+            sub = {k: np.ascontiguousarray(v[..., :n_s]) for k, v in tr.items()}
+            t_fit = timed_cpu(lambda: O.fit_bl(sub["hits"], sub["ge"], sub["nhit"], n_threads=th))
+            t_grad = timed_cpu(lambda: O.fit_grad_many(sub, frames, n_threads=th))
+            out["cpu_baseline"] = {
+                "unit": "tracks/s", "cores": th, "kind": "port",
+                "sample": f"the first {n_s} of the same tracks, {th} threads",
+                "fp64_fit_tracks_per_s": round(n_s / t_fit, 1),
+                "forward_mode_gradient_tracks_per_s": round(n_s / t_grad, 1),
+                "central_fd_gradient_tracks_per_s": round(n_s / t_fit / (2 * 5 * 4 + 1), 1)}
+            out["fused_adjoint_vs_cpu_forward_mode"] = round(out["mixed_fit_adjoint_fused_tracks_per_s"] /
+                                                             (n_s / t_grad), 1)
     finally:
         fb.free()
         ab.free()
     return out
 
 
-def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
+def pu200_stream(frames, n_events=200, group_size=8, n_groups=4, cpu=False, cpu_seconds=12.0):
     """PU200-like stream: 200 events (seeds 1001..1200, 7000 helices from 200
     vertices, +5% noise), resident in HBM, run through the graph pipeline with
     `group_size` events per captured hipGraph and `n_groups` groups in flight
@@ -259,7 +357,10 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4):
         wall = time.perf_counter() - t0
         per_event = np.repeat(np.array(lat), group_size)
         cnt = [pl.counters(g, k) for g in range(n_groups) for k in range(group_size)]
-        return {"workload": f"{n_events} pu200 events (seeds 1001..{1000 + n_events}), mean {hits.mean():.0f} hits "
+        cpu_res = run_cpu_baseline(events, frames, budget_s=cpu_seconds, label="PU200") if cpu else None
+        return {"cpu_baseline": cpu_res,
+                "gpu_vs_cpu": round(n_sub * group_size / wall / cpu_res["value"], 1) if cpu else None,
+                "workload": f"{n_events} pu200 events (seeds 1001..{1000 + n_events}), mean {hits.mean():.0f} hits "
                             f"(max {hits.max()}), mixed-precision fit",
                 "events_in_flight": group_size * n_groups, "events_per_graph": group_size, "graphs_in_flight": n_groups,
                 "events_per_s": round(n_sub * group_size / wall, 1),
@@ -316,7 +417,7 @@ def main():
             res = pu200_stream(frames)
         else:
             c = ca.Context(max_events=1, frames=frames)
-            res = fit_only(c) if args.only == "fit" else adjoint_only(c)
+            res = fit_only(c) if args.only == "fit" else adjoint_only(c, frames)
             c.close()
         print(json.dumps({"workload": args.only, "result": res}))
         return
@@ -353,7 +454,7 @@ def main():
     for c in ctxs:
         c.sync()
     counters = [ctx.counters(i) for i in range(len(events))]
-    cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0] if k != "n_by_len"}
+    cnt = {k: np.array([c[k] for c in counters], dtype=np.int64) for k in counters[0]}
 
     # timed region: production mode (no timing marks); step k is one pass of the
     # stage over the batch on context k % inflight.  The output TrackSOAs are
@@ -391,16 +492,46 @@ def main():
     avg = {k: v / n_inst for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
     dom_ms = avg[dom]
-    # traffic: measured HBM bytes per launch of the same kernel and workload
-    # (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled, profiles/hbm_traffic_pu50_batch100.md)
+    # achieved = the stage's algorithmic bytes (the measurement model above) over
+    # the kernel's average duration; traffic = measured HBM bytes per launch of
+    # the same kernel and workload (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled),
+    # read from the committed summary of that profile, not measured in this run
+    std_work = args.events == 100 and args.preset == "pu50"
     roof = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
-                traffic=profiled_traffic(dom) if args.events == 100 and args.preset == "pu50" else None,
-                avg_launch_ms=round(dom_ms, 5))
-    b = kernel_bytes(dom, cnt)
-    if b is not None and dom_ms > 0:
+                traffic=profiled_traffic(dom) if std_work else None,
+                traffic_source="profiles/hbm_traffic_pu50_batch100.md (rocprofv3 --pmc of this workload)"
+                if std_work else None, avg_launch_ms=round(dom_ms, 5))
+    if dom in STAGE_OF_KERNEL and len(STAGES[STAGE_OF_KERNEL[dom]][0]) == 1 and dom_ms > 0:
+        b = stage_bytes(STAGE_OF_KERNEL[dom], cnt)
         ach = b / (dom_ms * 1e-3) / 1e9
-        roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b)
+        roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), algorithmic_bytes=b,
+                    algorithmic_model=STAGE_OF_KERNEL[dom])
+    bk = kernel_model_bytes(dom, cnt)
+    if bk is not None and dom_ms > 0:
+        roof.update(kernel_model_bytes=bk, kernel_model_frac=round(bk / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
     kernels_ms = {k: round(v, 4) for k, v in sorted(avg.items(), key=lambda kv: -kv[1])}
+    # every stage of the measurement model against the HBM roofline
+    stages = {}
+    for st, (ks, _) in STAGES.items():
+        ms = sum(avg.get(k, 0.0) for k in ks)
+        if ms > 0:
+            b = stage_bytes(st, cnt)
+            stages[st] = {"kernels": list(ks), "ms": round(ms, 4), "bytes": b,
+                          "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1),
+                          "frac_of_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if "k_fit_tuples" in avg:
+        b = fit_bytes(cnt)
+        stages["fit (F9-F14)"] = {"kernels": ["k_fit_tuples"], "ms": round(avg["k_fit_tuples"], 4), "bytes": b,
+                                  "gb_per_s": round(b / (avg["k_fit_tuples"] * 1e-3) / 1e9, 1)}
+    tot_b = sum(v["bytes"] for v in stages.values())
+    tot_ms = sum(avg.values())
+    stages["whole stage, one batch on one stream"] = {
+        "ms": round(tot_ms, 4), "bytes": tot_b, "gb_per_s": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+        "frac_of_hbm_peak": round(tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    stages["whole stage, timed steps"] = {
+        "ms": round(wall * 1e3 / args.steps, 4), "bytes": tot_b,
+        "gb_per_s": round(tot_b / (wall / args.steps) / 1e9, 1),
+        "frac_of_hbm_peak": round(tot_b / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
 
     # the same stage building the phi histogram on the device first (no
     # histogram handed in), and the host-buffer entry point (PCIe included)
@@ -435,16 +566,17 @@ def main():
 
     fit = adj = stream = None
     if not args.no_fit:
-        fit = fit_only(ctx)
+        fit = fit_only(ctx, cpu=not args.no_cpu)
         print(f"[bench] fit-only: {fit}", file=sys.stderr, flush=True)
-        adj = adjoint_only(ctx)
+        adj = adjoint_only(ctx, frames, cpu=not args.no_cpu)
         print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
-        stream = pu200_stream(frames)
+        stream = pu200_stream(frames, cpu=not args.no_cpu, cpu_seconds=args.cpu_seconds)
         print(f"[bench] pu200 stream: {stream}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(events, frames, budget_s=args.cpu_seconds)
+        cpu["gpu_vs_cpu"] = round(value / cpu["value"], 1)
 
     line = {
         "metric": METRIC,
@@ -470,6 +602,7 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "stages": stages,
         "kernels_ms_per_step": kernels_ms,
         "kernels_sum_ms_single_stream": round(sum(avg.values()), 4),
         "streams": args.streams,

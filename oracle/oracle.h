@@ -61,6 +61,13 @@ int oracle_fit_opcount(int N, const double *hits, const float *ge, double B, uin
 int oracle_fit_grad(int N, const double *xyz, const double *sig, const float *rot, double B, const double *truth,
                     double *value, double *grad);
 
+/* oracle_fit_grad over a fit-only batch in the layout of
+ * pt_fit_bl_loss_grad_async (local sigmas, module ids, truth may be NULL),
+ * on n_threads host threads: loss[t], grad[(c*5+k)*n+t]. */
+int oracle_fit_grad_many(uint32_t n, const float *hits, const float *sig, const int16_t *det, const uint8_t *nhit,
+                         const float *rot, double b_field, const float *truth, double *loss, double *grad,
+                         int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

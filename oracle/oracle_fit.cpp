@@ -307,4 +307,51 @@ int oracle_fit_grad(int N, const double *xyz, const double *sig, const float *ro
     return PT_ERR_ARG;
 }
 
+// Forward-mode loss gradients of a fit-only batch (layout of
+// pt_fit_bl_loss_grad_async): hits[(c*5+k)*n+t], sig[(c*5+k)*n+t] (local
+// sigmas), det[k*n+t], nhit[t] (5 fitted as 4), truth[k*n+t] (may be NULL),
+// rot[9*m] module frames; loss[t] and grad[(c*5+k)*n+t], c = x, y, z,
+// sigma_rphi, sigma_z.  Used as the CPU baseline of the adjoint workload.
+int oracle_fit_grad_many(uint32_t n, const float *hits, const float *sig, const int16_t *det, const uint8_t *nhit,
+                         const float *rot, double b_field, const float *truth, double *loss, double *grad,
+                         int n_threads) {
+    const double B = (double)(float)b_field;
+    auto work = [&](uint32_t t0, uint32_t t1) {
+        for (uint32_t t = t0; t < t1; ++t) {
+            const int N = nhit[t] == 3 ? 3 : 4;
+            double xyz[15], sg[10], tr[5], g[20] = {0}, v = 0.0;
+            float R[36];
+            for (int k = 0; k < N; ++k) {
+                for (int c = 0; c < 3; ++c) xyz[3 * k + c] = hits[(size_t)(c * 5 + k) * n + t];
+                for (int c = 0; c < 2; ++c) sg[2 * k + c] = sig[(size_t)(c * 5 + k) * n + t];
+                int d = det[(size_t)k * n + t];
+                if (d < 0 || d >= (int)PT_N_MODULES) d = 0;
+                for (int c = 0; c < 9; ++c) R[9 * k + c] = rot[9 * d + c];
+            }
+            if (truth)
+                for (int k = 0; k < 5; ++k) tr[k] = truth[(size_t)k * n + t];
+            if (N == 3)
+                grad_impl<3>(xyz, sg, R, B, truth ? tr : nullptr, &v, g);
+            else
+                grad_impl<4>(xyz, sg, R, B, truth ? tr : nullptr, &v, g);
+            loss[t] = v;
+            for (int k = 0; k < 5; ++k)
+                for (int c = 0; c < 5; ++c) grad[(size_t)(c * 5 + k) * n + t] = (k < N) ? g[5 * k + c] : 0.0;
+        }
+    };
+    if (n_threads <= 1) {
+        work(0, n);
+    } else {
+        std::vector<std::thread> th;
+        const uint32_t chunk = (n + n_threads - 1) / n_threads;
+        for (int i = 0; i < n_threads; ++i) {
+            uint32_t a = i * chunk, b = std::min<uint32_t>(n, a + chunk);
+            if (a >= b) break;
+            th.emplace_back(work, a, b);
+        }
+        for (auto &x : th) x.join();
+    }
+    return PT_OK;
+}
+
 }  // extern "C"

@@ -37,6 +37,7 @@ def lib():
         L.oracle_bl_detail.argtypes = [C.c_int, vp, vp, C.c_double, vp, C.c_int]
         L.oracle_fit_opcount.argtypes = [C.c_int, vp, vp, C.c_double, vp]
         L.oracle_fit_grad.argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.oracle_fit_grad_many.argtypes = [C.c_uint32, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, C.c_int]
         L.oracle_plant_ms.argtypes = [C.c_double, C.c_int, C.c_int]
         L.oracle_plant_ms.restype = None
         L.oracle_plant_fishbone.argtypes = [C.c_int, C.c_int]
@@ -127,6 +128,22 @@ def load_tracks(tracks, pt_min=0.5):
                                     ezt2.ctypes.data, ptt2.ctypes.data) == 0
     k = int(n[0])
     return {"n_tracks": k, "index_track": idx[:k], "zt": zt[:k], "ezt2": ezt2[:k], "ptt2": ptt2[:k]}
+
+
+def fit_grad_many(tr, frames, b_field=A.B_FIELD_INV_GEV, n_threads=1, with_truth=True):
+    """Forward-mode loss gradients of a fit-only batch (generator dict) -> (loss [n], grad [5,5,n])."""
+    n = int(tr["nhit"].shape[0])
+    loss = np.zeros(n, np.float64)
+    grad = np.zeros((5, 5, n), np.float64)
+    arrs = [np.ascontiguousarray(tr[k], dt) for k, dt in (("hits", np.float32), ("sig", np.float32),
+                                                          ("det", np.int16), ("nhit", np.uint8))]
+    rot = np.ascontiguousarray(frames, np.float32)
+    truth = np.ascontiguousarray(tr["truth"], np.float32) if with_truth else None
+    rc = lib().oracle_fit_grad_many(n, *[a.ctypes.data for a in arrs], rot.ctypes.data, b_field,
+                                    truth.ctypes.data if truth is not None else None, loss.ctypes.data,
+                                    grad.ctypes.data, n_threads)
+    assert rc == 0
+    return loss, grad
 
 
 def fit_bl(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):

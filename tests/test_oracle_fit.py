@@ -454,3 +454,18 @@ def test_op_counts_are_stable():
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
     assert bench.FIT_FLOPS == {3: sum(counts[3][:4]), 4: sum(counts[4][:4])}
+
+
+def test_batched_gradient_equals_per_track(frames):
+    """oracle_fit_grad_many (the adjoint workload's CPU baseline) is
+    oracle_fit_grad applied track by track in the fit-only layout."""
+    tr = G.fit_tracks(64, seed=19, pt_min=0.5, pt_max=50.0)
+    loss, grad = O.fit_grad_many(tr, frames, n_threads=4)
+    for t in range(64):
+        N = 3 if tr["nhit"][t] == 3 else 4
+        xyz = tr["hits"][:, :N, t].T.astype(np.float64).copy()
+        sig = tr["sig"][:, :N, t].T.astype(np.float64).copy()
+        L, g = loss_and_grad(N, xyz, sig, frames[tr["det"][:N, t]], tr["truth"][:, t].astype(np.float64))
+        assert loss[t] == L
+        np.testing.assert_array_equal(grad[:, :N, t].T.reshape(-1), g)
+        assert not np.any(grad[:, N:, t])
