@@ -203,6 +203,39 @@ def timed_cpu(fn, min_s=2.0):
             return dt / n
 
 
+def host_tasks(events, frames, n_tasks, seconds=3.0):
+    """The reference's runner model on the GPU: n_tasks host threads, each with
+    its own context (the per-task scratch of ca_hit_ntuplet_generator.jl:9),
+    each calling the host-buffer entry point one event at a time."""
+    from pixeltrack import ca
+    gens = [ca.CAHitNtupletGeneratorOnGPU(frames) for _ in range(n_tasks)]
+    for g in gens:
+        g.make_tuples(events[0])
+    done = [0] * n_tasks
+    stop = [False]
+
+    def task(i):
+        k = i
+        while not stop[0]:
+            gens[i].make_tuples(events[k % len(events)])
+            done[i] += 1
+            k += n_tasks
+
+    ths = [threading.Thread(target=task, args=(i,)) for i in range(n_tasks)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    time.sleep(seconds)
+    stop[0] = True
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    for g in gens:
+        g.ctx.close()
+    return {"tasks": n_tasks, "events": int(sum(done)), "events_per_s": round(sum(done) / wall, 1),
+            "events_per_s_per_task": round(sum(done) / wall / n_tasks, 1)}
+
+
 def fit_only(ctx, n_tracks=1 << 20, steps=20, cpu=False):
     """Fit-only workload: 1,048,576 pre-gathered tracks (seed 7, lengths
     3/4/5 in a 30/55/15% mix in random order, pT log-uniform 0.5-10 GeV,
@@ -442,6 +475,7 @@ def main():
         w_p = time.perf_counter() - t0
         host_path = {"events_per_s": round(n_host / w_p, 1), "events": n_host,
                      "note": "pt_make_tuples: host hits in, host TrackSOA out, one event per call, synchronous"}
+        host_path["tasks"] = host_tasks(events, frames, BOX_CPU_SHARE)
         print(f"[bench] host buffers: {host_path}", file=sys.stderr, flush=True)
 
     n_fl = max(1, args.inflight)

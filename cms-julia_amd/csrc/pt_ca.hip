@@ -1995,12 +1995,12 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
 // ------------------------------------------------------------------ launchers
 // Blocks per event for a grid-stride kernel over `items` items per event:
 // enough to cover the items, at most what keeps about 2048 blocks in flight.
-static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t bs = 256) {
+static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t xcd, uint32_t bs = 256) {
     uint32_t per = (items + bs - 1) / bs;
     const uint32_t target = (2048u + n_events - 1) / n_events;
     if (per > target) per = target;
     if (per < 1) per = 1;
-    return EvGrid{n_events, per};
+    return EvGrid{n_events, per, xcd};
 }
 
 // Blocks per event for a kernel whose wavefronts take work by ticket: exactly
@@ -2009,7 +2009,7 @@ static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t bs = 25
 static std::mutex g_occ_mu;
 static std::map<std::pair<const void *, int>, std::pair<int, int>> g_occ;  // (kernel, device) -> (blocks/CU, CUs)
 template <class K>
-static EvGrid ev_grid_resident(K kernel, uint32_t n_events, int bs = 256) {
+static EvGrid ev_grid_resident(K kernel, uint32_t n_events, uint32_t xcd, int bs = 256) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     int per_cu = 0, cus = 0;
@@ -2031,7 +2031,7 @@ static EvGrid ev_grid_resident(K kernel, uint32_t n_events, int bs = 256) {
     const uint32_t ev_per_xcd = (n_events + 7u) / 8u;
     uint32_t per = (uint32_t)(cus / 8) * (uint32_t)per_cu / ev_per_xcd;
     if (per < 1) per = 1;
-    return EvGrid{n_events, per};
+    return EvGrid{n_events, per, xcd};
 }
 
 // Launch sequence of the CA stage.  When `tm` is given, an event is recorded
@@ -2042,45 +2042,46 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     auto mark = [&](const char *name) {
         if (tm) tm->record(name, st);
     };
-    const EvGrid one{n_events, 1u};
+    const uint32_t xcd = cfg.xcd;
+    const EvGrid one{n_events, 1u, xcd};
     mark("start");
     const EvGrid g_reset = ev_grid(max_hits + max_slots / kSlotTile + kDfsAgg + kMaxCells / 64u + 64u +
                                        (zero_state ? 20u * kMaxTuples : 0u),
-                                   n_events);
+                                   n_events, xcd);
     k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
     mark("k_reset");
-    const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events);
+    const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events, xcd);
     k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg, zero_state);
     mark("k_doublet_count");
-    const EvGrid g_dscan = ev_grid(max_slots / 4u, n_events);
+    const EvGrid g_dscan = ev_grid(max_slots / 4u, n_events, xcd);
     k_doublet_scan<<<ev_grid_dim(g_dscan), 256, 0, st>>>(d_evs, g_dscan, cfg);
     mark("k_doublet_scan");
-    const EvGrid g_fill = ev_grid_resident(k_doublet_fill, n_events);
+    const EvGrid g_fill = ev_grid_resident(k_doublet_fill, n_events, xcd);
     k_doublet_fill<<<ev_grid_dim(g_fill), 256, 0, st>>>(d_evs, g_fill, cfg);
     mark("k_doublet_fill");
     k_ohc_rebuild<<<ev_grid_dim(one), 64, 0, st>>>(d_evs, one);
     mark("k_ohc_rebuild");
-    const EvGrid g_conn = ev_grid_resident(k_connect, n_events);
+    const EvGrid g_conn = ev_grid_resident(k_connect, n_events, xcd);
     k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
     mark("k_connect");
     if (cfg.p.early_fish_bone) {
         k_fish_list<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one);
         mark("k_fish_list");
-        const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events);
+        const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events, xcd);
         k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish, cfg);
     } else {
         k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
     }
     mark("k_fishbone");
-    const EvGrid g_start = ev_grid(est_cells / 8u, n_events);
+    const EvGrid g_start = ev_grid(est_cells / 8u, n_events, xcd);
     k_dfs_count<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_count");
-    const EvGrid g_scan = ev_grid(est_cells / 4u, n_events);
+    const EvGrid g_scan = ev_grid(est_cells / 4u, n_events, xcd);
     k_dfs_scan<<<ev_grid_dim(g_scan), 256, 0, st>>>(d_evs, g_scan);
     mark("k_dfs_scan");
     k_dfs_emit<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_emit");
-    const EvGrid g_post = ev_grid(kMaxTuples + 1u + PT_MAX_TUPLE_HITS / 4u, n_events);
+    const EvGrid g_post = ev_grid(kMaxTuples + 1u + PT_MAX_TUPLE_HITS / 4u, n_events, xcd);
     k_tuples_post<<<ev_grid_dim(g_post), 256, 0, st>>>(d_evs, g_post);
     mark("k_tuples_post");
     k_multiplicity<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);

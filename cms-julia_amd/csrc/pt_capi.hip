@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -26,7 +27,7 @@ void launch_fit_adjoint(const AdjArgs &a, int loss_mode, const FitLaunch &L, hip
 void launch_local_to_global(uint32_t n, const float *sig, const int16_t *det, const float *frames, float *ge,
                             hipStream_t st);
 void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
-void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, hipStream_t st);
+void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, uint32_t xcd, hipStream_t st);
 void launch_load_tracks(const EvDesc *d_evs, uint32_t n_events, float pt_min, hipStream_t st);
 int noalloc_scratch(int device, uint32_t **counts, uint32_t **lists);
 }  // namespace pt
@@ -122,22 +123,46 @@ size_t carve(char *base, pt::Scratch *s) {
 }
 
 struct Staging {
-    // device copies for the host-memory entry point (one event)
-    float *xg = nullptr, *yg, *zg, *rg, *xerr, *yerr;
-    int16_t *iphi, *ysize, *det;
-    uint32_t *hist_off;
-    uint16_t *hist_bins;
+    // the host-memory entry point (one event): the inputs are packed into a
+    // pinned block and sent with one copy into `in` (a layout that depends on
+    // the event's hit count); the TrackSOA is written to fixed-capacity device
+    // arrays and the defined rows come back through a pinned block
+    char *in = nullptr;       // device, room for PT_MAX_HITS hits and the histogram
+    char *h_in = nullptr;     // pinned, same size
+    char *h_out = nullptr;    // pinned, room for a full TrackSOA
+    size_t in_bytes = 0, out_bytes = 0;
     int32_t *quality;
     float *chi2, *eta, *pt, *state, *cov;
     uint32_t *hit_off, *det_off;
     uint16_t *hit_bins, *det_bins;
-    void *block = nullptr;
+    void *block = nullptr;    // device TrackSOA arrays
+};
+
+// Byte offsets of the packed inputs of an event of n hits: six float
+// arrays, the 1281 histogram offsets, three int16 arrays and the histogram
+// bins; every array starts on a 16-byte boundary.
+struct PackedInputs {
+    size_t f[6], hist_off, i16[3], hist_bins, total;
+    explicit PackedInputs(size_t n) {
+        size_t o = 0;
+        auto take = [&](size_t b) {
+            const size_t r = o;
+            o += (b + 15) & ~size_t(15);
+            return r;
+        };
+        for (auto &x : f) x = take(4 * n);
+        hist_off = take(4 * PT_HIST_NOFF);
+        for (auto &x : i16) x = take(2 * n);
+        hist_bins = take(2 * n);
+        total = o;
+    }
 };
 
 }  // namespace
 
 struct pt_ctx {
     int device = 0;
+    uint32_t xcd = 0;  // XCD of the first event of this context's batches
     pt_params p;
     uint32_t max_events = 0;
     hipStream_t stream = nullptr;
@@ -264,6 +289,12 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         return fail(PT_ERR_HIP, "hipGetSymbolAddress of the fixed fit scratch failed on device %d", device);
     pt_ctx *c = new pt_ctx();
     c->device = device;
+    {
+        // contexts take the XCDs in turn, so single-event contexts used
+        // side by side (one per host task) run on different XCDs
+        static std::atomic<uint32_t> next_xcd{0};
+        c->xcd = next_xcd.fetch_add(1) & 7u;
+    }
     c->p = prm;
     c->max_events = max_events;
     auto cleanup = [&](int rc) {
@@ -338,6 +369,9 @@ int pt_ctx_destroy(pt_ctx *c) {
     (void)hipHostFree(c->pl_h_evs);
     (void)hipFree(c->d_frames);
     (void)hipFree(c->stg.block);
+    (void)hipFree(c->stg.in);
+    (void)hipHostFree(c->stg.h_in);
+    (void)hipHostFree(c->stg.h_out);
     (void)hipFree(c->fit_lists);
     (void)hipFree(c->fit_counts);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -413,6 +447,7 @@ static int build_desc(pt_ctx *c, pt::EvDesc &e, uint32_t i, const pt_hits_view &
 static pt::Config make_config(const pt_ctx *c, double b_field) {
     pt::Config cfg;
     cfg.p = c->p;
+    cfg.xcd = c->xcd;
     cfg.B = (double)(float)b_field;  // HelixFitOnGPU(Float32(b_field)) then Float64
     cfg.frames = c->d_frames;
     cfg.fit_precision = c->fit_precision;
@@ -485,7 +520,7 @@ static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_t
     const uint32_t n_sub = marks ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)c->n_sub, n));
     auto launch_range = [&](uint32_t e0, uint32_t e1, hipStream_t st) {
         if (any_hist) {
-            pt::launch_phi_hist(c->d_evs + e0, e1 - e0, st);
+            pt::launch_phi_hist(c->d_evs + e0, e1 - e0, c->xcd, st);
             if (marks) marks->record("k_phi_hist", st);
         }
         uint32_t max_hits = 0, max_slots = 0;
@@ -547,7 +582,7 @@ int pt_fill_phi_histogram_async(pt_ctx *c, uint32_t n_events, const pt_hits_view
     }
     rc = upload_descs(c, h, n_events, false);
     if (rc) return rc;
-    pt::launch_phi_hist(c->d_evs, n_events, c->stream);
+    pt::launch_phi_hist(c->d_evs, n_events, c->xcd, c->stream);
     HIP_OK(hipGetLastError());
     return PT_OK;
 }
@@ -613,24 +648,22 @@ int pt_get_counters(pt_ctx *c, uint32_t event, pt_counters *cnt) {
 static int ensure_staging(pt_ctx *c) {
     if (c->stg.block) return PT_OK;
     Staging &s = c->stg;
-    const size_t H = PT_MAX_HITS, T = PT_MAX_TUPLES, M = PT_MAX_TUPLE_HITS;
+    const size_t T = PT_MAX_TUPLES, M = PT_MAX_TUPLE_HITS;
     size_t off = 0;
     auto take = [&](size_t b) {
         size_t o = off;
         off += align256(b);
         return o;
     };
-    const size_t o_xg = take(6 * H * 4), o_i16 = take(3 * H * 2), o_ho = take(PT_HIST_NOFF * 4), o_hb = take(H * 2);
     const size_t o_q = take(T * 4), o_f = take(3 * T * 4), o_st = take(5 * T * 4), o_cv = take(15 * T * 4);
     const size_t o_off = take(2 * (T + 1) * 4), o_bins = take(2 * M * 2);
+    s.in_bytes = PackedInputs(PT_MAX_HITS).total;
+    s.out_bytes = off;
     HIP_OK(hipMalloc(&s.block, off));
+    HIP_OK(hipMalloc((void **)&s.in, s.in_bytes));
+    HIP_OK(hipHostMalloc((void **)&s.h_in, s.in_bytes, hipHostMallocDefault));
+    HIP_OK(hipHostMalloc((void **)&s.h_out, s.out_bytes, hipHostMallocDefault));
     char *b = (char *)s.block;
-    float *f = (float *)(b + o_xg);
-    s.xg = f, s.yg = f + H, s.zg = f + 2 * H, s.rg = f + 3 * H, s.xerr = f + 4 * H, s.yerr = f + 5 * H;
-    int16_t *i16 = (int16_t *)(b + o_i16);
-    s.iphi = i16, s.ysize = i16 + H, s.det = i16 + 2 * H;
-    s.hist_off = (uint32_t *)(b + o_ho);
-    s.hist_bins = (uint16_t *)(b + o_hb);
     s.quality = (int32_t *)(b + o_q);
     float *g = (float *)(b + o_f);
     s.chi2 = g, s.eta = g + T, s.pt = g + 2 * T;
@@ -649,30 +682,40 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     HIP_OK(hipSetDevice(c->device));
     if (hits->n_hits > PT_MAX_HITS)
         return fail(PT_ERR_CAPACITY, "%u hits above MAX_NUMBER_OF_HITS = %u", hits->n_hits, PT_MAX_HITS);
+    if (hits->n_hits && (!hits->xg || !hits->yg || !hits->zg || !hits->rg || !hits->xerr || !hits->yerr ||
+                         !hits->iphi || !hits->ysize || !hits->det_ind))
+        return fail(PT_ERR_ARG, "null hit array");
+    if (!hits->hist_off != !hits->hist_bins)
+        return fail(PT_ERR_ARG, "give both histogram arrays, or neither to build it on the device");
+    if (!out->quality || !out->chi2 || !out->eta || !out->pt || !out->state || !out->covariance || !out->hit_off ||
+        !out->hit_bins || !out->det_off || !out->det_bins)
+        return fail(PT_ERR_ARG, "null TrackSOA array");
     int rc = ensure_staging(c);
     if (rc) return rc;
     Staging &s = c->stg;
     const size_t n = hits->n_hits;
     hipStream_t st = c->stream;
-    if (n) {
-        HIP_OK(hipMemcpyAsync(s.xg, hits->xg, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.yg, hits->yg, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.zg, hits->zg, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.rg, hits->rg, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.xerr, hits->xerr, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.yerr, hits->yerr, n * 4, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.iphi, hits->iphi, n * 2, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.ysize, hits->ysize, n * 2, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(s.det, hits->det_ind, n * 2, hipMemcpyHostToDevice, st));
-        if (hits->hist_bins) HIP_OK(hipMemcpyAsync(s.hist_bins, hits->hist_bins, n * 2, hipMemcpyHostToDevice, st));
+    // inputs: packed into the pinned block, one copy
+    const PackedInputs L(n);
+    {
+        const float *fsrc[6] = {hits->xg, hits->yg, hits->zg, hits->rg, hits->xerr, hits->yerr};
+        const int16_t *isrc[3] = {hits->iphi, hits->ysize, hits->det_ind};
+        for (int k = 0; k < 6; ++k)
+            if (n) std::memcpy(s.h_in + L.f[k], fsrc[k], 4 * n);
+        for (int k = 0; k < 3; ++k)
+            if (n) std::memcpy(s.h_in + L.i16[k], isrc[k], 2 * n);
+        if (hits->hist_off) std::memcpy(s.h_in + L.hist_off, hits->hist_off, 4 * PT_HIST_NOFF);
+        if (hits->hist_bins && n) std::memcpy(s.h_in + L.hist_bins, hits->hist_bins, 2 * n);
+        HIP_OK(hipMemcpyAsync(s.in, s.h_in, L.total, hipMemcpyHostToDevice, st));
     }
-    if (hits->hist_off)
-        HIP_OK(hipMemcpyAsync(s.hist_off, hits->hist_off, PT_HIST_NOFF * 4, hipMemcpyHostToDevice, st));
     pt_hits_view dv = *hits;
-    dv.xg = s.xg, dv.yg = s.yg, dv.zg = s.zg, dv.rg = s.rg, dv.xerr = s.xerr, dv.yerr = s.yerr;
-    dv.iphi = s.iphi, dv.ysize = s.ysize, dv.det_ind = s.det;
-    dv.hist_off = hits->hist_off ? s.hist_off : nullptr;  // none given: built on the device
-    dv.hist_bins = hits->hist_bins ? s.hist_bins : nullptr;
+    dv.xg = (const float *)(s.in + L.f[0]), dv.yg = (const float *)(s.in + L.f[1]);
+    dv.zg = (const float *)(s.in + L.f[2]), dv.rg = (const float *)(s.in + L.f[3]);
+    dv.xerr = (const float *)(s.in + L.f[4]), dv.yerr = (const float *)(s.in + L.f[5]);
+    dv.iphi = (const int16_t *)(s.in + L.i16[0]), dv.ysize = (const int16_t *)(s.in + L.i16[1]);
+    dv.det_ind = (const int16_t *)(s.in + L.i16[2]);
+    dv.hist_off = hits->hist_off ? (const uint32_t *)(s.in + L.hist_off) : nullptr;  // none given: built on the device
+    dv.hist_bins = hits->hist_bins ? (const uint16_t *)(s.in + L.hist_bins) : nullptr;
     pt_track_soa dout;
     dout.quality = s.quality, dout.chi2 = s.chi2, dout.eta = s.eta, dout.pt = s.pt, dout.state = s.state;
     dout.covariance = s.cov, dout.hit_off = s.hit_off, dout.hit_bins = s.hit_bins, dout.det_off = s.det_off;
@@ -688,33 +731,50 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     const size_t T = PT_MAX_TUPLES;
     const size_t m = std::min<size_t>(k.n_tuples, T);
     const size_t nb = std::min<size_t>(k.n_tuple_hits, PT_MAX_TUPLE_HITS);
-    if (m) {
-        HIP_OK(hipMemcpyAsync(out->quality, s.quality, m * 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(out->chi2, s.chi2, m * 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(out->eta, s.eta, m * 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(out->pt, s.pt, m * 4, hipMemcpyDeviceToHost, st));
-    }
-    // state and covariance: only rows of stored tuples can be non-zero, so the
-    // first m rows of each component come back and the rest is zero on the host
-    if (m) {
-        HIP_OK(hipMemcpy2DAsync(out->state, T * 4, s.state, T * 4, m * 4, 5, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpy2DAsync(out->covariance, T * 4, s.cov, T * 4, m * 4, 15, hipMemcpyDeviceToHost, st));
-    }
     // offsets: past the stored tuples they all equal off[m] (bulk_finalize_fill),
     // unless the container is full; the copy stops there and the host fills the rest
     const size_t n_off = (k.n_tuples < T) ? m + 1 : T + 1;
-    HIP_OK(hipMemcpyAsync(out->hit_off, s.hit_off, n_off * 4, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(out->det_off, s.det_off, n_off * 4, hipMemcpyDeviceToHost, st));
-    if (nb) {
-        HIP_OK(hipMemcpyAsync(out->hit_bins, s.hit_bins, nb * 2, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(out->det_bins, s.det_bins, nb * 2, hipMemcpyDeviceToHost, st));
-    }
-    if (!out_zeroed)
-        for (int comp = 0; comp < 20; ++comp) {
-            float *col = (comp < 5 ? out->state + comp * T : out->covariance + (comp - 5) * T);
-            std::memset(col + m, 0, (T - m) * 4);
+    // the defined rows into the pinned block (state and covariance: only rows of
+    // stored tuples can be non-zero), then into the caller's arrays
+    size_t po = 0;
+    hipError_t copy_err = hipSuccess;
+    auto grab = [&](const void *src, size_t bytes) -> char * {
+        char *dst = s.h_out + po;
+        po += (bytes + 15) & ~size_t(15);
+        if (bytes) {
+            const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) copy_err = e;
         }
+        return dst;
+    };
+    char *p_q = grab(s.quality, m * 4), *p_chi2 = grab(s.chi2, m * 4), *p_eta = grab(s.eta, m * 4);
+    char *p_pt = grab(s.pt, m * 4);
+    char *p_st = s.h_out + po;
+    po += ((5 * m * 4) + 15) & ~size_t(15);
+    char *p_cv = s.h_out + po;
+    po += ((15 * m * 4) + 15) & ~size_t(15);
+    if (m) {
+        HIP_OK(hipMemcpy2DAsync(p_st, m * 4, s.state, T * 4, m * 4, 5, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpy2DAsync(p_cv, m * 4, s.cov, T * 4, m * 4, 15, hipMemcpyDeviceToHost, st));
+    }
+    char *p_ho = grab(s.hit_off, n_off * 4), *p_do = grab(s.det_off, n_off * 4);
+    char *p_hb = grab(s.hit_bins, nb * 2), *p_db = grab(s.det_bins, nb * 2);
+    HIP_OK(copy_err);
     HIP_OK(hipStreamSynchronize(st));
+    std::memcpy(out->quality, p_q, m * 4);
+    std::memcpy(out->chi2, p_chi2, m * 4);
+    std::memcpy(out->eta, p_eta, m * 4);
+    std::memcpy(out->pt, p_pt, m * 4);
+    for (int comp = 0; comp < 20; ++comp) {
+        float *col = (comp < 5 ? out->state + comp * T : out->covariance + (comp - 5) * T);
+        const char *src = comp < 5 ? p_st + comp * m * 4 : p_cv + (comp - 5) * m * 4;
+        std::memcpy(col, src, m * 4);
+        if (!out_zeroed) std::memset(col + m, 0, (T - m) * 4);
+    }
+    std::memcpy(out->hit_off, p_ho, n_off * 4);
+    std::memcpy(out->det_off, p_do, n_off * 4);
+    std::memcpy(out->hit_bins, p_hb, nb * 2);
+    std::memcpy(out->det_bins, p_db, nb * 2);
     for (size_t i = n_off; i <= T; ++i) {
         out->hit_off[i] = out->hit_off[m];
         out->det_off[i] = out->det_off[m];
@@ -848,7 +908,7 @@ int pt_pipeline_init(pt_ctx *c, uint32_t group_size, uint32_t n_groups, double b
         const uint32_t e0 = g * group_size;
         // grids sized for the capacity limits, so any event of the stream fits the captured shape
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-        pt::launch_phi_hist(c->d_evs + e0, group_size, st);  // returns at once for events with a histogram
+        pt::launch_phi_hist(c->d_evs + e0, group_size, c->xcd, st);  // returns at once for events with a histogram
         pt::launch_ca(c->d_evs + e0, group_size, cfg, PT_MAX_HITS, pt::kMaxSlots, out_zeroed ? 0 : 1, st, nullptr);
         pt::launch_fit_tuples(c->d_evs + e0, group_size, cfg, st, nullptr);
         pt::launch_classify(c->d_evs + e0, group_size, cfg, st, nullptr);

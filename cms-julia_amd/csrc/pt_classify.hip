@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(1024) k_classify_event(const EvDesc *__restric
 
 void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm) {
     (void)cfg;
-    const EvGrid one{n_events, 1u};
+    const EvGrid one{n_events, 1u, cfg.xcd};
     k_classify_event<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
     if (tm) tm->record("k_classify_event", st);
 }

@@ -159,6 +159,7 @@ struct EvDesc {
 
 struct Config {
     pt_params p;
+    uint32_t xcd;             // XCD of the batch's first event (EvGrid::xcd)
     double B;                 // Float64(Float32(b_field))
     PT_G const float *frames;      // [1856*9]
     PairTables pairs;
@@ -263,15 +264,18 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
 // event e gets the blocks L = e % 8 + 8 * (per_event * (e / 8) + b),
 // b < per_event, all on one XCD, and its working set (hit records, cells,
 // lists) stays in that XCD's L2 instead of being pulled into all eight.
+// xcd rotates the events over the XCDs: event e runs on XCD (e + xcd) % 8, so
+// contexts that each run one event at a time spread over the chip.
 struct EvGrid {
     uint32_t n_events, per_event;
+    uint32_t xcd = 0;
 };
 
 __device__ __forceinline__ bool ev_block(const EvGrid &g, uint32_t &ev, uint32_t &b) {
     const uint32_t L = blockIdx.x;
     const uint32_t s = L >> 3;
     const uint32_t q = s / g.per_event;
-    ev = (L & 7u) + 8u * q;
+    ev = ((L - g.xcd) & 7u) + 8u * q;
     b = s - q * g.per_event;
     return ev < g.n_events;
 }

@@ -385,7 +385,7 @@ void launch_fit_tuples(const EvDesc *d_evs, uint32_t n_events, const Config &cfg
     const uint32_t target = (2048u + n_events - 1) / n_events / 3u;
     if (per > target) per = target;
     if (per < 1) per = 1;
-    const EvGrid grid{n_events, 3u * per};
+    const EvGrid grid{n_events, 3u * per, cfg.xcd};
     if (cfg.fit_precision == PT_FIT_MIXED)
         k_fit_tuples<PolicyMixed><<<ev_grid_dim(grid), 256, 0, st>>>(d_evs, grid, cfg);
     else

@@ -60,11 +60,15 @@ __device__ __forceinline__ uint64_t bin_match(uint32_t b, bool valid) {
 }
 }  // namespace
 
-__global__ void __launch_bounds__(1024) k_phi_hist(const EvDesc *__restrict__ evs, uint32_t n_events) {
+__global__ void __launch_bounds__(1024) k_phi_hist(const EvDesc *__restrict__ evs, uint32_t n_events,
+                                                    uint32_t xcd) {
     __shared__ uint16_t cnt[kHistWaves][kHistBins];
     __shared__ uint32_t off[kHistBins];
     __shared__ uint32_t scan_lds[1024 / 64 + 1];
-    const uint32_t ev = blockIdx.x;
+    // block xcd + e takes event e, so the event's histogram is built on the
+    // XCD its stage kernels run on
+    if (blockIdx.x < xcd) return;
+    const uint32_t ev = blockIdx.x - xcd;
     if (ev >= n_events) return;
     const EvDesc &e = evs[ev];
     if (!e.hist_bins_build) return;
@@ -127,8 +131,8 @@ __global__ void __launch_bounds__(1024) k_phi_hist(const EvDesc *__restrict__ ev
     }
 }
 
-void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, hipStream_t st) {
-    if (n_events) k_phi_hist<<<n_events, 1024, 0, st>>>(d_evs, n_events);
+void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, uint32_t xcd, hipStream_t st) {
+    if (n_events) k_phi_hist<<<n_events + xcd, 1024, 0, st>>>(d_evs, n_events, xcd);
 }
 
 }  // namespace pt
