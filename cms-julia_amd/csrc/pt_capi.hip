@@ -268,7 +268,9 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         prm = *p;
     else
         pt_params_default(&prm);
-    if (prm.use_riemann_fit) return fail(PT_ERR_UNSUPPORTED, "the Riemann fit is not wired in the reference");
+    // use_riemann_fit is accepted and, as in the reference, has no effect: its
+    // make_tuples never reads it and always runs the broken-line fit
+    // (ca_hit_ntuplet_generator.jl:83-93)
     if (!prm.fit_5_as_4) return fail(PT_ERR_UNSUPPORTED, "fit_5_as_4 = false throws in the reference");
     // late fishbone: fish_bone(..., check_track = true) calls empty() on the
     // UInt32 column id of a cell (gpu_fishbone.jl:29), a MethodError

@@ -96,7 +96,8 @@ extern "C" {
 typedef struct pt_params {
     uint32_t min_hits_per_ntuplet;             /* 3      */
     uint32_t max_num_of_doublets;              /* 458752 */
-    uint8_t  use_riemann_fit;                  /* 0 (1 is unsupported: the reference only prints) */
+    uint8_t  use_riemann_fit;                  /* 0 (no effect, as in the reference: make_tuples
+                                                  always runs the broken-line fit)                 */
     uint8_t  fit_5_as_4;                       /* 1 (0 is unsupported: the reference throws)       */
     uint8_t  include_jumping_forward_doublets; /* 1      */
     uint8_t  early_fish_bone;                  /* 1      */

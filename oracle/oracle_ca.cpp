@@ -715,9 +715,10 @@ int oracle_make_tuples(const pt_params *P, const pt_hits_view *hv, const float *
     // switches on which the reference throws before producing tracks:
     // late_fish_bone (empty() on a UInt32, gpu_fishbone.jl:29), do_stats
     // (kernel_mark_used is not defined, ca_hit_ntuplet_generator_kernels.jl:182-184),
-    // fit_5_as_4 = false (broken_line_fit_on_gpu.jl:190-199), use_riemann_fit
-    // (launch_riemann_kernels! only prints, helix_fit_on_gpu.jl:94-96)
-    if (P->late_fish_bone || P->do_stats || !P->fit_5_as_4 || P->use_riemann_fit) return PT_ERR_UNSUPPORTED;
+    // fit_5_as_4 = false (log_file is not defined, broken_line_fit_on_gpu.jl:195-200).
+    // use_riemann_fit is never read: make_tuples runs the broken-line fit
+    // whatever its value (ca_hit_ntuplet_generator.jl:83-93)
+    if (P->late_fish_bone || P->do_stats || !P->fit_5_as_4) return PT_ERR_UNSUPPORTED;
     // per-thread scratch reused across events, like CACHED_KERNELS (a TaskLocalValue,
     // ca_hit_ntuplet_generator.jl:9) reset by resetCAHitNTupletGeneratorKernels
     thread_local State st_tls;

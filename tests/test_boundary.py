@@ -7,8 +7,10 @@ Switches on which the reference throws instead of producing tracks:
                    UInt32 column id of a cell (gpu_fishbone.jl:29)
   do_stats         kernel_mark_used((...)) is not a defined function and is given
                    one tuple argument (ca_hit_ntuplet_generator_kernels.jl:182-184)
-  fit_5_as_4=false the 5-hit fit path (broken_line_fit_on_gpu.jl:190-199)
-  use_riemann_fit  launch_riemann_kernels! only prints (helix_fit_on_gpu.jl:94-96)
+  fit_5_as_4=false the 5-hit fit path names an undefined log_file
+                   (broken_line_fit_on_gpu.jl:195-200)
+use_riemann_fit is never read by the reference (make_tuples always runs the
+broken-line fit, ca_hit_ntuplet_generator.jl:83-93): accepted, no effect.
 """
 import ctypes as C
 import re
@@ -46,7 +48,7 @@ def test_python_mirror_lists_every_declared_function():
     assert set(declared("pixeltrack_gen.h")) <= set(A.GEN_SYMBOLS)
 
 
-THROWING = [dict(late_fish_bone=1), dict(do_stats=1), dict(fit_5_as_4=0), dict(use_riemann_fit=1)]
+THROWING = [dict(late_fish_bone=1), dict(do_stats=1), dict(fit_5_as_4=0)]
 
 
 @pytest.mark.parametrize("kw", THROWING, ids=lambda kw: next(iter(kw)))
@@ -83,3 +85,20 @@ def test_oracle_rejects_switches_the_reference_throws_on(frames, kw):
 def test_default_parameters_agree_with_the_oracle():
     a, b = ca.default_params(), O.params()
     assert bytes(a) == bytes(b)
+
+
+def test_riemann_switch_is_accepted_and_has_no_effect(frames):
+    """pt_ctx_create passes use_riemann_fit = 1 through validation (here it
+    then fails only for want of a device), and the oracle's output with it is
+    the default output."""
+    h = C.c_void_p()
+    rc = A.hip_lib().pt_ctx_create(C.byref(h), C.byref(ca.default_params(use_riemann_fit=1)), 0, 1)
+    assert rc != A.PT_ERR_UNSUPPORTED and rc != A.PT_ERR_ARG, rc
+    if h.value:
+        A.hip_lib().pt_ctx_destroy(h)
+    hits = G.generate_event("single_event", seed=1)
+    a, ca_, _ = O.make_tuples(hits, frames, debug=False)
+    b, cb, _ = O.make_tuples(hits, frames, debug=False, p=O.params(use_riemann_fit=1))
+    assert ca_ == cb
+    for k in ("quality", "state", "covariance", "hit_off", "hit_bins"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k

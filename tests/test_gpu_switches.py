@@ -10,6 +10,8 @@ fit at the stage tolerance:
   include_jumping_forward_doublets = 0   15 layer pairs (:152-156)
   do_cluster_cut / do_z0_cut / do_pt_cut = 0   (gpu_pixel_doublets.jl:229-245,326,338)
   ideal_conditions = 0             the outer-ladder test of the cluster cut (:233)
+  use_riemann_fit = 1              never read: the broken-line fit still runs
+                                   (ca_hit_ntuplet_generator.jl:83-93)
 
 List columns: the reference keeps at most 65,535 neighbour lists
 (CellNeighborsVector(64, 65536) with column 1 the empty sentinel); extend!
@@ -30,7 +32,7 @@ pytestmark = pytest.mark.gpu
 
 SWITCHES = [dict(early_fish_bone=0), dict(min_hits_per_ntuplet=4), dict(min_hits_per_ntuplet=5),
             dict(include_jumping_forward_doublets=0), dict(do_cluster_cut=0), dict(do_z0_cut=0), dict(do_pt_cut=0),
-            dict(ideal_conditions=0)]
+            dict(ideal_conditions=0), dict(use_riemann_fit=1)]
 EVENTS = [("single_event", 1), ("pu50", 1)]
 
 
