@@ -1477,6 +1477,195 @@ __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs
     wave_add_atomic(&e.cnt->n_killed, killed);
 }
 
+// ------------------------------------------------------------------ neighbour-list columns
+// The reference keeps neighbour lists in CellNeighborsVector(64, 65536),
+// column 1 the empty sentinel, and hands a column to a cell at its first
+// connection (add_outer_neighbor, gpu_ca_cell.jl:164-177).  extend! returns
+// the last column again once all are taken (simple_matrix.jl:13-19): the
+// 65,535th cell to get a list and every later one share column 65,536, which
+// holds the first 64 of their connections in connect's order.  Connect walks
+// the `this` cells in id order and each outer-hit list in ascending cell
+// order, so a cell o asks for its column at key (first this, o) and the
+// connections (this, o) reach the shared column in (this, o) order.  Here the
+// lists are kept per cell; when more than 65,535 cells have one, this kernel
+// (one block per event, after the lists are complete) finds the cells past the
+// 65,534th key and gives them all the shared list.
+__device__ __forceinline__ uint32_t comp4_(uint4 v, uint32_t i) {
+    return (i == 0) ? v.x : (i == 1) ? v.y : (i == 2) ? v.z : v.w;
+}
+constexpr uint32_t kOwnCols = PT_MAX_ACTIVE_LISTS - 2u;  // columns 2..65535 stay private
+constexpr uint32_t kShareCand = 4096;                    // candidate connections sorted in LDS
+constexpr uint32_t kShareHist = 8192;
+
+__device__ __forceinline__ uint64_t owner_key(const EvDesc &e, uint32_t c) {
+    return ((uint64_t)e.s.nbr4[c].x << 19) | c;
+}
+__device__ __forceinline__ uint32_t nbr_of(const EvDesc &e, uint32_t c, uint32_t i) {
+    return (i < 4u) ? comp4_(e.s.nbr4[c], i) : e.s.nbr_pool[e.s.nbr_ext[c] + i - 4u];
+}
+
+__global__ void __launch_bounds__(1024) k_nbr_share(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ union {
+        uint32_t hist[kShareHist];
+        uint64_t cand[kShareCand];
+    } L;
+    __shared__ uint32_t scan_lds[1024 / 64 + 1];
+    __shared__ uint32_t n_cand, sum_nn;
+    __shared__ uint64_t sel;
+    uint32_t ev, blk;
+    if (!ev_block(grid, ev, blk)) return;
+    const EvDesc &e = evs[ev];
+    if (e.s.misc[kMiscNbrLists] <= kOwnCols + 1u) return;  // every list keeps its own column
+    const uint32_t n_cells = e.cnt->n_cells, tid = threadIdx.x;
+    // 1. the key of rank kOwnCols among the owners' keys (38 bits), by radix
+    //    selection over 13 + 13 + 12 bits
+    uint64_t prefix = 0;
+    uint32_t rank = kOwnCols;
+    const int bits[3] = {12, 13, 13};
+    int shift = 38;
+    for (int pass = 0; pass < 3; ++pass) {
+        const int nb = bits[pass];
+        shift -= nb;
+        for (uint32_t i = tid; i < kShareHist; i += blockDim.x) L.hist[i] = 0u;
+        __syncthreads();
+        for (uint32_t c = tid; c < n_cells; c += blockDim.x) {
+            if (!e.s.cell_nn[2u * c]) continue;
+            const uint64_t k = owner_key(e, c);
+            if ((k >> (shift + nb)) == prefix) atomicAdd(&L.hist[(uint32_t)(k >> shift) & ((1u << nb) - 1u)], 1u);
+        }
+        __syncthreads();
+        // the bucket holding `rank`: 8 bins per thread, block scan of the sums
+        uint32_t loc = 0;
+        for (uint32_t j = 0; j < 8u; ++j) loc += L.hist[8u * tid + j];
+        uint32_t total;
+        const uint32_t before = block_excl_scan(loc, scan_lds, total);
+        if (rank >= before && rank < before + loc) {
+            uint32_t run = before;
+            for (uint32_t j = 0; j < 8u; ++j) {
+                const uint32_t h = L.hist[8u * tid + j];
+                if (rank < run + h) {
+                    sel = ((uint64_t)(8u * tid + j));
+                    n_cand = rank - run;
+                    break;
+                }
+                run += h;
+            }
+        }
+        __syncthreads();
+        prefix = (prefix << nb) | sel;
+        rank = n_cand;
+        __syncthreads();
+    }
+    const uint64_t kstar = prefix;
+    // 2. the shared set S = owners with key >= kstar; the 64th smallest first
+    //    connection among S bounds the connections that can make the column
+    if (tid == 0) {
+        n_cand = 0u;
+        sum_nn = 0u;
+    }
+    for (uint32_t i = tid; i < kShareHist; i += blockDim.x) L.hist[i] = 0u;
+    __syncthreads();
+    // histogram of the first connection's top 13 bits (of 19) over S
+    uint32_t my_nn = 0;
+    for (uint32_t c = tid; c < n_cells; c += blockDim.x) {
+        const uint32_t nn = e.s.cell_nn[2u * c];
+        if (!nn || owner_key(e, c) < kstar) continue;
+        my_nn += nn;
+        atomicAdd(&L.hist[e.s.nbr4[c].x >> 6], 1u);
+    }
+    atomicAdd(&sum_nn, my_nn);
+    __syncthreads();
+    uint32_t loc = 0;
+    for (uint32_t j = 0; j < 8u; ++j) loc += L.hist[8u * tid + j];
+    uint32_t total;
+    const uint32_t before = block_excl_scan(loc, scan_lds, total);
+    const uint32_t want = min(total, PT_MAX_NEIGHBORS) - 1u;  // rank of the 64th (or last) first connection
+    if (want >= before && want < before + loc) {
+        uint32_t run = before;
+        for (uint32_t j = 0; j < 8u; ++j) {
+            const uint32_t h = L.hist[8u * tid + j];
+            if (want < run + h) {
+                sel = ((uint64_t)(8u * tid + j + 1u)) << 6;  // connections below this `this` bound qualify
+                break;
+            }
+            run += h;
+        }
+    }
+    __syncthreads();
+    // fewer than 64 cells share: all their connections can make the column
+    const uint32_t tbound = (total < PT_MAX_NEIGHBORS) ? ~0u : (uint32_t)sel;
+    __syncthreads();
+    // 3. candidates: every connection (this, o) of S with this < tbound
+    bool full = false;
+    for (uint32_t c = tid; c < n_cells; c += blockDim.x) {
+        const uint32_t nn = e.s.cell_nn[2u * c];
+        if (!nn || owner_key(e, c) < kstar) continue;
+        for (uint32_t i = 0; i < nn; ++i) {
+            const uint32_t t = nbr_of(e, c, i);
+            if (t >= tbound) break;  // ascending
+            const uint32_t slot = atomicAdd(&n_cand, 1u);
+            if (slot < kShareCand) L.cand[slot] = ((uint64_t)t << 19) | c;
+            else full = true;
+        }
+    }
+    __syncthreads();
+    const uint32_t nc = min(n_cand, kShareCand);
+    const bool overflow_cand = __syncthreads_or(full);
+    if (overflow_cand) {  // more candidates than the sort holds: not reproduced
+        if (tid == 0) atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
+        return;
+    }
+    // 4. sort the candidates (bitonic, padded with ~0) and keep the first 64
+    uint32_t np = 1;
+    while (np < nc) np <<= 1;
+    for (uint32_t i = nc + tid; i < np; i += blockDim.x) L.cand[i] = ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= np; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < np; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = L.cand[i], b = L.cand[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        L.cand[i] = b;
+                        L.cand[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    const uint32_t len = min(nc, PT_MAX_NEIGHBORS);
+    // 5. the shared list: entries 5..len in the pool, the first four inline,
+    //    the same for every cell of S
+    __shared__ uint32_t pool_base;
+    if (tid == 0) {
+        pool_base = (len > 4u) ? atomicAdd(&e.s.misc[kMiscPool], len - 4u) : 0u;
+        uint32_t ovf = 0, err = 0;
+        if (len > 4u && pool_base + (len - 4u) > kNbrPool) err |= PT_INT_NBR_POOL;
+        if (sum_nn > PT_MAX_NEIGHBORS) ovf |= PT_OVF_NEIGHBORS;  // pushes dropped from the shared column
+        if (ovf) atomicOr(&e.cnt->overflow, ovf);
+        if (err) atomicOr(&e.cnt->internal_error, err);
+        // each list counted once: the shared column replaces the lists of S
+        atomicSub(&e.cnt->n_connections, sum_nn - len);
+    }
+    __syncthreads();
+    if (len > 4u && pool_base + (len - 4u) > kNbrPool) return;
+    for (uint32_t i = 4u + tid; i < len; i += blockDim.x) e.s.nbr_pool[pool_base + i - 4u] = (uint32_t)(L.cand[i] >> 19);
+    uint4 first4 = make_uint4(~0u, ~0u, ~0u, ~0u);
+    if (len > 0) first4.x = (uint32_t)(L.cand[0] >> 19);
+    if (len > 1) first4.y = (uint32_t)(L.cand[1] >> 19);
+    if (len > 2) first4.z = (uint32_t)(L.cand[2] >> 19);
+    if (len > 3) first4.w = (uint32_t)(L.cand[3] >> 19);
+    for (uint32_t c = tid; c < n_cells; c += blockDim.x) {
+        if (!e.s.cell_nn[2u * c] || owner_key(e, c) < kstar) continue;
+        // (reads of nbr4 for the key above happen before this write for c only)
+        e.s.nbr_ext[c] = pool_base;
+        e.s.cell_nn[2u * c] = (uint8_t)len;
+        e.s.nbr4[c] = first4;
+    }
+}
+
 // ------------------------------------------------------------------ ntuplets (gpu_ca_cell.jl:232-270)
 // DFS with the serial visiting order, one template level per depth (the
 // reference asserts at most 4 doublets per chain, gpu_ca_cell.jl:234), so the
@@ -1537,6 +1726,9 @@ __device__ __forceinline__ void dfs_level(const EvDesc &e, const DNode &node, ui
                 const DNode c = (i < 4u) ? pick4(i, c0, c1, c2, c3) : load_node(e, e.s.nbr_pool[ext + i - 4u]);
                 if (c.killed) continue;
                 child = true;
+                // @assert(inner hit of the neighbour != own inner hit) (gpu_ca_cell.jl:247):
+                // only a shared neighbour column can break it
+                if (((c.io ^ node.io) & 0xFFFFu) == 0u) ref_bits |= PT_REF_DEPTH;
                 dfs_level<D + 1>(e, c, cells, ios, min_doublets, ref_bits, leaf);
             }
         } else {
@@ -1924,13 +2116,11 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
             uint32_t o = 0;
             if (e.s.misc[kMiscNbrLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_NEIGHBOR_LISTS;
             if (e.s.misc[kMiscTrkLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_TRACK_LISTS;
-            if (o) {
-                atomicOr(&e.cnt->overflow, o);
-                // the reference's later cells then share its last list column
-                // (extend!, simple_matrix.jl:13-19); these lists are kept per
-                // cell here, so the event's results are not the reference's
-                atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
-            }
+            if (o) atomicOr(&e.cnt->overflow, o);
+            // tuple lists: the reference's later cells would share its last
+            // column (extend!, simple_matrix.jl:13-19); kept per cell here, so
+            // the event's results are not the reference's
+            if (o & PT_OVF_TRACK_LISTS) atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
         }
         __syncthreads();
     }
@@ -2073,6 +2263,8 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
         k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
     }
     mark("k_fishbone");
+    k_nbr_share<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
+    mark("k_nbr_share");
     const EvGrid g_start = ev_grid(est_cells / 8u, n_events, xcd);
     k_dfs_count<<<ev_grid_dim(g_start), 256, 0, st>>>(d_evs, g_start, cfg);
     mark("k_dfs_count");

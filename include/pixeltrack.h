@@ -81,9 +81,11 @@ extern "C" {
 #define PT_INT_NBR_POOL        0x002u /* neighbour pool (lists longer than 4) full   */
 #define PT_INT_OVF_LIST        0x004u /* internal overflow list full                 */
 #define PT_INT_ROUNDS          0x008u /* duplicate-remover rounds exceeded the shared cells */
-#define PT_INT_LIST_SHARE      0x010u /* > 65535 cells need neighbour or tuple lists: the reference's
-                                         extend! then hands the later cells its last column, shared
-                                         (simple_matrix.jl:13-19); lists stay per cell here */
+#define PT_INT_LIST_SHARE      0x010u /* > 65535 cells on tuples: the reference's extend! would hand
+                                         the later cells its last tuple-list column, shared
+                                         (simple_matrix.jl:13-19), which is not reproduced (the shared
+                                         neighbour column is); or the shared neighbour column's
+                                         candidates exceed this library's sort buffer */
 
 /* Quality enum (track_quality.jl:129-136) */
 #define PT_QUALITY_BAD         0

@@ -405,3 +405,25 @@ def test_fishbone_kills_the_farther_collinear_doublet(frames):
         assert same, "a killed doublet must share its outer hit with a surviving one"
         ri = lambda x: float(h.rg[c["inner"][x] - 1])
         assert any(ri(k) < ri(s) for s in same)  # the killed one reaches further in (longer)
+
+
+def test_neighbour_column_exhaustion_shares_the_last_column(frames):
+    """extend! (simple_matrix.jl:13-19) past 65,535 lists: the cells that asked
+    last - all but the first 65,534 owners - hold one and the same list, at
+    most 64 long (push! drops the rest), and the overflow bits say so."""
+    import crafted as K
+    h = K.make_event(K.bundles(80, (0.0, 0.3, -0.3), 10))
+    out, cnt, dbg = O.make_tuples(h, frames, stop_after=2)
+    lists = dbg.lists("nbr")
+    owners = [l for l in lists if len(l)]
+    assert len(owners) > 65535
+    distinct = {}
+    for l in owners:
+        distinct[tuple(l.tolist())] = distinct.get(tuple(l.tolist()), 0) + 1
+    # (cells that share an inner hit can hold equal lists of their own; the
+    # shared column is the one list held by at least every late cell)
+    late = len(owners) - 65534
+    shared = [k for k, v in distinct.items() if v >= late]
+    assert len(shared) == 1, [distinct[k] for k in shared]
+    assert len(shared[0]) == 64 and list(shared[0]) == sorted(shared[0])
+    assert cnt["overflow"] & 0x8 and cnt["overflow"] & 0x4
