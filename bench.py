@@ -98,10 +98,6 @@ def kernel_model_bytes(name, c):
         # per neighbour entry 4 B + neighbour flags 1 B; per stored tuple: offsets 4 B,
         # hits 2 B each, cells 16 B, quality 4 B (emit)
         b = 5 * C + 4 * D // 10 + ((4 + 16 + 4) * T + 2 * Th if name == "k_dfs_emit" else 0)
-    elif name == "k_classify_event":
-        # per stored tuple: its cells 16 B, quality read and written 8 B; per tuple hit:
-        # the hit id 2 B (shared-cell lists and per-hit reductions come on top)
-        b = 24 * T + 2 * Th
     elif name == "k_fit_tuples":
         # per fit (mostly triplets at PU50): hits and module frames gathered, outputs
         # written, about 24 N + 98 B with N = 3

@@ -1288,28 +1288,24 @@ __global__ void __launch_bounds__(256) k_connect_ext(const EvDesc *__restrict__ 
 
 // The fishbone's work list: the hits with two or more marks from connect (a
 // hit with fewer has at most one used cell ending on it, so nothing to
-// compare), in hit order; one block per event, 16 hits per thread per round.
+// compare).  The fishbone decides every hit on its own (a cell is on one
+// hit's list), so the list is in no particular order: several blocks per
+// event, one hit per thread per round, one atomic per wavefront to reserve
+// its entries.
 __global__ void __launch_bounds__(256) k_fish_list(const EvDesc *__restrict__ evs, EvGrid grid) {
-    __shared__ uint32_t lsum[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     const uint32_t n = e.n_hits;
-    uint32_t base = 0;
-    for (uint32_t h0 = 0; h0 < n; h0 += 256u * 16u) {
-        const uint32_t hb = h0 + 16u * threadIdx.x;
-        uint32_t m = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 16u; ++j)
-            if (hb + j < n && e.s.fish_cnt[hb + j] >= 2u) m |= 1u << j;
-        uint32_t tot;
-        uint32_t pos = base + block_excl_scan4((uint32_t)__popc(m), lsum, tot);
-#pragma unroll
-        for (uint32_t j = 0; j < 16u; ++j)
-            if ((m >> j) & 1u) e.s.fish_hits[pos++] = hb + j;
-        base += tot;
+    for (uint32_t h0 = b * blockDim.x; h0 < n; h0 += grid.per_event * blockDim.x) {
+        const uint32_t h = h0 + threadIdx.x;
+        const bool take = h < n && e.s.fish_cnt[h] >= 2u;
+        const uint64_t m = __ballot(take);
+        uint32_t pos = 0;
+        if (lane_id() == 0 && m) pos = atomicAdd(&e.s.misc[kMiscFish], (uint32_t)__popcll(m));
+        pos = __shfl(pos, 0, 64) + (uint32_t)__popcll(m & lanes_below());
+        if (take) e.s.fish_hits[pos] = h;
     }
-    if (threadIdx.x == 0) e.s.misc[kMiscFish] = base;
 }
 
 // ------------------------------------------------------------------ fishbone (gpu_fishbone.jl:4-62)
@@ -2090,9 +2086,13 @@ __device__ void trk_exact(const EvDesc &e, uint32_t cell, uint32_t mc) {
 // kernel_countMultiplicity / finalize! / kernel_fillMultiplicity
 // (ca_hit_ntuplet_generator_kernels_impl.jl:174-211): bucket = hit count, each
 // bucket lists tuple ids in DESCENDING order (fill_direct decrements).
+constexpr uint32_t kMulU = 8, kMulChunk = 1024u * kMulU;  // shared cells per chunk of the list offsets
+constexpr uint8_t kMulNone = 5, kMulLong = 6;
 __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint32_t lds[5][1024 / 64 + 1];
     __shared__ uint32_t tot[5];
+    __shared__ uint32_t s_dcn[kMulChunk];
+    __shared__ uint8_t s_code[kMaxTuples];
     uint32_t ev, blk;
     if (!ev_block(grid, ev, blk)) return;
     const EvDesc &e = evs[ev];
@@ -2125,33 +2125,78 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
         __syncthreads();
     }
     {
-        // offsets of the shared cells' tuple lists (classification), in slot order
+        // offsets of the shared cells' tuple lists (classification), in slot
+        // order, kMulChunk cells at a time: their tuple counts are gathered with
+        // coalesced loads into LDS, then each thread scans a contiguous run
+        uint32_t base = 0;
         const uint32_t nd = min(e.s.misc[kMiscDc], kDcCap);
-        const uint32_t dch = (nd + blockDim.x - 1) / blockDim.x;
-        const uint32_t d0 = min(nd, threadIdx.x * dch), d1 = min(nd, d0 + dch);
-        uint32_t sum = 0;
-        for (uint32_t d = d0; d < d1; ++d) sum += e.s.trk[e.s.dc_cells[d]].x;
-        uint32_t total;
-        uint32_t run = block_excl_scan(sum, lds[0], total);
-        for (uint32_t d = d0; d < d1; ++d) {
-            e.s.dc_off[d] = run;
-            run += e.s.trk[e.s.dc_cells[d]].x;
-            e.s.dc_cur[d] = 0u;
-            e.s.dc_done[d] = 0u;
+        for (uint32_t d0 = 0; d0 < nd; d0 += kMulChunk) {
+            const uint32_t nc = min(nd - d0, kMulChunk);
+            uint32_t cell[kMulU], v[kMulU];
+#pragma unroll
+            for (uint32_t u = 0; u < kMulU; ++u) {
+                const uint32_t k = threadIdx.x + 1024u * u;
+                cell[u] = (k < nc) ? e.s.dc_cells[d0 + k] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kMulU; ++u) v[u] = (threadIdx.x + 1024u * u < nc) ? e.s.trk[cell[u]].x : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kMulU; ++u) {
+                const uint32_t k = threadIdx.x + 1024u * u;
+                if (k < nc) {
+                    s_dcn[k] = v[u];
+                    e.s.dc_cur[d0 + k] = 0u;
+                    e.s.dc_done[d0 + k] = 0u;
+                }
+            }
+            __syncthreads();
+            const uint32_t k0 = threadIdx.x * kMulU;
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < kMulU; ++u) sum += (k0 + u < nc) ? s_dcn[k0 + u] : 0u;
+            uint32_t total;
+            uint32_t run = base + block_excl_scan(sum, lds[0], total);
+#pragma unroll
+            for (uint32_t u = 0; u < kMulU; ++u)
+                if (k0 + u < nc) {
+                    e.s.dc_off[d0 + k0 + u] = run;
+                    run += s_dcn[k0 + u];
+                }
+            base += total;
+            __syncthreads();  // s_dcn is refilled by the next chunk
         }
     }
+    // per tuple, coalesced: its bucket (hit count 3..7 -> 0..4), kMulNone when
+    // it takes no part (fewer than 3 hits or dup), kMulLong for 8 or more hits
+    for (uint32_t t0 = 0; t0 < m; t0 += 4096u) {
+        uint32_t a[4], b[4];
+        int32_t q[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t t = t0 + threadIdx.x + 1024u * u;
+            a[u] = (t < m) ? e.hit_off[t] : 0u;
+            b[u] = (t < m) ? e.hit_off[t + 1] : 0u;
+            q[u] = (t < m) ? e.quality[t] : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u) {
+            const uint32_t t = t0 + threadIdx.x + 1024u * u;
+            const uint32_t nh = b[u] - a[u];
+            if (t < m)
+                s_code[t] = (nh < 3u || q[u] == PT_QUALITY_DUP) ? kMulNone : (nh >= 8u) ? kMulLong : (uint8_t)(nh - 3u);
+        }
+    }
+    __syncthreads();
+    // each thread a contiguous run of tuples, so ranks in a bucket follow id order
     const uint32_t chunk = (m + blockDim.x - 1) / blockDim.x;
     const uint32_t t0 = threadIdx.x * chunk, t1 = min(m, t0 + chunk);
     uint32_t c[5] = {0, 0, 0, 0, 0};
     uint32_t ref = 0;
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t nh = e.hit_off[t + 1] - e.hit_off[t];
-        if (nh < 3 || e.quality[t] == PT_QUALITY_DUP) continue;
-        if (nh >= 8) {
-            ref |= PT_REF_MULT;
-            continue;
-        }
-        c[nh - 3]++;
+        const uint32_t k = s_code[t];
+        if (k == kMulLong) ref |= PT_REF_MULT;
+#pragma unroll
+        for (uint32_t bk = 0; bk < 5u; ++bk) c[bk] += (k == bk) ? 1u : 0u;
     }
     uint32_t ex[5];
     for (int bk = 0; bk < 5; ++bk) {
@@ -2173,26 +2218,15 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
     }
     __syncthreads();
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t nh = e.hit_off[t + 1] - e.hit_off[t];
-        if (nh < 3 || nh >= 8 || e.quality[t] == PT_QUALITY_DUP) continue;
-        const int bk = nh - 3;
+        const uint32_t bk = s_code[t];
+        if (bk >= 5u) continue;
         const uint32_t rank = ex[bk]++;
-        e.s.tm_bins[e.s.tm_off[nh - 1] + (tot[bk] - 1u - rank)] = (uint16_t)(t + 1u);
+        e.s.tm_bins[e.s.tm_off[bk + 2u] + (tot[bk] - 1u - rank)] = (uint16_t)(t + 1u);
     }
     if (ref) atomicOr(&e.cnt->ref_assert, ref);
 }
 
 // ------------------------------------------------------------------ launchers
-// Blocks per event for a grid-stride kernel over `items` items per event:
-// enough to cover the items, at most what keeps about 2048 blocks in flight.
-static inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t xcd, uint32_t bs = 256) {
-    uint32_t per = (items + bs - 1) / bs;
-    const uint32_t target = (2048u + n_events - 1) / n_events;
-    if (per > target) per = target;
-    if (per < 1) per = 1;
-    return EvGrid{n_events, per, xcd};
-}
-
 // Blocks per event for a kernel whose wavefronts take work by ticket: exactly
 // the blocks that are resident at once on the events' XCDs (occupancy from
 // the runtime), so every wavefront starts at once and none waits for a slot.
@@ -2255,7 +2289,8 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
     mark("k_connect");
     if (cfg.p.early_fish_bone) {
-        k_fish_list<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one);
+        const EvGrid g_fl = ev_grid(max_hits, n_events, xcd);
+        k_fish_list<<<ev_grid_dim(g_fl), 256, 0, st>>>(d_evs, g_fl);
         mark("k_fish_list");
         const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events, xcd);
         k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish, cfg);

@@ -55,10 +55,12 @@ constexpr uint32_t kDoneLds = 8192u;
 
 }  // namespace
 
-// One block of 1024 threads per event runs the whole of classify_tuples after
-// the fit (classification itself is in the fit epilogue), phase by phase with
-// block barriers between them: every phase reads what the previous one wrote,
-// and all of it belongs to the block's own event.
+// classify_tuples after the fit (classification itself is in the fit
+// epilogue) runs as five kernels, one per phase: every phase reads what the
+// previous one wrote for the same event.  Phases 1, 2, 4 and 5 are independent
+// per tuple or per shared cell and take several blocks per event (item i of
+// the event goes to the block-strided loop of that event's blocks); phase 3's
+// rounds need block barriers and run in one block of 1024 threads per event.
 
 // The phases are chains of dependent loads (tuple -> cells -> cell fields ->
 // lists); each is written so that the loads of one step are issued together
@@ -87,8 +89,8 @@ __device__ __forceinline__ void shared_cells_of(const EvDesc &e, uint32_t t, uin
 
 // Phase 1: every stored tuple appends its id to the lists of its shared cells
 // (list offsets come from the multiplicity scan, cursors start at zero).
-__device__ __forceinline__ void cl_lists(const EvDesc &e) {
-    for (uint32_t t = threadIdx.x; t < n_stored(e); t += blockDim.x) {
+__device__ __forceinline__ void cl_lists(const EvDesc &e, uint32_t i0, uint32_t step) {
+    for (uint32_t t = i0; t < n_stored(e); t += step) {
         uint32_t cs[4], sl[4], pos[4], off[4];
         bool v[4];
         shared_cells_of(e, t, cs, sl, v);
@@ -108,8 +110,8 @@ __device__ __forceinline__ void cl_lists(const EvDesc &e) {
 // its 64 smallest tuple ids in ascending order in front (the reference's list
 // is the first 64 pushed, in tuple id order); then the predecessors: the
 // shared cells with a smaller id on the tuples of the (capped) list.
-__device__ __forceinline__ void cl_prepare(const EvDesc &e) {
-    for (uint32_t s = threadIdx.x; s < n_shared(e); s += blockDim.x) {
+__device__ __forceinline__ void cl_prepare(const EvDesc &e, uint32_t i0, uint32_t step) {
+    for (uint32_t s = i0; s < n_shared(e); s += step) {
         uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
         const uint32_t n_all = e.s.dc_cur[s];
         const uint32_t c = e.s.dc_cells[s];
@@ -335,8 +337,8 @@ constexpr uint32_t kHitsAtOnce = 6u;
 // Phase 4: per loose tuple and hit, the hit's loose-tuple count, their largest
 // hit count and the (|tip|, largest id) minimum; the association's entries
 // are counted against its capacity.  The per-hit fields start from k_reset.
-__device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
-    for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
+__device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e, uint32_t i0, uint32_t step) {
+    for (uint32_t t = i0; t < n_classified(e); t += step) {
         const bool loose = e.quality[t] == PT_QUALITY_LOOSE;
         const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
         const float tip = abs_tip(e, t);
@@ -369,8 +371,8 @@ __device__ __forceinline__ void cl_h2t_reduce(const EvDesc &e) {
 // another loose tuple it becomes dup when shorter than the longest there, or,
 // when they are all triplets, when it is not the one with the smallest |tip|.
 // The survivors are the loose count.
-__device__ __forceinline__ void cl_h2t_decide(const EvDesc &e) {
-    for (uint32_t t = threadIdx.x; t < n_classified(e); t += blockDim.x) {
+__device__ __forceinline__ void cl_h2t_decide(const EvDesc &e, uint32_t i0, uint32_t step, bool first) {
+    for (uint32_t t = i0; t < n_classified(e); t += step) {
         bool keep = false;
         const bool loose = e.quality[t] == PT_QUALITY_LOOSE;
         const uint32_t b0 = e.hit_off[t] - 1u, b1 = e.hit_off[t + 1] - 1u;
@@ -402,34 +404,62 @@ __device__ __forceinline__ void cl_h2t_decide(const EvDesc &e) {
         const uint64_t m = __ballot(keep);
         if ((threadIdx.x & 63u) == 0 && m) atomicAdd(&e.cnt->n_loose, (uint32_t)__popcll(m));
     }
-    // the entry count was summed by atomics in this kernel: read it from L2 (an
-    // earlier plain load of misc may have left the line in this CU's cache)
-    if (threadIdx.x == 0 && atomicAdd(&e.s.misc[kMiscH2tEnt], 0u) > kH2tCap)
-        atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
+    // the entry count was summed by the previous kernel
+    if (first && threadIdx.x == 0 && e.s.misc[kMiscH2tEnt] > kH2tCap) atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
 }
 
-__global__ void __launch_bounds__(1024) k_classify_event(const EvDesc *__restrict__ evs, EvGrid grid) {
+// the block-strided loop of a multi-block phase: first item and stride
+#define CL_PHASE(e_, i0_, step_)                                                                    \
+    uint32_t ev_, blk_;                                                                             \
+    if (!ev_block(grid, ev_, blk_)) return;                                                         \
+    const EvDesc &e_ = evs[ev_];                                                                    \
+    const uint32_t i0_ = blk_ * blockDim.x + threadIdx.x, step_ = grid.per_event * blockDim.x
+
+__global__ void __launch_bounds__(256) k_cl_lists(const EvDesc *__restrict__ evs, EvGrid grid) {
+    CL_PHASE(e, i0, step);
+    cl_lists(e, i0, step);
+}
+
+__global__ void __launch_bounds__(256) k_cl_prepare(const EvDesc *__restrict__ evs, EvGrid grid) {
+    CL_PHASE(e, i0, step);
+    cl_prepare(e, i0, step);
+}
+
+__global__ void __launch_bounds__(1024) k_cl_rounds(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint32_t s_done[kDoneLds];
     __shared__ uint32_t s_pending[3];
     uint32_t ev, blk;
     if (!ev_block(grid, ev, blk)) return;
-    const EvDesc &e = evs[ev];
-    cl_lists(e);
-    __syncthreads();
-    cl_prepare(e);
-    __syncthreads();
-    cl_rounds(e, s_done, s_pending);
-    __syncthreads();
-    cl_h2t_reduce(e);
-    __syncthreads();
-    cl_h2t_decide(e);
+    cl_rounds(evs[ev], s_done, s_pending);
+}
+
+__global__ void __launch_bounds__(256) k_cl_h2t_reduce(const EvDesc *__restrict__ evs, EvGrid grid) {
+    CL_PHASE(e, i0, step);
+    cl_h2t_reduce(e, i0, step);
+}
+
+__global__ void __launch_bounds__(256) k_cl_h2t_decide(const EvDesc *__restrict__ evs, EvGrid grid) {
+    CL_PHASE(e, i0, step);
+    cl_h2t_decide(e, i0, step, blk_ == 0);
 }
 
 void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm) {
-    (void)cfg;
     const EvGrid one{n_events, 1u, cfg.xcd};
-    k_classify_event<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
-    if (tm) tm->record("k_classify_event", st);
+    const EvGrid tup = ev_grid(kMaxTuples, n_events, cfg.xcd);
+    const EvGrid shr = ev_grid(kDcCap, n_events, cfg.xcd);
+    auto mark = [&](const char *name) {
+        if (tm) tm->record(name, st);
+    };
+    k_cl_lists<<<ev_grid_dim(tup), 256, 0, st>>>(d_evs, tup);
+    mark("k_cl_lists");
+    k_cl_prepare<<<ev_grid_dim(shr), 256, 0, st>>>(d_evs, shr);
+    mark("k_cl_prepare");
+    k_cl_rounds<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
+    mark("k_cl_rounds");
+    k_cl_h2t_reduce<<<ev_grid_dim(tup), 256, 0, st>>>(d_evs, tup);
+    mark("k_cl_h2t_reduce");
+    k_cl_h2t_decide<<<ev_grid_dim(tup), 256, 0, st>>>(d_evs, tup);
+    mark("k_cl_h2t_decide");
 }
 
 }  // namespace pt

@@ -282,6 +282,16 @@ __device__ __forceinline__ bool ev_block(const EvGrid &g, uint32_t &ev, uint32_t
 
 inline dim3 ev_grid_dim(const EvGrid &g) { return dim3(8u * g.per_event * ((g.n_events + 7u) / 8u)); }
 
+// Blocks per event for a grid-stride kernel over `items` items per event:
+// enough to cover the items, at most what keeps about 2048 blocks in flight.
+inline EvGrid ev_grid(uint32_t items, uint32_t n_events, uint32_t xcd, uint32_t bs = 256) {
+    uint32_t per = (items + bs - 1) / bs;
+    const uint32_t target = (2048u + n_events - 1) / n_events;
+    if (per > target) per = target;
+    if (per < 1) per = 1;
+    return EvGrid{n_events, per, xcd};
+}
+
 // Grid-stride loop over the items of this block's event; blocks beyond the
 // last event return.
 #define PT_EVENT_LOOP(e_, i_, n_)                                                                   \
