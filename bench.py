@@ -350,7 +350,7 @@ def adjoint_only(ctx, frames, n_tracks=262144, steps=20, cpu=False):
     return out
 
 
-def pu200_stream(frames, n_events=200, group_size=8, n_groups=4, cpu=False, cpu_seconds=12.0):
+def pu200_stream(frames, n_events=200, group_size=8, n_groups=8, cpu=False, cpu_seconds=12.0):
     """PU200-like stream: 200 events (seeds 1001..1200, 7000 helices from 200
     vertices, +5% noise), resident in HBM, run through the graph pipeline with
     `group_size` events per captured hipGraph and `n_groups` groups in flight
@@ -364,6 +364,10 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4, cpu=False, cpu_
     batch = ca.DeviceBatch(ctx, events)
     try:
         ctx.set_fit_precision(A.FIT_MIXED)
+        # CA tile sizes tuned on one group of the stream before the graphs are captured
+        tb = ca.DeviceBatch(ctx, events[:group_size])
+        tuning, _ = tb.autotune_ca()
+        tb.free()
         pl = ca.Pipeline(ctx, group_size, n_groups)
         n_sub = n_events // group_size
 
@@ -392,6 +396,7 @@ def pu200_stream(frames, n_events=200, group_size=8, n_groups=4, cpu=False, cpu_
                 "workload": f"{n_events} pu200 events (seeds 1001..{1000 + n_events}), mean {hits.mean():.0f} hits "
                             f"(max {hits.max()}), mixed-precision fit",
                 "events_in_flight": group_size * n_groups, "events_per_graph": group_size, "graphs_in_flight": n_groups,
+                "ca_tuning": tuning,
                 "events_per_s": round(n_sub * group_size / wall, 1),
                 "latency_ms_p50": round(float(np.percentile(per_event, 50)), 4),
                 "latency_ms_p99": round(float(np.percentile(per_event, 99)), 4),
@@ -478,6 +483,11 @@ def main():
     ctxs = [ca.Context(max_events=len(events), frames=frames) for _ in range(n_fl)]
     batches = [ca.DeviceBatch(c, events) for c in ctxs]
     ctx, batch = ctxs[0], batches[0]
+    # CA tile sizes: tuned on the batch (pt_ca_autotune), used by every context
+    ca_tuning, _ = batch.autotune_ca()
+    for c in ctxs:
+        c.set_ca_tuning(**ca_tuning)
+    print(f"[bench] CA tuning {ca_tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         for b in batches:
             b.run_async(out_zeroed=False)
@@ -627,6 +637,7 @@ def main():
                         f"{np.mean(cnt['n_tuples']):.0f} tuples, {np.mean(cnt['n_fitted']):.0f} fits per event",
             "events_per_step": len(events),
             "batches_in_flight": n_fl,
+            "ca_tuning": ca_tuning,
             "parallelism": "events batched in one launch sequence; an event's blocks on one XCD; batches in flight "
                            "on separate streams",
         },

@@ -497,24 +497,25 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
 // ------------------------------------------------------------------ K1: doublet count
 // A wavefront takes a tile of 64 consecutive inner slots.  Their candidates
 // (about 16 per slot, lane-contiguous in slot order) are spread over all 64
-// lanes, kEvalU rounds of 64 loads in flight at a time.  Each item finds its
+// lanes, U rounds of 64 loads in flight at a time (U = 1, 2 or 4, tuned).  Each item finds its
 // slot through an owner map in LDS (a slot's lane at its first item, carried
 // forward by a running maximum), and each round's pass ballot is stored as a
 // 64-bit word; afterwards every lane reads its own slot's bits back: the
 // count and, for the first 64 candidates, the bit mask.  Windows of
 // kCountWin items bound the map.  Per tile of kSlotTile slots the counts are
 // also summed.
-constexpr int kEvalU = 4;
 constexpr uint32_t kCountWin = 1024;
-static_assert(kCountWin % (64u * kEvalU) == 0 && kCountWin == 64u * 16u, "window = 16 words per lane");
+static_assert(kCountWin == 64u * 16u, "window = 16 words per lane");
 
 struct alignas(16) SlotItem {
     SlotCtx s;
     uint32_t excl;  // the slot's first item in the tile
 };
 
+template <int U>
 __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg,
                                                          int zero_state) {
+    static_assert(kCountWin % (64u * U) == 0, "whole rounds per window");
     __shared__ SlotItem ctx[4][64];
     __shared__ uint4 omap[4][kCountWin / 4];
     __shared__ uint64_t pbits[4][kCountWin / 64 + 1];
@@ -576,11 +577,11 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
                 omap[w][4u * lane + v] = make_uint4(max(m[4 * v], carry), max(m[4 * v + 1], carry),
                                                     max(m[4 * v + 2], carry), max(m[4 * v + 3], carry));
             wave_sync();
-            for (uint32_t base = 0; base < wn; base += 64u * kEvalU) {
-                SlotItem c[kEvalU];
-                uint4 q[kEvalU];
+            for (uint32_t base = 0; base < wn; base += 64u * U) {
+                SlotItem c[U];
+                uint4 q[U];
 #pragma unroll
-                for (int u = 0; u < kEvalU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const uint32_t g = base + 64u * u + lane;
                     const bool act = g < wn;
                     const uint32_t o = act ? own[g] - 1u : 0u;
@@ -589,7 +590,7 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
                     if (act) q[u] = e.s.binrec[cand_pos(c[u].s, w0 + g - c[u].excl)];
                 }
 #pragma unroll
-                for (int u = 0; u < kEvalU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     // every lane evaluates the cuts (no branch); only real items count
                     const bool valid = base + 64u * u + lane < wn;
                     uint32_t r = 0;
@@ -682,20 +683,22 @@ __global__ void __launch_bounds__(256) k_doublet_scan(const EvDesc *__restrict__
 // consecutive entries of the reservation.  That cuts the cursor atomics, and
 // the entries of one outer hit are written by one instruction to adjacent
 // addresses.
-constexpr int kFillU = 3;                    // rounds of 64 cells per chunk
-constexpr uint32_t kFillHash = 256;          // > 64 * kFillU distinct keys, so an insert always ends
+// U rounds of 64 cells per chunk (U = 1, 2 or 3, tuned); the table holds more
+// than 64 * U distinct keys, so an insert always ends
+constexpr uint32_t kFillHash = 256;
 constexpr uint32_t kFillEmpty = 0xFFFFFFFFu;
-static_assert(kFillHash > 64u * kFillU && (kFillHash & (kFillHash - 1u)) == 0u, "hash table size");
+static_assert((kFillHash & (kFillHash - 1u)) == 0u, "hash table size");
 
-struct FillHash {
+template <int U> struct FillHash {
+    static_assert(kFillHash > 64u * U, "hash table size");
     uint32_t key[kFillHash], cnt[kFillHash], base[kFillHash];
     uint32_t pre[kFillHash];   // the slot's first entry in the chunk's entries grouped by slot
-    uint2 ent[64u * kFillU];   // those entries: (cell, slot | inner hit << 8)
+    uint2 ent[64u * U];        // those entries: (cell, slot | inner hit << 8)
 };
 
 // table slot of key k (inserted if absent); the table never fills, so the
 // linear probe ends
-__device__ __forceinline__ uint32_t fill_hash_slot(FillHash &H, uint32_t k) {
+template <class HT> __device__ __forceinline__ uint32_t fill_hash_slot(HT &H, uint32_t k) {
     uint32_t s = (k * 2654435761u) >> 24;  // 8 bits
     for (;;) {
         const uint32_t old = atomicCAS(&H.key[s], kFillEmpty, k);
@@ -704,18 +707,19 @@ __device__ __forceinline__ uint32_t fill_hash_slot(FillHash &H, uint32_t k) {
     }
 }
 
+template <int U>
 __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ uint32_t lid[4][64];
     __shared__ unsigned long long lmsk[4][64];
     __shared__ uint4 lpos[4][64];
-    __shared__ FillHash hash_all[4];
+    __shared__ FillHash<U> hash_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     __shared__ PairLds PT;
     load_pairs(PT, e, cfg);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    FillHash &H = hash_all[w];
+    FillHash<U> &H = hash_all[w];
 #pragma unroll
     for (uint32_t v = 0; v < kFillHash / 64u; ++v) {
         H.key[64u * v + lane] = kFillEmpty;
@@ -775,21 +779,21 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         const uint32_t fexcl = fincl - nm;
         const uint32_t ftotal = __shfl(fincl, 63, 64);
         const uint32_t jw = t * 64u;
-        for (uint32_t base = 0; base < ftotal; base += 64u * kFillU) {
-            // a chunk of kFillU rounds: loads; cell writes and the outer hits into the
+        for (uint32_t base = 0; base < ftotal; base += 64u * U) {
+            // a chunk of U rounds: loads; cell writes and the outer hits into the
             // table; one cursor reservation per table entry; list entries
-            bool act[kFillU];
-            uint32_t cid[kFillU], pos[kFillU], js[kFillU], oh[kFillU], i1[kFillU], hs[kFillU];
-            int pp[kFillU];
-            uint4 q[kFillU];
-            float2 xy[kFillU];
-            uint32_t gg[kFillU];
-            int ow[kFillU];
+            bool act[U];
+            uint32_t cid[U], pos[U], js[U], oh[U], i1[U], hs[U];
+            int pp[U];
+            uint4 q[U];
+            float2 xy[U];
+            uint32_t gg[U];
+            int ow[U];
 #pragma unroll
-            for (int u = 0; u < kFillU; ++u) gg[u] = base + 64u * u + lane;
+            for (int u = 0; u < U; ++u) gg[u] = base + 64u * u + lane;
             owners_of(fincl, gg, ow);
 #pragma unroll
-            for (int u = 0; u < kFillU; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const uint32_t g = gg[u];
                 const int o = ow[u];
                 const uint32_t rk = g - (uint32_t)__shfl(fexcl, o, 64);
@@ -805,7 +809,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kFillU; ++u)
+            for (int u = 0; u < U; ++u)
                 if (act[u]) {
                     write_cell(js[u], pp[u], q[u], xy[u], cid[u], oh[u], i1[u]);
                     hs[u] = fill_hash_slot(H, oh[u]);
@@ -833,7 +837,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
             // each entry takes a rank in its slot (the decrements bring the counts back
             // to zero) and its place in the grouped order
 #pragma unroll
-            for (int u = 0; u < kFillU; ++u)
+            for (int u = 0; u < U; ++u)
                 if (act[u]) {
                     const uint32_t r = atomicSub(&H.cnt[hs[u]], 1u) - 1u;
                     H.ent[H.pre[hs[u]] + r] = make_uint2(cid[u], hs[u] | ((i1[u] - 1u) << 8));
@@ -971,7 +975,7 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 
 // Per hit h, the rows are the cells ending on h that kernel_connect visits (the
 // first 128 of h's list) and the columns the cells starting on h, in id order
-// (one contiguous id range per layer pair).  A wavefront takes kConnHits hits
+// (one contiguous id range per layer pair).  A wavefront takes H hits
 // per ticket and sorts them by column count, descending (so that rows of
 // similar length share a round).  Consecutive sorted hits form groups whose
 // columns fit the LDS staging area; a hit with more columns is a group of its
@@ -984,30 +988,32 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 // cut (about 2% of the pairs get there) runs on the survivors of a whole round
 // of columns, compacted through LDS.  Each mark of a used cell is counted per
 // outer hit for the fishbone's work list.
-constexpr uint32_t kConnHits = 24;
+// H hits per ticket (4, 8, 16, 24 or 32) and ST columns staged at a time (128,
+// 192, 256 or 384) are tuned: denser events favour fewer hits per ticket.
 constexpr int kConnCols = 4;
-constexpr uint32_t kConnStage = 192;  // columns staged at a time
 
-struct ConnLds {
-    float hx[kConnHits], hy[kConnHits], hz[kConnHits], hr[kConnHits], theta[kConnHits];
-    uint32_t off[6][kConnHits], cum[6][kConnHits];  // hit lane's column c of range q: cell off[q] + c
-    uint32_t coh[kConnStage];                       // staged columns: outer hit, its (z, r) and (x, y)
-    float2 czr[kConnStage], cxy[kConnStage];
+template <uint32_t H, uint32_t ST> struct ConnLds {
+    static_assert(H <= 64u && ST % 64u == 0u, "one hit per lane, whole staging rounds");
+    float hx[H], hy[H], hz[H], hr[H], theta[H];
+    uint32_t off[6][H], cum[6][H];  // hit lane's column c of range q: cell off[q] + c
+    uint32_t coh[ST];               // staged columns: outer hit, its (z, r) and (x, y)
+    float2 czr[ST], cxy[ST];
     // the row batch's rows (per lane), and the pairs waiting for the dca cut:
     // row lane | staged column << 8
     float rx1[64], ry1[64], rreg[64];
     uint32_t rhl[64], q[64];
 };
 
+template <uint32_t kConnHits, uint32_t kConnStage>
 __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
-    __shared__ ConnLds lds_all[4];
+    __shared__ ConnLds<kConnHits, kConnStage> lds_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
     __shared__ PairLds PT;
     load_pairs(PT, e, cfg);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    ConnLds &S = lds_all[w];
+    ConnLds<kConnHits, kConnStage> &S = lds_all[w];
     const uint32_t n_cells = e.cnt->n_cells, n_hits = e.n_hits;
     uint32_t n_eval = 0, n_conn = 0, n_lists = 0, ovf = 0;
     for (;;) {
@@ -1317,12 +1323,11 @@ __global__ void __launch_bounds__(256) k_fish_list(const EvDesc *__restrict__ ev
 // exists, and jc dies if some ic qualifies with it and jc < first(ic).
 // Cells that connect did not use take no part (about a tenth of the cells are
 // used at PU50), so only the used entries of each list are kept, in order.
-// A wavefront takes kHitTile hits per ticket, packs them into groups of at
+// A wavefront takes kHitTile hits per ticket (8, 16 or 32, tuned), packs them into groups of at
 // most kFishRows list entries, keeps the used ones (rows) in LDS and spreads
 // every unordered row pair of a hit over the lanes; a second step decides
 // each row.
 constexpr uint32_t kFishRows = 128;
-constexpr uint32_t kHitTile = 16;  // hits per ticket
 
 struct FishLds {
     uint32_t id[kFishRows], mod[kFishRows], first[kFishRows];
@@ -1340,7 +1345,9 @@ __device__ __forceinline__ uint32_t tri_b(uint32_t k) {
     return bb;
 }
 
+template <uint32_t kHitTile>
 __global__ void __launch_bounds__(256) k_fishbone(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    static_assert(kHitTile <= 64u, "one hit per lane");
     __shared__ FishLds lds_all[4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
@@ -2259,7 +2266,8 @@ static EvGrid ev_grid_resident(K kernel, uint32_t n_events, uint32_t xcd, int bs
 }
 
 // Launch sequence of the CA stage.  When `tm` is given, an event is recorded
-// after every kernel so the host can read per-kernel durations.
+// after every kernel so the host can read per-kernel durations (the caller
+// records the first one before the stage).
 void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32_t max_hits, uint32_t max_slots,
                int zero_state, hipStream_t st, Marks *tm) {
     const uint32_t est_cells = max_slots * 4u + 1024u;
@@ -2268,32 +2276,65 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     };
     const uint32_t xcd = cfg.xcd;
     const EvGrid one{n_events, 1u, xcd};
-    mark("start");
     const EvGrid g_reset = ev_grid(max_hits + max_slots / kSlotTile + kDfsAgg + kMaxCells / 64u + 64u +
                                        (zero_state ? 20u * kMaxTuples : 0u),
                                    n_events, xcd);
     k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
     mark("k_reset");
-    const EvGrid g_cnt = ev_grid_resident(k_doublet_count, n_events, xcd);
-    k_doublet_count<<<ev_grid_dim(g_cnt), 256, 0, st>>>(d_evs, g_cnt, cfg, zero_state);
+    // the ticketed kernels: exactly the resident blocks, in the tuned instantiation
+    auto resident = [&](auto kernel, auto... args) {
+        const EvGrid g = ev_grid_resident(kernel, n_events, xcd);
+        hipLaunchKernelGGL(kernel, ev_grid_dim(g), dim3(256), 0, st, d_evs, g, args...);
+    };
+    switch (cfg.count_rounds) {
+        case 1: resident(k_doublet_count<1>, cfg, zero_state); break;
+        case 2: resident(k_doublet_count<2>, cfg, zero_state); break;
+        default: resident(k_doublet_count<4>, cfg, zero_state); break;
+    }
     mark("k_doublet_count");
     const EvGrid g_dscan = ev_grid(max_slots / 4u, n_events, xcd);
     k_doublet_scan<<<ev_grid_dim(g_dscan), 256, 0, st>>>(d_evs, g_dscan, cfg);
     mark("k_doublet_scan");
-    const EvGrid g_fill = ev_grid_resident(k_doublet_fill, n_events, xcd);
-    k_doublet_fill<<<ev_grid_dim(g_fill), 256, 0, st>>>(d_evs, g_fill, cfg);
+    switch (cfg.fill_rounds) {
+        case 1: resident(k_doublet_fill<1>, cfg); break;
+        case 2: resident(k_doublet_fill<2>, cfg); break;
+        default: resident(k_doublet_fill<3>, cfg); break;
+    }
     mark("k_doublet_fill");
     k_ohc_rebuild<<<ev_grid_dim(one), 64, 0, st>>>(d_evs, one);
     mark("k_ohc_rebuild");
-    const EvGrid g_conn = ev_grid_resident(k_connect, n_events, xcd);
-    k_connect<<<ev_grid_dim(g_conn), 256, 0, st>>>(d_evs, g_conn, cfg);
+    switch (cfg.conn_hits * 1000u + cfg.conn_stage) {
+        case 4128: resident(k_connect<4, 128>, cfg); break;
+        case 4192: resident(k_connect<4, 192>, cfg); break;
+        case 4256: resident(k_connect<4, 256>, cfg); break;
+        case 4384: resident(k_connect<4, 384>, cfg); break;
+        case 8128: resident(k_connect<8, 128>, cfg); break;
+        case 8192: resident(k_connect<8, 192>, cfg); break;
+        case 8256: resident(k_connect<8, 256>, cfg); break;
+        case 8384: resident(k_connect<8, 384>, cfg); break;
+        case 16128: resident(k_connect<16, 128>, cfg); break;
+        case 16192: resident(k_connect<16, 192>, cfg); break;
+        case 16256: resident(k_connect<16, 256>, cfg); break;
+        case 16384: resident(k_connect<16, 384>, cfg); break;
+        case 24128: resident(k_connect<24, 128>, cfg); break;
+        case 24256: resident(k_connect<24, 256>, cfg); break;
+        case 24384: resident(k_connect<24, 384>, cfg); break;
+        case 32128: resident(k_connect<32, 128>, cfg); break;
+        case 32192: resident(k_connect<32, 192>, cfg); break;
+        case 32256: resident(k_connect<32, 256>, cfg); break;
+        case 32384: resident(k_connect<32, 384>, cfg); break;
+        default: resident(k_connect<24, 192>, cfg); break;
+    }
     mark("k_connect");
     if (cfg.p.early_fish_bone) {
         const EvGrid g_fl = ev_grid(max_hits, n_events, xcd);
         k_fish_list<<<ev_grid_dim(g_fl), 256, 0, st>>>(d_evs, g_fl);
         mark("k_fish_list");
-        const EvGrid g_fish = ev_grid_resident(k_fishbone, n_events, xcd);
-        k_fishbone<<<ev_grid_dim(g_fish), 256, 0, st>>>(d_evs, g_fish, cfg);
+        switch (cfg.fish_hits) {
+            case 8: resident(k_fishbone<8>, cfg); break;
+            case 32: resident(k_fishbone<32>, cfg); break;
+            default: resident(k_fishbone<16>, cfg); break;
+        }
     } else {
         k_connect_ext<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
     }

@@ -206,6 +206,7 @@ struct pt_ctx {
     size_t fit_lists_cap = 0;  // track ids
     uint32_t *fit_counts = nullptr;
     int fit_precision = PT_FIT_FP64;  // precision of the stage's fit
+    pt_ca_tuning ca_tuning = {24, 192, 4, 3, 16};  // tile sizes of the ticketed CA kernels
     // pipeline: group g = event slots [g*pl_size, (g+1)*pl_size) on stream sub[g],
     // its launch sequence captured once into a graph
     uint32_t pl_size = 0, pl_groups = 0;
@@ -453,6 +454,11 @@ static pt::Config make_config(const pt_ctx *c, double b_field) {
     cfg.B = (double)(float)b_field;  // HelixFitOnGPU(Float32(b_field)) then Float64
     cfg.frames = c->d_frames;
     cfg.fit_precision = c->fit_precision;
+    cfg.conn_hits = c->ca_tuning.connect_hits;
+    cfg.conn_stage = c->ca_tuning.connect_stage;
+    cfg.count_rounds = c->ca_tuning.count_rounds;
+    cfg.fill_rounds = c->ca_tuning.fill_rounds;
+    cfg.fish_hits = c->ca_tuning.fish_hits;
     std::memset(cfg.pairs.by_inner, 0, sizeof(cfg.pairs.by_inner));
     std::memset(cfg.pairs.pair_of, -1, sizeof(cfg.pairs.pair_of));
     for (int p = 0; p < 19; ++p) cfg.pairs.pair_of[kLP[2 * p]][kLP[2 * p + 1]] = (int8_t)p;
@@ -521,6 +527,7 @@ static int run_batch(pt_ctx *c, uint32_t n, const pt_hits_view *hits, const pt_t
     if (marks) marks->n = 0;
     const uint32_t n_sub = marks ? 1u : std::max(1u, std::min<uint32_t>((uint32_t)c->n_sub, n));
     auto launch_range = [&](uint32_t e0, uint32_t e1, hipStream_t st) {
+        if (marks) marks->record("start", st);
         if (any_hist) {
             pt::launch_phi_hist(c->d_evs + e0, e1 - e0, c->xcd, st);
             if (marks) marks->record("k_phi_hist", st);
@@ -1216,6 +1223,119 @@ int pt_fit_autotune(pt_ctx *c, uint32_t n, const float *hits, const float *ge, c
     c->tuning[precision == PT_FIT_MIXED ? 1 : 0] = top;
     if (best) *best = top;
     if (ms) *ms = top_ms;
+    return PT_OK;
+}
+
+static bool one_of(uint32_t v, uint32_t a, uint32_t b, uint32_t c3, uint32_t d = 0, uint32_t e = 0) {
+    return v == a || v == b || v == c3 || (d && v == d) || (e && v == e);
+}
+
+static int check_ca_tuning(const pt_ca_tuning &t) {
+    if (!one_of(t.connect_hits, 4, 8, 16, 24, 32))
+        return fail(PT_ERR_ARG, "connect_hits must be 4, 8, 16, 24 or 32");
+    if (!one_of(t.connect_stage, 128, 192, 256, 384))
+        return fail(PT_ERR_ARG, "connect_stage must be 128, 192, 256 or 384");
+    if (!one_of(t.count_rounds, 1, 2, 4)) return fail(PT_ERR_ARG, "count_rounds must be 1, 2 or 4");
+    if (!one_of(t.fill_rounds, 1, 2, 3)) return fail(PT_ERR_ARG, "fill_rounds must be 1, 2 or 3");
+    if (!one_of(t.fish_hits, 8, 16, 32)) return fail(PT_ERR_ARG, "fish_hits must be 8, 16 or 32");
+    return PT_OK;
+}
+
+int pt_ca_set_tuning(pt_ctx *c, const pt_ca_tuning *t) {
+    if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
+    const int rc = check_ca_tuning(*t);
+    if (rc) return rc;
+    c->ca_tuning = *t;
+    return PT_OK;
+}
+
+int pt_ca_get_tuning(pt_ctx *c, pt_ca_tuning *t) {
+    if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
+    *t = c->ca_tuning;
+    return PT_OK;
+}
+
+// Kernel time of `name` in the last timed batch (ms), or a negative value.
+static float marked_ms(pt_ctx *c, const char *name) {
+    float total = -1.0f;
+    for (int k = 0; k + 1 < c->marks.n; ++k) {
+        if (std::strcmp(c->marks.name[k + 1], name) != 0) continue;
+        float t = 0;
+        if (hipEventElapsedTime(&t, c->marks.ev[k], c->marks.ev[k + 1]) != hipSuccess) return -1.0f;
+        total = (total < 0.0f ? 0.0f : total) + t;
+    }
+    return total;
+}
+
+int pt_ca_autotune(pt_ctx *c, uint32_t n, const pt_hits_view *hits, double b_field, const pt_track_soa *out,
+                   pt_ca_tuning *best, float *ms) {
+    if (!c || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
+    if (n == 0) return fail(PT_ERR_ARG, "empty batch");
+    HIP_OK(hipSetDevice(c->device));
+    const bool timing = c->timing;
+    c->timing = true;
+    // one timed run of the stage with tuning t: (kernel time of `name`, stage time)
+    auto run = [&](const pt_ca_tuning &t, const char *name, float &k_ms, float &s_ms) -> int {
+        c->ca_tuning = t;
+        int rc = run_batch(c, n, hits, out, b_field, 0);
+        if (rc) return rc;
+        HIP_OK(hipStreamSynchronize(c->stream));
+        k_ms = marked_ms(c, name);
+        s_ms = 0.0f;
+        if (c->marks.n >= 2) HIP_OK(hipEventElapsedTime(&s_ms, c->marks.ev[0], c->marks.ev[c->marks.n - 1]));
+        return PT_OK;
+    };
+    auto score = [&](const pt_ca_tuning &t, const char *name, float &k_best, float &s_best) -> int {
+        k_best = s_best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            float k = 0, st = 0;
+            const int rc = run(t, name, k, st);
+            if (rc) return rc;
+            if (rep == 0) continue;  // the first run warms up
+            if (k < 0.0f) return fail(PT_ERR_HIP, "kernel %s was not timed", name);
+            k_best = std::min(k_best, k);
+            s_best = std::min(s_best, st);
+        }
+        return PT_OK;
+    };
+    pt_ca_tuning cur = c->ca_tuning;
+    const uint32_t hits5[5] = {4, 8, 16, 24, 32}, stages[4] = {128, 192, 256, 384};
+    const uint32_t count3[3] = {1, 2, 4}, fill3[3] = {1, 2, 3}, fish3[3] = {8, 16, 32};
+    int rc = PT_OK;
+    float stage_ms = 0.0f;
+    for (int knob = 0; knob < 4 && rc == PT_OK; ++knob) {
+        const char *name = knob == 0 ? "k_connect" : knob == 1 ? "k_doublet_count" : knob == 2 ? "k_doublet_fill"
+                                                                                                 : "k_fishbone";
+        pt_ca_tuning top = cur;
+        float top_ms = 1e30f;
+        const int n_cand = knob == 0 ? 20 : 3;
+        for (int i = 0; i < n_cand && rc == PT_OK; ++i) {
+            pt_ca_tuning t = cur;
+            if (knob == 0) {
+                t.connect_hits = hits5[i / 4];
+                t.connect_stage = stages[i % 4];
+            } else if (knob == 1) {
+                t.count_rounds = count3[i];
+            } else if (knob == 2) {
+                t.fill_rounds = fill3[i];
+            } else {
+                t.fish_hits = fish3[i];
+            }
+            float k = 0, st = 0;
+            rc = score(t, name, k, st);
+            if (rc == PT_OK && k < top_ms) {
+                top_ms = k;
+                top = t;
+                stage_ms = st;
+            }
+        }
+        cur = top;
+    }
+    c->ca_tuning = cur;
+    c->timing = timing;
+    if (rc) return rc;
+    if (best) *best = cur;
+    if (ms) *ms = stage_ms;
     return PT_OK;
 }
 

@@ -164,6 +164,9 @@ struct Config {
     PT_G const float *frames;      // [1856*9]
     PairTables pairs;
     int fit_precision;        // PT_FIT_FP64 (the reference's) or PT_FIT_MIXED
+    // tile sizes of the ticketed CA kernels (pt_ca_tuning; every setting gives
+    // the same results)
+    uint32_t conn_hits = 24, conn_stage = 192, count_rounds = 4, fill_rounds = 3, fish_hits = 16;
 };
 
 // Fit-only batch (pre-gathered tracks, track-minor layout of pixeltrack.h).

@@ -179,6 +179,20 @@ class PtFitTuning(C.Structure):
         return {"block": self.block, "tile": self.tile, "blocks": self.blocks}
 
 
+class PtCaTuning(C.Structure):
+    """pt_ca_tuning: tile sizes of the ticketed CA kernels."""
+    _fields_ = [("connect_hits", C.c_uint32), ("connect_stage", C.c_uint32), ("count_rounds", C.c_uint32),
+                ("fill_rounds", C.c_uint32), ("fish_hits", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+CA_TUNING_VALUES = {"connect_hits": (4, 8, 16, 24, 32), "connect_stage": (128, 192, 256, 384), "count_rounds": (1, 2, 4),
+                    "fill_rounds": (1, 2, 3), "fish_hits": (8, 16, 32)}
+CA_TUNING_DEFAULT = {"connect_hits": 24, "connect_stage": 192, "count_rounds": 4, "fill_rounds": 3, "fish_hits": 16}
+
+
 class PtVertexWs(C.Structure):
     """pt_vertex_ws: the WorkSpace arrays load_tracks fills (device pointers)."""
     _fields_ = [("n_tracks", c_u32p), ("index_track", c_u16p), ("zt", c_f32p), ("ezt2", c_f32p),
@@ -194,7 +208,7 @@ HIP_SYMBOLS = [
     "pt_fit_bl_loss_grad_async", "pt_fit_bl_vjp_async", "pt_ctx_set_fit_precision", "pt_pipeline_init",
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
     "pt_set_hits_layer_start", "pt_load_tracks_async", "pt_fit_bl_sched_noalloc_async",
-    "pt_fit_set_schedule",
+    "pt_fit_set_schedule", "pt_ca_set_tuning", "pt_ca_get_tuning", "pt_ca_autotune",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -252,6 +266,10 @@ def hip_lib():
         lib.pt_fit_get_tuning.argtypes = [vp, C.c_int, C.POINTER(PtFitTuning)]
         lib.pt_fit_autotune.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp,
                                         C.POINTER(PtFitTuning), C.POINTER(C.c_float)]
+        lib.pt_ca_set_tuning.argtypes = [vp, C.POINTER(PtCaTuning)]
+        lib.pt_ca_get_tuning.argtypes = [vp, C.POINTER(PtCaTuning)]
+        lib.pt_ca_autotune.argtypes = [vp, C.c_uint32, C.POINTER(PtHitsView), C.c_double, C.POINTER(PtTrackSoA),
+                                       C.POINTER(PtCaTuning), C.POINTER(C.c_float)]
         lib.pt_hit_errors_to_global_async.argtypes = [vp, C.c_uint32, vp, vp, vp]
         lib.pt_fit_bl_loss_grad_async.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp]
         lib.pt_fit_bl_vjp_async.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, C.c_double, vp, vp, vp]

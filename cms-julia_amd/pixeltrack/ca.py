@@ -94,6 +94,20 @@ class Context:
     def set_streams(self, n):
         A.check(self.lib.pt_ctx_set_streams(self.h, n), "pt_ctx_set_streams")
 
+    def set_ca_tuning(self, **kw):
+        """Tile sizes of the ticketed CA kernels (pt_ca_set_tuning); unnamed
+        sizes keep their current value."""
+        t = A.PtCaTuning()
+        A.check(self.lib.pt_ca_get_tuning(self.h, C.byref(t)), "pt_ca_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        A.check(self.lib.pt_ca_set_tuning(self.h, C.byref(t)), "pt_ca_set_tuning")
+
+    def ca_tuning(self):
+        t = A.PtCaTuning()
+        A.check(self.lib.pt_ca_get_tuning(self.h, C.byref(t)), "pt_ca_get_tuning")
+        return t.as_dict()
+
     def enable_timing(self, on=True):
         A.check(self.lib.pt_ctx_enable_timing(self.h, 1 if on else 0), "pt_ctx_enable_timing")
 
@@ -241,6 +255,15 @@ class DeviceBatch:
     def run_async(self, b_field=A.B_FIELD_INV_GEV, out_zeroed=False):
         A.check(self.ctx.lib.pt_make_tuples_device_async(self.ctx.h, self.n, self.views, b_field, self.outs,
                                                          1 if out_zeroed else 0), "pt_make_tuples_device_async")
+
+    def autotune_ca(self, b_field=A.B_FIELD_INV_GEV):
+        """Tune the CA tile sizes on this batch (pt_ca_autotune; the outputs
+        are overwritten).  Returns (tuning, stage ms per batch)."""
+        best = A.PtCaTuning()
+        ms = C.c_float()
+        A.check(self.ctx.lib.pt_ca_autotune(self.ctx.h, self.n, self.views, b_field, self.outs, C.byref(best),
+                                            C.byref(ms)), "pt_ca_autotune")
+        return best.as_dict(), ms.value
 
     def download(self, i) -> TrackSOA:
         """Copy event i's device TrackSOA back to a host TrackSOA."""

@@ -305,6 +305,32 @@ int pt_pipeline_get_counters(pt_ctx *ctx, uint32_t group, uint32_t event, pt_cou
 int pt_ctx_enable_timing(pt_ctx *ctx, int enable);
 int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages);
 
+/* Tile sizes of the CA kernels whose wavefronts take work by ticket (no
+ * reference counterpart).  Every setting gives the same results; only the
+ * speed differs, and the best one depends on the hit density (cells per hit
+ * grow about fourfold from PU50 to PU200).  Defaults: 24, 192, 4, 3, 16. */
+typedef struct pt_ca_tuning {
+    uint32_t connect_hits;   /* hits per connect ticket: 4, 8, 16, 24 or 32             */
+    uint32_t connect_stage;  /* cell columns a connect wavefront stages in LDS: 128, 192,
+                                256 or 384                                               */
+    uint32_t count_rounds;   /* doublet count: rounds of 64 candidate loads in flight:
+                                1, 2 or 4                                                */
+    uint32_t fill_rounds;    /* doublet fill: rounds of 64 cells per chunk: 1, 2 or 3    */
+    uint32_t fish_hits;      /* hits per fishbone ticket: 8, 16 or 32                    */
+} pt_ca_tuning;
+/* PT_ERR_ARG for a value not listed above.  Used by later batches and by
+ * pipelines initialised later. */
+int pt_ca_set_tuning(pt_ctx *ctx, const pt_ca_tuning *t);
+int pt_ca_get_tuning(pt_ctx *ctx, pt_ca_tuning *t);
+/* Tunes the five sizes on the given batch (device-resident events as for
+ * pt_make_tuples_device_async; the outputs are overwritten), one kernel at a
+ * time with the others at their current setting: each candidate runs the
+ * whole stage three times and is scored by its kernel's own duration (the
+ * lowest of the last two runs).  Keeps the fastest in the context, returns it
+ * in *best and the stage's time per batch with it in *ms.  Synchronous. */
+int pt_ca_autotune(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits, double b_field,
+                   const pt_track_soa *out, pt_ca_tuning *best, float *ms);
+
 /* ---------------------------------------------------------------------------
  * Broken-line fit on pre-gathered tracks (BrokenLineFitOnGPU kernelBLFastFit +
  * kernelBLFit, broken_line_fit_on_gpu.jl:24-163), for fit-only workloads.
