@@ -206,7 +206,7 @@ struct pt_ctx {
     size_t fit_lists_cap = 0;  // track ids
     uint32_t *fit_counts = nullptr;
     int fit_precision = PT_FIT_FP64;  // precision of the stage's fit
-    pt_ca_tuning ca_tuning = {24, 192, 4, 3, 16};  // tile sizes of the ticketed CA kernels
+    pt_ca_tuning ca_tuning = {24, 192, 4, 3, 16, 1};  // launch shapes of the CA kernels
     // pipeline: group g = event slots [g*pl_size, (g+1)*pl_size) on stream sub[g],
     // its launch sequence captured once into a graph
     uint32_t pl_size = 0, pl_groups = 0;
@@ -459,6 +459,7 @@ static pt::Config make_config(const pt_ctx *c, double b_field) {
     cfg.count_rounds = c->ca_tuning.count_rounds;
     cfg.fill_rounds = c->ca_tuning.fill_rounds;
     cfg.fish_hits = c->ca_tuning.fish_hits;
+    cfg.classify_split = c->ca_tuning.classify_split;
     std::memset(cfg.pairs.by_inner, 0, sizeof(cfg.pairs.by_inner));
     std::memset(cfg.pairs.pair_of, -1, sizeof(cfg.pairs.pair_of));
     for (int p = 0; p < 19; ++p) cfg.pairs.pair_of[kLP[2 * p]][kLP[2 * p + 1]] = (int8_t)p;
@@ -1238,6 +1239,7 @@ static int check_ca_tuning(const pt_ca_tuning &t) {
     if (!one_of(t.count_rounds, 1, 2, 4)) return fail(PT_ERR_ARG, "count_rounds must be 1, 2 or 4");
     if (!one_of(t.fill_rounds, 1, 2, 3)) return fail(PT_ERR_ARG, "fill_rounds must be 1, 2 or 3");
     if (!one_of(t.fish_hits, 8, 16, 32)) return fail(PT_ERR_ARG, "fish_hits must be 8, 16 or 32");
+    if (t.classify_split > 1u) return fail(PT_ERR_ARG, "classify_split must be 0 or 1");
     return PT_OK;
 }
 
@@ -1255,11 +1257,14 @@ int pt_ca_get_tuning(pt_ctx *c, pt_ca_tuning *t) {
     return PT_OK;
 }
 
-// Kernel time of `name` in the last timed batch (ms), or a negative value.
-static float marked_ms(pt_ctx *c, const char *name) {
+// Time of the kernels named `name` (or, with prefix, whose name starts with
+// it) in the last timed batch (ms), or a negative value.
+static float marked_ms(pt_ctx *c, const char *name, bool prefix) {
     float total = -1.0f;
+    const size_t len = std::strlen(name);
     for (int k = 0; k + 1 < c->marks.n; ++k) {
-        if (std::strcmp(c->marks.name[k + 1], name) != 0) continue;
+        if (prefix ? std::strncmp(c->marks.name[k + 1], name, len) != 0 : std::strcmp(c->marks.name[k + 1], name) != 0)
+            continue;
         float t = 0;
         if (hipEventElapsedTime(&t, c->marks.ev[k], c->marks.ev[k + 1]) != hipSuccess) return -1.0f;
         total = (total < 0.0f ? 0.0f : total) + t;
@@ -1280,7 +1285,7 @@ int pt_ca_autotune(pt_ctx *c, uint32_t n, const pt_hits_view *hits, double b_fie
         int rc = run_batch(c, n, hits, out, b_field, 0);
         if (rc) return rc;
         HIP_OK(hipStreamSynchronize(c->stream));
-        k_ms = marked_ms(c, name);
+        k_ms = marked_ms(c, name, std::strcmp(name, "k_cl") == 0);
         s_ms = 0.0f;
         if (c->marks.n >= 2) HIP_OK(hipEventElapsedTime(&s_ms, c->marks.ev[0], c->marks.ev[c->marks.n - 1]));
         return PT_OK;
@@ -1303,12 +1308,13 @@ int pt_ca_autotune(pt_ctx *c, uint32_t n, const pt_hits_view *hits, double b_fie
     const uint32_t count3[3] = {1, 2, 4}, fill3[3] = {1, 2, 3}, fish3[3] = {8, 16, 32};
     int rc = PT_OK;
     float stage_ms = 0.0f;
-    for (int knob = 0; knob < 4 && rc == PT_OK; ++knob) {
-        const char *name = knob == 0 ? "k_connect" : knob == 1 ? "k_doublet_count" : knob == 2 ? "k_doublet_fill"
-                                                                                                 : "k_fishbone";
+    // classification: every kernel whose name starts with k_cl (either shape)
+    const char *names[5] = {"k_connect", "k_doublet_count", "k_doublet_fill", "k_fishbone", "k_cl"};
+    for (int knob = 0; knob < 5 && rc == PT_OK; ++knob) {
+        const char *name = names[knob];
         pt_ca_tuning top = cur;
         float top_ms = 1e30f;
-        const int n_cand = knob == 0 ? 20 : 3;
+        const int n_cand = knob == 0 ? 20 : knob == 4 ? 2 : 3;
         for (int i = 0; i < n_cand && rc == PT_OK; ++i) {
             pt_ca_tuning t = cur;
             if (knob == 0) {
@@ -1318,8 +1324,10 @@ int pt_ca_autotune(pt_ctx *c, uint32_t n, const pt_hits_view *hits, double b_fie
                 t.count_rounds = count3[i];
             } else if (knob == 2) {
                 t.fill_rounds = fill3[i];
-            } else {
+            } else if (knob == 3) {
                 t.fish_hits = fish3[i];
+            } else {
+                t.classify_split = (uint32_t)i;
             }
             float k = 0, st = 0;
             rc = score(t, name, k, st);

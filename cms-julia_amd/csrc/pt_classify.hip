@@ -56,11 +56,14 @@ constexpr uint32_t kDoneLds = 8192u;
 }  // namespace
 
 // classify_tuples after the fit (classification itself is in the fit
-// epilogue) runs as five kernels, one per phase: every phase reads what the
-// previous one wrote for the same event.  Phases 1, 2, 4 and 5 are independent
-// per tuple or per shared cell and take several blocks per event (item i of
-// the event goes to the block-strided loop of that event's blocks); phase 3's
-// rounds need block barriers and run in one block of 1024 threads per event.
+// epilogue) runs in five phases; every phase reads what the previous one wrote
+// for the same event.  Two launch shapes (tuned per context): five kernels,
+// where phases 1, 2, 4 and 5, independent per tuple or per shared cell, take
+// several blocks per event (item i of the event goes to the block-strided loop
+// of that event's blocks) and phase 3's rounds, which need block barriers, run
+// in one block of 1024 threads per event; or one block of 1024 threads per
+// event for all five phases with barriers between them, which saves four
+// launches where events are small and many.
 
 // The phases are chains of dependent loads (tuple -> cells -> cell fields ->
 // lists); each is written so that the loads of one step are issued together
@@ -404,8 +407,11 @@ __device__ __forceinline__ void cl_h2t_decide(const EvDesc &e, uint32_t i0, uint
         const uint64_t m = __ballot(keep);
         if ((threadIdx.x & 63u) == 0 && m) atomicAdd(&e.cnt->n_loose, (uint32_t)__popcll(m));
     }
-    // the entry count was summed by the previous kernel
-    if (first && threadIdx.x == 0 && e.s.misc[kMiscH2tEnt] > kH2tCap) atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
+    // the entry count was summed by atomics (in this kernel, in the one-block
+    // shape): read it from L2, where an earlier plain load of misc may have left
+    // a stale line in this CU's cache
+    if (first && threadIdx.x == 0 && atomicAdd(&e.s.misc[kMiscH2tEnt], 0u) > kH2tCap)
+        atomicOr(&e.cnt->overflow, PT_OVF_HIT_TO_TUPLE);
 }
 
 // the block-strided loop of a multi-block phase: first item and stride
@@ -443,8 +449,30 @@ __global__ void __launch_bounds__(256) k_cl_h2t_decide(const EvDesc *__restrict_
     cl_h2t_decide(e, i0, step, blk_ == 0);
 }
 
+__global__ void __launch_bounds__(1024) k_classify_event(const EvDesc *__restrict__ evs, EvGrid grid) {
+    __shared__ uint32_t s_done[kDoneLds];
+    __shared__ uint32_t s_pending[3];
+    uint32_t ev, blk;
+    if (!ev_block(grid, ev, blk)) return;
+    const EvDesc &e = evs[ev];
+    cl_lists(e, threadIdx.x, blockDim.x);
+    __syncthreads();
+    cl_prepare(e, threadIdx.x, blockDim.x);
+    __syncthreads();
+    cl_rounds(e, s_done, s_pending);
+    __syncthreads();
+    cl_h2t_reduce(e, threadIdx.x, blockDim.x);
+    __syncthreads();
+    cl_h2t_decide(e, threadIdx.x, blockDim.x, true);
+}
+
 void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, hipStream_t st, Marks *tm) {
     const EvGrid one{n_events, 1u, cfg.xcd};
+    if (!cfg.classify_split) {
+        k_classify_event<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
+        if (tm) tm->record("k_classify_event", st);
+        return;
+    }
     const EvGrid tup = ev_grid(kMaxTuples, n_events, cfg.xcd);
     const EvGrid shr = ev_grid(kDcCap, n_events, cfg.xcd);
     auto mark = [&](const char *name) {

@@ -167,6 +167,7 @@ struct Config {
     // tile sizes of the ticketed CA kernels (pt_ca_tuning; every setting gives
     // the same results)
     uint32_t conn_hits = 24, conn_stage = 192, count_rounds = 4, fill_rounds = 3, fish_hits = 16;
+    uint32_t classify_split = 1;  // classification: five kernels (1) or one block per event (0)
 };
 
 // Fit-only batch (pre-gathered tracks, track-minor layout of pixeltrack.h).

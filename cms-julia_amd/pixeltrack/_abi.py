@@ -182,15 +182,16 @@ class PtFitTuning(C.Structure):
 class PtCaTuning(C.Structure):
     """pt_ca_tuning: tile sizes of the ticketed CA kernels."""
     _fields_ = [("connect_hits", C.c_uint32), ("connect_stage", C.c_uint32), ("count_rounds", C.c_uint32),
-                ("fill_rounds", C.c_uint32), ("fish_hits", C.c_uint32)]
+                ("fill_rounds", C.c_uint32), ("fish_hits", C.c_uint32), ("classify_split", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 CA_TUNING_VALUES = {"connect_hits": (4, 8, 16, 24, 32), "connect_stage": (128, 192, 256, 384), "count_rounds": (1, 2, 4),
-                    "fill_rounds": (1, 2, 3), "fish_hits": (8, 16, 32)}
-CA_TUNING_DEFAULT = {"connect_hits": 24, "connect_stage": 192, "count_rounds": 4, "fill_rounds": 3, "fish_hits": 16}
+                    "fill_rounds": (1, 2, 3), "fish_hits": (8, 16, 32), "classify_split": (0, 1)}
+CA_TUNING_DEFAULT = {"connect_hits": 24, "connect_stage": 192, "count_rounds": 4, "fill_rounds": 3, "fish_hits": 16,
+                     "classify_split": 1}
 
 
 class PtVertexWs(C.Structure):

@@ -308,7 +308,7 @@ int pt_get_stage_times(pt_ctx *ctx, float *ms, const char **names, int *n_stages
 /* Tile sizes of the CA kernels whose wavefronts take work by ticket (no
  * reference counterpart).  Every setting gives the same results; only the
  * speed differs, and the best one depends on the hit density (cells per hit
- * grow about fourfold from PU50 to PU200).  Defaults: 24, 192, 4, 3, 16. */
+ * grow about fourfold from PU50 to PU200).  Defaults: 24, 192, 4, 3, 16, 1. */
 typedef struct pt_ca_tuning {
     uint32_t connect_hits;   /* hits per connect ticket: 4, 8, 16, 24 or 32             */
     uint32_t connect_stage;  /* cell columns a connect wavefront stages in LDS: 128, 192,
@@ -317,15 +317,17 @@ typedef struct pt_ca_tuning {
                                 1, 2 or 4                                                */
     uint32_t fill_rounds;    /* doublet fill: rounds of 64 cells per chunk: 1, 2 or 3    */
     uint32_t fish_hits;      /* hits per fishbone ticket: 8, 16 or 32                    */
+    uint32_t classify_split; /* classification as five kernels, several blocks per event
+                                for the per-tuple phases (1), or one block per event (0) */
 } pt_ca_tuning;
 /* PT_ERR_ARG for a value not listed above.  Used by later batches and by
  * pipelines initialised later. */
 int pt_ca_set_tuning(pt_ctx *ctx, const pt_ca_tuning *t);
 int pt_ca_get_tuning(pt_ctx *ctx, pt_ca_tuning *t);
-/* Tunes the five sizes on the given batch (device-resident events as for
+/* Tunes the six settings on the given batch (device-resident events as for
  * pt_make_tuples_device_async; the outputs are overwritten), one kernel at a
  * time with the others at their current setting: each candidate runs the
- * whole stage three times and is scored by its kernel's own duration (the
+ * whole stage three times and is scored by its kernels' own duration (the
  * lowest of the last two runs).  Keeps the fastest in the context, returns it
  * in *best and the stage's time per batch with it in *ms.  Synchronous. */
 int pt_ca_autotune(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits, double b_field,

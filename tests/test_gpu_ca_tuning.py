@@ -1,7 +1,7 @@
-"""GPU tests of the CA tile sizes (pt_ca_tuning): every size of every tuned
+"""GPU tests of the CA launch shapes (pt_ca_tuning): every size of every tuned
 kernel (connect hits per ticket x staged columns, doublet-count rounds,
-doublet-fill rounds, fishbone hits per ticket) gives the stage of the default
-sizes bit for bit, and the oracle's CA (cells, outer-hit, neighbour and track
+doublet-fill rounds, fishbone hits per ticket) and both shapes of the
+classification give the stage of the default settings bit for bit, and the oracle's CA (cells, outer-hit, neighbour and track
 lists, counters, quality) on a batch that holds a one-event case, two PU50
 events and a PU200 event past the doublet cap (about 4x the cells per hit of
 PU50, so the staging of connect runs in chunks).  The autotuner keeps the
@@ -27,7 +27,7 @@ def variants():
     d = A.CA_TUNING_DEFAULT
     out = [dict(connect_hits=h, connect_stage=s) for h in A.CA_TUNING_VALUES["connect_hits"]
            for s in A.CA_TUNING_VALUES["connect_stage"]]
-    for k in ("count_rounds", "fill_rounds", "fish_hits"):
+    for k in ("count_rounds", "fill_rounds", "fish_hits", "classify_split"):
         out += [{k: v} for v in A.CA_TUNING_VALUES[k]]
     return [v for v in out if any(d[k] != x for k, x in v.items())]
 
@@ -96,7 +96,7 @@ def test_autotune_keeps_results(setup):
 
 
 @pytest.mark.parametrize("field,value", [("connect_hits", 20), ("connect_stage", 64), ("count_rounds", 8),
-                                         ("fill_rounds", 4), ("fish_hits", 0)])
+                                         ("fill_rounds", 4), ("fish_hits", 0), ("classify_split", 2)])
 def test_unsupported_sizes_are_rejected(setup, field, value):
     ctx = setup[1]
     t = A.PtCaTuning(**A.CA_TUNING_DEFAULT)
