@@ -433,10 +433,11 @@ __device__ __forceinline__ void zero_chunks(const EvDesc &e) {
 // zeroes state and covariance.
 __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, EvGrid grid, int zero_state) {
     constexpr uint32_t kHead = 32u;
-    constexpr uint32_t kBits = (kMaxCells + 63u) / 64u;
     // state and covariance are cleared here only when they are not both 16-byte
-    // aligned; otherwise the doublet count clears them (zero_chunks)
-    PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kSlotTile - 1) / kSlotTile + kDfsAgg + kBits +
+    // aligned; otherwise the doublet count clears them (zero_chunks).  The
+    // per-cell bits and tile sums are cleared by the doublet fill, which knows
+    // the cell count.
+    PT_EVENT_LOOP(e, i, kHead + e.n_hits + (e.n_slots + kSlotTile - 1) / kSlotTile +
                             ((zero_state && !zero_vec(e)) ? 20u * kMaxTuples : 0u)) {
         uint32_t k = i;
         if (k < kHead) {
@@ -477,16 +478,6 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
             continue;
         }
         k -= n_dbl_agg;
-        if (k < kDfsAgg) {
-            e.s.dfs_agg[k] = 0ull;
-            continue;
-        }
-        k -= kDfsAgg;
-        if (k < kBits) {
-            e.s.start_bits[k] = 0ull;
-            continue;
-        }
-        k -= kBits;
         if (k < 5u * kMaxTuples)
             e.state[k] = 0.0f;
         else
@@ -729,6 +720,18 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
     const uint32_t n_tiles = (n_slots + 63u) / 64u;
     const uint32_t cap = cfg.p.max_num_of_doublets;
     uint32_t ref = 0, ovf = 0, err = 0;
+    {
+        // the start-cell bits of connect and the tile sums of the ntuplet count
+        // start from zero over the stored cells
+        const uint32_t nc = e.cnt->n_cells;
+        const uint32_t nw = (nc + 63u) / 64u, nt = (nc + kDfsTile - 1u) / kDfsTile;
+        for (uint32_t k = b * blockDim.x + threadIdx.x; k < nw + nt; k += grid.per_event * blockDim.x) {
+            if (k < nw)
+                e.s.start_bits[k] = 0ull;
+            else
+                e.s.dfs_agg[k - nw] = 0ull;
+        }
+    }
     for (;;) {
         const uint32_t t = take_ticket(&e.s.misc[kMiscTkFill]);
         if (t >= n_tiles) break;
@@ -911,6 +914,27 @@ __device__ __forceinline__ bool are_aligned(float r1, float z1, float ri, float 
     const float p_min = pt_min * sqrtf(d13);
     const float t = fabsf(z1 * (ri - ro) + zi * (ro - r1) + zo * (r1 - ri));
     return t * p_min <= theta_cut * d13 * radius_diff;
+}
+// The same test without the square root where that is safe: with every
+// quantity >= 0, t * (pt_min * sqrt(d13)) <= rhs squares to
+// (t * pt_min)^2 * d13 <= rhs^2.  Each side of the float expression above is
+// within (1 +- 2^-24)^3 of its real value and the squares here within
+// (1 +- 2^-24)^4 of theirs, so a squared comparison that clears a relative
+// margin of 16 * 2^-24 (rhs^2 normal) decides it exactly as the float
+// expression would.  Returns 1 (aligned), -1 (not aligned) or 0: inside the
+// margin, or out of range, where are_aligned decides.
+__device__ __forceinline__ int aligned_sq(float r1, float z1, float ri, float zi, float ro, float zo, float pt_min,
+                                          float theta_cut) {
+    const float radius_diff = fabsf(r1 - ro);
+    const float d13 = radius_diff * radius_diff + (z1 - zo) * (z1 - zo);
+    const float t = fabsf(z1 * (ri - ro) + zi * (ro - r1) + zo * (r1 - ri));
+    const float rhs = theta_cut * d13 * radius_diff;
+    const float a = t * pt_min;
+    const float lhs2 = (a * a) * d13, rhs2 = rhs * rhs;
+    const bool normal = rhs2 >= 0x1p-100f && rhs2 <= 0x1p100f;
+    if (normal && lhs2 * (1.0f + 0x1p-20f) < rhs2) return 1;
+    if (normal && lhs2 * (1.0f - 0x1p-20f) > rhs2) return -1;
+    return 0;
 }
 // circle_eq.jl:24-44,89-93 and gpu_ca_cell.jl:138-154
 __device__ __forceinline__ bool dca_cut(float x1, float y1, float x2, float y2, float x3, float y3, float region,
@@ -1152,10 +1176,22 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                             sl[k] = (j0 + (uint32_t)k < span) ? lo + j0 + (uint32_t)k : 0u;
                             o3[k] = S.czr[sl[k]];
                         }
+                        // the alignment cut squared; the rare pairs it leaves open (a
+                        // wavefront-uniform branch) take the reference's expression
+                        uint32_t open = 0;
 #pragma unroll
-                        for (int k = 0; k < kConnCols; ++k)
-                            al[k] = (j0 + (uint32_t)k < span) &
-                                    are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
+                        for (int k = 0; k < kConnCols; ++k) {
+                            const int d = aligned_sq(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
+                            const bool in = j0 + (uint32_t)k < span;
+                            al[k] = in & (d > 0);
+                            open |= (in && d == 0) ? 1u << k : 0u;
+                        }
+                        if (__ballot(open != 0u)) {
+#pragma unroll
+                            for (int k = 0; k < kConnCols; ++k)
+                                if ((open >> k) & 1u)
+                                    al[k] = are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
+                        }
 #pragma unroll
                         for (int k = 0; k < kConnCols; ++k) {
                             const uint64_t am = __ballot(al[k]);
@@ -2276,7 +2312,7 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     };
     const uint32_t xcd = cfg.xcd;
     const EvGrid one{n_events, 1u, xcd};
-    const EvGrid g_reset = ev_grid(max_hits + max_slots / kSlotTile + kDfsAgg + kMaxCells / 64u + 64u +
+    const EvGrid g_reset = ev_grid(max_hits + max_slots / kSlotTile + 64u +
                                        (zero_state ? 20u * kMaxTuples : 0u),
                                    n_events, xcd);
     k_reset<<<ev_grid_dim(g_reset), 256, 0, st>>>(d_evs, g_reset, zero_state);
