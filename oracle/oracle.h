@@ -68,6 +68,25 @@ int oracle_fit_grad_many(uint32_t n, const float *hits, const float *sig, const 
                          const float *rot, double b_field, const float *truth, double *loss, double *grad,
                          int n_threads);
 
+/* Riemann helix fit (riemann_fit.jl restated, see oracle_riemann.hpp for the
+ * statements it corrects) of pre-gathered tracks; same input layout and
+ * out64 layout as oracle_fit_bl (state, cov15, pt, eta, chi2 in the TrackSOA
+ * convention).  5-hit tracks are fitted on their first 4 hits. */
+int oracle_fit_riemann(uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
+                       double *out64, int n_threads);
+
+/* One Riemann fit with every intermediate (tests/oracle_layout.py unpacks it).
+ * hits: 3xN column-major doubles, ge: 6xN column-major floats, N = 3, 4 or 5. */
+int oracle_riemann_detail(int N, const double *hits, const float *ge, double B, double *out, int cap);
+
+/* Plants one mistake in the Riemann restatement (bit mask of
+ * oracle::riemann::kPlant*); 0 restores it.  Process-wide; tests only. */
+void oracle_plant_riemann(uint32_t mask);
+
+/* The symmetric 3x3 eigen-solver of the Riemann restatement: A row-major,
+ * eigenvalues w, eigenvectors in the columns of Q (row-major). */
+int oracle_sym_eigen3(const double *A, double *w, double *Q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,11 @@ def lib():
         L.oracle_plant_ms.restype = None
         L.oracle_plant_fishbone.argtypes = [C.c_int, C.c_int]
         L.oracle_plant_fishbone.restype = None
+        L.oracle_fit_riemann.argtypes = [C.c_uint32, vp, vp, vp, C.c_double, vp, C.c_int]
+        L.oracle_riemann_detail.argtypes = [C.c_int, vp, vp, C.c_double, vp, C.c_int]
+        L.oracle_plant_riemann.argtypes = [C.c_uint32]
+        L.oracle_plant_riemann.restype = None
+        L.oracle_sym_eigen3.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -158,3 +163,28 @@ def fit_bl(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):
                              flags.ctypes.data, n_threads)
     assert rc == 0
     return out, flags
+
+
+def fit_riemann(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):
+    """Oracle Riemann fit of pre-gathered tracks -> out64 [n, 23] (state, cov15, pt, eta, chi2)."""
+    n = int(nhit.shape[0])
+    out = np.zeros((n, 23), np.float64)
+    h = np.ascontiguousarray(hits, np.float32)
+    g = np.ascontiguousarray(ge, np.float32)
+    nh = np.ascontiguousarray(nhit, np.uint8)
+    rc = lib().oracle_fit_riemann(n, h.ctypes.data, g.ctypes.data, nh.ctypes.data, b_field, out.ctypes.data,
+                                  n_threads)
+    assert rc == 0
+    return out
+
+
+def plant_riemann(mask):
+    lib().oracle_plant_riemann(mask)
+
+
+def sym_eigen3(A3):
+    a = np.ascontiguousarray(A3, np.float64)
+    w = np.zeros(3)
+    q = np.zeros((3, 3))
+    lib().oracle_sym_eigen3(a.ctypes.data, w.ctypes.data, q.ctypes.data)
+    return w, q
