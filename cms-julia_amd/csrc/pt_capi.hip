@@ -199,9 +199,9 @@ struct pt_ctx {
     // fit-only batches: persistent-launch shape per precision, scratch for
     // the per-length lists (grown on demand) and the counters / ticket
     int n_cu = 256;
-    pt_fit_tuning tuning[3] = {{256, 512, 1024}, {256, 512, 1024}, {256, 256, 1024}};
+    pt_fit_tuning tuning[4] = {{256, 512, 1024}, {256, 512, 1024}, {256, 256, 1024}, {256, 512, 1024}};
     // schedule of pt_fit_bl_device_async (FP64, mixed) and of the adjoint
-    int schedule[3] = {PT_SCHED_PERSISTENT, PT_SCHED_PERSISTENT, PT_SCHED_PLAIN};
+    int schedule[4] = {PT_SCHED_PERSISTENT, PT_SCHED_PERSISTENT, PT_SCHED_PLAIN, PT_SCHED_PERSISTENT};
     uint32_t *fit_lists = nullptr;
     size_t fit_lists_cap = 0;  // track ids
     uint32_t *fit_counts = nullptr;
@@ -323,7 +323,7 @@ int pt_ctx_create(pt_ctx **out, const pt_params *p, int device, uint32_t max_eve
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
             c->n_cu = ncu;
-        for (int k = 0; k < 3; ++k) c->tuning[k].blocks = 4u * (uint32_t)c->n_cu;
+        for (int k = 0; k < 4; ++k) c->tuning[k].blocks = 4u * (uint32_t)c->n_cu;
     }
     std::memset(c->h_cnt, 0, sizeof(pt_counters) * max_events);
     for (int k = 0; k < pt_ctx::kDescRing; ++k)
@@ -1021,7 +1021,8 @@ static int fit_flat(pt_ctx *c, uint32_t n, const float *hits, const float *ge, c
                     float *eta, float *chi2) {
     if (!c || (n && (!hits || !ge || !nhit || !state || !cov || !pt || !eta || !chi2)))
         return fail(PT_ERR_ARG, "null pointer");
-    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_RIEMANN)
+        return fail(PT_ERR_ARG, "unknown precision");
     if (schedule != PT_SCHED_PLAIN && schedule != PT_SCHED_PER_LENGTH && schedule != PT_SCHED_PERSISTENT)
         return fail(PT_ERR_ARG, "unknown schedule");
     if (n > (1u << 30)) return fail(PT_ERR_ARG, "n_tracks above 2^30");
@@ -1051,6 +1052,7 @@ int pt_fit_bl_device_async(pt_ctx *c, uint32_t n, const float *hits, const float
                            double b_field, int precision, float *state, float *cov, float *pt, float *eta,
                            float *chi2) {
     if (!c) return fail(PT_ERR_ARG, "null context");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
     const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
     return fit_flat(c, n, hits, ge, nhit, b_field, precision, c->schedule[pi], c->tuning[pi], state, cov, pt, eta,
                     chi2);
@@ -1060,6 +1062,7 @@ int pt_fit_bl_sched_async(pt_ctx *c, uint32_t n, const float *hits, const float 
                           double b_field, int precision, int schedule, float *state, float *cov, float *pt,
                           float *eta, float *chi2) {
     if (!c) return fail(PT_ERR_ARG, "null context");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
     const int pi = (precision == PT_FIT_MIXED) ? 1 : 0;
     return fit_flat(c, n, hits, ge, nhit, b_field, precision, schedule, c->tuning[pi], state, cov, pt, eta, chi2);
 }
@@ -1094,9 +1097,25 @@ int pt_fit_bl_sched_noalloc_async(pt_ctx *c, uint32_t n, const float *hits, cons
     return PT_OK;
 }
 
+int pt_fit_riemann_device_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                                double b_field, float *state, float *cov, float *pt, float *eta, float *chi2) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    return fit_flat(c, n, hits, ge, nhit, b_field, PT_FIT_RIEMANN, c->schedule[PT_FIT_RIEMANN],
+                    c->tuning[PT_FIT_RIEMANN], state, cov, pt, eta, chi2);
+}
+
+int pt_fit_riemann_sched_async(pt_ctx *c, uint32_t n, const float *hits, const float *ge, const uint8_t *nhit,
+                               double b_field, int schedule, float *state, float *cov, float *pt, float *eta,
+                               float *chi2) {
+    if (!c) return fail(PT_ERR_ARG, "null context");
+    return fit_flat(c, n, hits, ge, nhit, b_field, PT_FIT_RIEMANN, schedule, c->tuning[PT_FIT_RIEMANN], state, cov,
+                    pt, eta, chi2);
+}
+
 int pt_fit_set_tuning(pt_ctx *c, int precision, const pt_fit_tuning *t) {
     if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
-    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT &&
+        precision != PT_FIT_RIEMANN)
         return fail(PT_ERR_ARG, "unknown precision");
     int rc = check_tuning(*t);
     if (rc) return rc;
@@ -1106,7 +1125,8 @@ int pt_fit_set_tuning(pt_ctx *c, int precision, const pt_fit_tuning *t) {
 
 int pt_fit_set_schedule(pt_ctx *c, int precision, int schedule) {
     if (!c) return fail(PT_ERR_ARG, "null context");
-    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT &&
+        precision != PT_FIT_RIEMANN)
         return fail(PT_ERR_ARG, "unknown precision");
     if (schedule != PT_SCHED_PLAIN && schedule != PT_SCHED_PER_LENGTH && schedule != PT_SCHED_PERSISTENT)
         return fail(PT_ERR_ARG, "unknown schedule");
@@ -1116,7 +1136,8 @@ int pt_fit_set_schedule(pt_ctx *c, int precision, int schedule) {
 
 int pt_fit_get_tuning(pt_ctx *c, int precision, pt_fit_tuning *t) {
     if (!c || !t) return fail(PT_ERR_ARG, "null pointer");
-    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT)
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_ADJOINT &&
+        precision != PT_FIT_RIEMANN)
         return fail(PT_ERR_ARG, "unknown precision");
     *t = c->tuning[precision];
     return PT_OK;
@@ -1182,14 +1203,16 @@ int pt_fit_autotune(pt_ctx *c, uint32_t n, const float *hits, const float *ge, c
                     int precision, float *state, float *cov, float *pt, float *eta, float *chi2,
                     pt_fit_tuning *best, float *ms) {
     if (!c) return fail(PT_ERR_ARG, "null context");
-    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED) return fail(PT_ERR_ARG, "unknown precision");
+    if (precision != PT_FIT_FP64 && precision != PT_FIT_MIXED && precision != PT_FIT_RIEMANN)
+        return fail(PT_ERR_ARG, "unknown precision");
     if (n == 0) return fail(PT_ERR_ARG, "empty batch");
+    const int slot = precision == PT_FIT_MIXED ? 1 : (precision == PT_FIT_RIEMANN ? PT_FIT_RIEMANN : 0);
     HIP_OK(hipSetDevice(c->device));
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     const uint32_t blocks_[3] = {64, 128, 256}, tiles[4] = {128, 256, 512, 1024}, per_cu[4] = {1, 2, 4, 8};
-    pt_fit_tuning top = c->tuning[precision == PT_FIT_MIXED ? 1 : 0];
+    pt_fit_tuning top = c->tuning[slot];
     float top_ms = 1e30f;
     int rc = PT_OK;
     for (uint32_t bs : blocks_)
@@ -1221,7 +1244,7 @@ int pt_fit_autotune(pt_ctx *c, uint32_t n, const float *hits, const float *ge, c
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != PT_OK) return rc;
-    c->tuning[precision == PT_FIT_MIXED ? 1 : 0] = top;
+    c->tuning[slot] = top;
     if (best) *best = top;
     if (ms) *ms = top_ms;
     return PT_OK;

@@ -7,7 +7,9 @@
 //   classify each fitted tuple (the first step of classify_tuples).
 //   Gather per hit: global position, module frame, local variances rotated to
 //   the global covariance in float (SOARotation.jl:139-147).
-// k_fit_flat: fit of pre-gathered tracks (fit-only workloads).
+// k_fit_flat: fit of pre-gathered tracks (fit-only workloads), broken-line or
+//   Riemann (pt_riemann.hpp; the counterpart of launch_riemann_kernels!,
+//   helix_fit_on_gpu.jl:94-96).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +18,7 @@
 #include "pt_bl_adjoint.hpp"
 #include "pt_bl_fit.hpp"
 #include "pt_device.hpp"
+#include "pt_riemann.hpp"
 
 namespace pt {
 
@@ -166,6 +169,44 @@ template <class P> struct FlatJob {
         for (int k = 0; k < 5; ++k) a.state[(size_t)k * n + t] = (float)o.state[k];
 #pragma unroll
         for (int k = 0; k < 15; ++k) a.cov[(size_t)k * n + t] = (float)o.cov[k];
+        a.pt[t] = (float)o.pt;
+        a.eta[t] = (float)o.eta;
+        a.chi2[t] = (float)o.chi2;
+    }
+    __device__ __forceinline__ void invalid(uint32_t t) const {
+        const float nan = __builtin_nanf("");
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a.state[(size_t)k * a.n + t] = nan;
+        a.pt[t] = nan;
+        a.eta[t] = nan;
+        a.chi2[t] = nan;
+    }
+};
+
+// Riemann helix fit of a pre-gathered track (FP64), same input and output
+// layout as FlatJob.
+struct RiemannJob {
+    static constexpr bool kOutOfLine = false;
+    FlatArgs a;
+    __device__ __forceinline__ uint32_t size() const { return a.n; }
+    __device__ __forceinline__ int cls(uint32_t t) const { return fit_class(a.nhit[t]); }
+    template <int N> __device__ __forceinline__ void one(uint32_t t) const {
+        const size_t n = a.n;
+        double x[N], y[N], z[N], g[6][N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            x[k] = a.hits[(size_t)(0 * 5 + k) * n + t];
+            y[k] = a.hits[(size_t)(1 * 5 + k) * n + t];
+            z[k] = a.hits[(size_t)(2 * 5 + k) * n + t];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) g[c][k] = a.ge[(size_t)(c * 5 + k) * n + t];
+        }
+        riemann::RiemannOut<N> o;
+        riemann::riemann_fit<N>(x, y, z, g, a.B, o);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a.state[(size_t)k * n + t] = (float)o.state[k];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) a.cov[(size_t)k * n + t] = (float)o.cov15[k];
         a.pt[t] = (float)o.pt;
         a.eta[t] = (float)o.eta;
         a.chi2[t] = (float)o.chi2;
@@ -430,7 +471,9 @@ static void launch_job(const J &j, uint32_t n, const FitLaunch &L, hipStream_t s
 }
 
 void launch_fit_flat(const FlatArgs &a, int precision, const FitLaunch &L, hipStream_t st) {
-    if (precision == PT_FIT_MIXED)
+    if (precision == PT_FIT_RIEMANN)
+        launch_job(RiemannJob{a}, a.n, L, st);
+    else if (precision == PT_FIT_MIXED)
         launch_job(FlatJob<PolicyMixed>{a}, a.n, L, st);
     else
         launch_job(FlatJob<PolicyFP64>{a}, a.n, L, st);

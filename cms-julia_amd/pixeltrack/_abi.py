@@ -47,7 +47,7 @@ REF = {
 }
 INT = {"LOOKBACK": 0x001, "NBR_POOL": 0x002, "OVF_LIST": 0x004, "ROUNDS": 0x008, "LIST_SHARE": 0x010}
 QUALITY_BAD, QUALITY_DUP, QUALITY_LOOSE = 0, 1, 2
-FIT_FP64, FIT_MIXED, FIT_ADJOINT = 0, 1, 2
+FIT_FP64, FIT_MIXED, FIT_ADJOINT, FIT_RIEMANN = 0, 1, 2, 3
 SCHED_PLAIN, SCHED_PER_LENGTH, SCHED_PERSISTENT = 0, 1, 2
 
 # bf used by CAHitNtuplet.produce (ca_hit_ntuplet.jl:11)
@@ -210,6 +210,7 @@ HIP_SYMBOLS = [
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
     "pt_set_hits_layer_start", "pt_load_tracks_async", "pt_fit_bl_sched_noalloc_async",
     "pt_fit_set_schedule", "pt_ca_set_tuning", "pt_ca_get_tuning", "pt_ca_autotune",
+    "pt_fit_riemann_device_async", "pt_fit_riemann_sched_async",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -258,6 +259,9 @@ def hip_lib():
         lib.pt_ctx_set_streams.argtypes = [vp, C.c_int]
         lib.pt_get_stage_times.argtypes = [vp, c_f32p, C.POINTER(C.c_char_p), C.POINTER(C.c_int)]
         lib.pt_fit_bl_device_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
+        lib.pt_fit_riemann_device_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, vp, vp, vp, vp, vp]
+        lib.pt_fit_riemann_sched_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp, vp,
+                                                   vp]
         lib.pt_fit_bl_sched_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
                                               vp, vp]
         lib.pt_fit_bl_sched_noalloc_async.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp,

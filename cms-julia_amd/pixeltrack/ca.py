@@ -431,6 +431,17 @@ class FitBatch:
                                                        b_field, precision, schedule, *self._outs()),
                     "pt_fit_bl_sched_async")
 
+    def run_riemann_async(self, schedule=None, b_field=A.B_FIELD_INV_GEV):
+        """The Riemann fit (pt_fit_riemann_*; launch_riemann_kernels!, helix_fit_on_gpu.jl:94-96)."""
+        if schedule is None:
+            A.check(self.ctx.lib.pt_fit_riemann_device_async(self.ctx.h, self.n, self.d_hits, self.d_ge,
+                                                             self.d_nhit, b_field, *self._outs()),
+                    "pt_fit_riemann_device_async")
+        else:
+            A.check(self.ctx.lib.pt_fit_riemann_sched_async(self.ctx.h, self.n, self.d_hits, self.d_ge,
+                                                            self.d_nhit, b_field, schedule, *self._outs()),
+                    "pt_fit_riemann_sched_async")
+
     def autotune(self, precision=A.FIT_FP64, b_field=A.B_FIELD_INV_GEV):
         best = A.PtFitTuning()
         ms = C.c_float()
@@ -511,13 +522,24 @@ class AdjointBatch:
 
 
 def set_fit_schedule(ctx: Context, precision, schedule):
-    """Schedule of pt_fit_bl_device_async (FIT_FP64 / FIT_MIXED) or of the adjoint (FIT_ADJOINT)."""
+    """Schedule of pt_fit_bl_device_async (FIT_FP64 / FIT_MIXED), of the adjoint (FIT_ADJOINT) or of
+    pt_fit_riemann_device_async (FIT_RIEMANN)."""
     A.check(ctx.lib.pt_fit_set_schedule(ctx.h, precision, schedule), "pt_fit_set_schedule")
 
 
 def set_fit_tuning(ctx: Context, precision, block, tile, blocks):
     t = A.PtFitTuning(block, tile, blocks)
     A.check(ctx.lib.pt_fit_set_tuning(ctx.h, precision, C.byref(t)), "pt_fit_set_tuning")
+
+
+def fit_riemann(ctx: Context, hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, schedule=None):
+    """Riemann fit of pre-gathered tracks on the GPU; returns host arrays."""
+    fb = FitBatch(ctx, hits, ge, nhit)
+    try:
+        fb.run_riemann_async(schedule, b_field)
+        return fb.download()
+    finally:
+        fb.free()
 
 
 def fit_broken_line(ctx: Context, hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, precision=A.FIT_FP64, schedule=None):

@@ -381,8 +381,9 @@ int pt_fit_bl_sched_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, con
 int pt_fit_bl_sched_noalloc_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
                                   const uint8_t *nhit, double b_field, int precision, int schedule,
                                   float *state, float *cov, float *pt, float *eta, float *chi2);
-/* Tuning of the persistent schedule per precision (PT_FIT_FP64, PT_FIT_MIXED
- * or PT_FIT_ADJOINT; default: 256 threads, 512-track tiles, 4 blocks per CU). */
+/* Tuning of the persistent schedule per precision (PT_FIT_FP64, PT_FIT_MIXED,
+ * PT_FIT_ADJOINT or PT_FIT_RIEMANN; default: 256 threads, 512-track tiles,
+ * 4 blocks per CU). */
 int pt_fit_set_tuning(pt_ctx *ctx, int precision, const pt_fit_tuning *t);
 /* Schedule used by pt_fit_bl_device_async for PT_FIT_FP64 / PT_FIT_MIXED
  * (default PT_SCHED_PERSISTENT) and by the adjoint entry points for
@@ -393,11 +394,35 @@ int pt_fit_set_schedule(pt_ctx *ctx, int precision, int schedule);
 int pt_fit_get_tuning(pt_ctx *ctx, int precision, pt_fit_tuning *t);
 /* Times every candidate launch shape of the persistent schedule on the given
  * batch (device pointers; outputs are overwritten), keeps the fastest in the
- * context and returns it.  *ms: its time per launch.  precision: PT_FIT_FP64
- * or PT_FIT_MIXED. */
+ * context and returns it.  *ms: its time per launch.  precision: PT_FIT_FP64,
+ * PT_FIT_MIXED or PT_FIT_RIEMANN (the Riemann fit's slot). */
 int pt_fit_autotune(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit,
                     double b_field, int precision, float *state, float *cov, float *pt, float *eta,
                     float *chi2, pt_fit_tuning *best, float *ms);
+
+/* ---------------------------------------------------------------------------
+ * Riemann helix fit of pre-gathered tracks (launch_riemann_kernels!,
+ * helix_fit_on_gpu.jl:94-96, which only prints; the method is helix_fit of
+ * riemann_fit.jl:989-1023: fast_fit, circle_fit on the paraboloid with its
+ * error propagation, line_fit in the rotated s-z plane), FP64.  The
+ * reference's make_tuples never runs it (use_riemann_fit is not read,
+ * ca_hit_ntuplet_generator.jl:83-93), so the stage keeps the broken-line fit
+ * and this is a fit-only entry point.  Same layout as pt_fit_bl_device_async;
+ * outputs in the TrackSOA convention (perigee phi, tip, q/pT, cot(theta), zip
+ * through fromCircleToPerigee and copyFromCircle!, fit_utils.jl:115-141,
+ * TrajectoryStateSOA.jl:20-44); chi2 = (chi2_circle + chi2_line) / (2N - 5);
+ * 5-hit tracks are fitted on their first 4 hits.  PT_FIT_RIEMANN is its slot
+ * in pt_fit_set_tuning / pt_fit_set_schedule / pt_fit_autotune (default:
+ * persistent schedule).
+ * ------------------------------------------------------------------------- */
+#define PT_FIT_RIEMANN 3
+int pt_fit_riemann_device_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
+                                const uint8_t *nhit, double b_field, float *state, float *cov, float *pt, float *eta,
+                                float *chi2);
+/* The same under a given schedule (PT_SCHED_*; no reference counterpart). */
+int pt_fit_riemann_sched_async(pt_ctx *ctx, uint32_t n_tracks, const float *hits, const float *ge,
+                               const uint8_t *nhit, double b_field, int schedule, float *state, float *cov, float *pt,
+                               float *eta, float *chi2);
 
 /* ---------------------------------------------------------------------------
  * Adjoint of the mixed-precision broken-line fit (no reference counterpart:

@@ -75,6 +75,12 @@ int oracle_fit_grad_many(uint32_t n, const float *hits, const float *sig, const 
 int oracle_fit_riemann(uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
                        double *out64, int n_threads);
 
+/* oracle_fit_riemann with the hit positions multiplied by `scale` in double
+ * after the gather: scale = 1 +- 2^-52 measures how much a one-ulp change of
+ * the input moves each output (the conditioning of the method per track). */
+int oracle_fit_riemann_scaled(uint32_t n_tracks, const float *hits, const float *ge, const uint8_t *nhit,
+                              double b_field, double scale, double *out64, int n_threads);
+
 /* One Riemann fit with every intermediate (tests/oracle_layout.py unpacks it).
  * hits: 3xN column-major doubles, ge: 6xN column-major floats, N = 3, 4 or 5. */
 int oracle_riemann_detail(int N, const double *hits, const float *ge, double B, double *out, int cap);

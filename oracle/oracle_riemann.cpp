@@ -21,9 +21,13 @@ template <int N> void gather(const float *hits, const float *ge, uint32_t n, uin
     }
 }
 
-template <int N> void fit_one(const float *hits, const float *ge, uint32_t n, uint32_t t, double B, double *o) {
+template <int N>
+void fit_one(const float *hits, const float *ge, uint32_t n, uint32_t t, double B, double scale, double *o) {
     double h[3][N], g[6][N];
     gather<N>(hits, ge, n, t, h, g);
+    if (scale != 1.0)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < N; ++k) h[c][k] = h[c][k] * scale;
     RiemannTrack<N> r;
     oracle::riemann::helix_fit<N>(h, g, B, r);
     for (int k = 0; k < 5; ++k) o[k] = r.state[k];
@@ -90,13 +94,18 @@ extern "C" {
 
 int oracle_fit_riemann(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
                        double *out64, int n_threads) {
+    return oracle_fit_riemann_scaled(n, hits, ge, nhit, b_field, 1.0, out64, n_threads);
+}
+
+int oracle_fit_riemann_scaled(uint32_t n, const float *hits, const float *ge, const uint8_t *nhit, double b_field,
+                              double scale, double *out64, int n_threads) {
     const double B = (double)(float)b_field;
     auto work = [&](uint32_t t0, uint32_t t1) {
         for (uint32_t t = t0; t < t1; ++t) {
             if (nhit[t] == 3)
-                fit_one<3>(hits, ge, n, t, B, out64 + 23 * (size_t)t);
+                fit_one<3>(hits, ge, n, t, B, scale, out64 + 23 * (size_t)t);
             else
-                fit_one<4>(hits, ge, n, t, B, out64 + 23 * (size_t)t);  // 4, and 5 fitted as 4
+                fit_one<4>(hits, ge, n, t, B, scale, out64 + 23 * (size_t)t);  // 4, and 5 fitted as 4
         }
     };
     if (n_threads <= 1) {

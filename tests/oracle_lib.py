@@ -43,6 +43,7 @@ def lib():
         L.oracle_plant_fishbone.argtypes = [C.c_int, C.c_int]
         L.oracle_plant_fishbone.restype = None
         L.oracle_fit_riemann.argtypes = [C.c_uint32, vp, vp, vp, C.c_double, vp, C.c_int]
+        L.oracle_fit_riemann_scaled.argtypes = [C.c_uint32, vp, vp, vp, C.c_double, C.c_double, vp, C.c_int]
         L.oracle_riemann_detail.argtypes = [C.c_int, vp, vp, C.c_double, vp, C.c_int]
         L.oracle_plant_riemann.argtypes = [C.c_uint32]
         L.oracle_plant_riemann.restype = None
@@ -176,6 +177,24 @@ def fit_riemann(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):
                                   n_threads)
     assert rc == 0
     return out
+
+
+def fit_riemann_conditioning(hits, ge, nhit, b_field=A.B_FIELD_INV_GEV, n_threads=1):
+    """Per track and output, the larger change of oracle_fit_riemann when the
+    hit positions are scaled by 1 + 2^-52 or 1 - 2^-52 (one ulp): how far the
+    method itself moves each output under a rounding-size change of its input."""
+    n = int(nhit.shape[0])
+    h = np.ascontiguousarray(hits, np.float32)
+    g = np.ascontiguousarray(ge, np.float32)
+    nh = np.ascontiguousarray(nhit, np.uint8)
+    outs = []
+    for sc in (1.0, 1.0 + 2.0 ** -52, 1.0 - 2.0 ** -53):
+        o = np.zeros((n, 23), np.float64)
+        rc = lib().oracle_fit_riemann_scaled(n, h.ctypes.data, g.ctypes.data, nh.ctypes.data, b_field, sc,
+                                             o.ctypes.data, n_threads)
+        assert rc == 0
+        outs.append(o)
+    return outs[0], np.maximum(np.abs(outs[1] - outs[0]), np.abs(outs[2] - outs[0]))
 
 
 def plant_riemann(mask):
