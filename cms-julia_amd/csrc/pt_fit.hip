@@ -192,7 +192,7 @@ struct RiemannJob {
     __device__ __forceinline__ int cls(uint32_t t) const { return fit_class(a.nhit[t]); }
     template <int N> __device__ __forceinline__ void one(uint32_t t) const {
         const size_t n = a.n;
-        double x[N], y[N], z[N], g[6][N];
+        float x[N], y[N], z[N], g[6][N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             x[k] = a.hits[(size_t)(0 * 5 + k) * n + t];

@@ -141,14 +141,214 @@ template <int N> struct RiemannOut {
     double pt, eta, chi2;
 };
 
+// The line fit (riemann_fit.jl:841-955), the perigee form (fit_utils.jl:115-141)
+// and the TrackSOA output (TrajectoryStateSOA.jl:20-44), given the circle
+// (X0, Y0, R, q) and its covariance.  Called out of line: inlined into the
+// circle fit, the per-length N = 4 kernel of the fit-only schedules gave line
+// results (cot(theta), zip, their covariance, chi2) different from the same
+// code in the plain and persistent kernels and from the oracle, with the
+// circle bit-identical.
 template <int N>
-PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double (&z)[N], const double (&ge)[6][N],
+__host__ __device__ __attribute__((noinline)) void riemann_line_and_output(
+    const float (&x)[N], const float (&y)[N], const float (&z)[N], const float (&ge)[6][N], const double (&ff)[4],
+    double X0, double Y0, double Rc, int q, const double (&ccov)[3][3], double chi2_circle, double B,
+    RiemannOut<N> &out) {
+    auto X = [&](int i) -> double { return (double)x[i]; };
+    auto Y = [&](int i) -> double { return (double)y[i]; };
+    auto Z = [&](int i) -> double { return (double)z[i]; };
+    auto GE = [&](int c, int i) -> double { return (double)ge[c][i]; };
+    // ---------------------------------------------------------------- line fit (:841-955)
+    double theta = -q * atan(ff[3]);
+    theta = theta < 0 ? theta + kPiD : theta;
+    const double sn = sin(theta), cs = cos(theta);
+    double sa[N], czz[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double p0 = X(i) - X0, p1 = Y(i) - Y0;
+        const double cross = (-X0) * p1 - (-Y0) * p0;
+        const double dot = (-X0) * p0 + (-Y0) * p1;
+        const double at = -q * atan2_bl(cross, dot);
+        sa[i] = at * Rc;
+        const double temp0 = -q * Rc * 1 / (sqr(dot) + sqr(cross));
+        const double jc[3] = {-temp0 * ((p1 + Y0) * dot - (p0 - X0) * cross),
+                              temp0 * ((p0 + X0) * dot - (Y0 - p1) * cross), at};
+        const double dx = temp0 * (Y0 * dot + X0 * cross);
+        const double dy = temp0 * (-X0 * dot + Y0 * cross);
+        // Jx Cov Jx' with Cov = diag(circle cov, hit cov), then the z'z' entry of rot (.) rot'
+        double t00 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n2 = 0; n2 < 3; ++n2) t00 += jc[m] * ccov[m][n2] * jc[n2];
+        t00 += dx * (GE(0, i) * dx + GE(1, i) * dy) + dy * (GE(1, i) * dx + GE(2, i) * dy);
+        const double t01 = dx * GE(3, i) + dy * GE(4, i);
+        const double t11 = GE(5, i);
+        czz[i] = cs * cs * t00 - 2.0 * cs * sn * t01 + sn * sn * t11;
+    }
+    double cm[N][N];
+    {
+        const double p_t = jl_min(20.0, ff[2] * B);
+        const double p_2 = p_t * p_t * (1.0 + 1.0 / (ff[3] * ff[3]));
+        double S[N], rl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            S[i] = sqrt(sa[i] * sa[i] + Z(i) * Z(i));
+            rl[i] = S[i];
+        }
+        rad_len_uniform<N>(rl);
+        double sig2[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double tt = 1.0 + 0.038 * log(rl[i]);
+            sig2[i] = 0.000225 / p_2 * (tt * tt) * rl[i];
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int l = k; l < N; ++l) {
+                double acc = k == l ? czz[k] : 0.0;
+#pragma unroll
+                for (int i = 0; i <= (k < l ? k : l); ++i) acc += fabs(S[k] - S[i]) * fabs(S[l] - S[i]) * sig2[i];
+                cm[k][l] = acc;
+                cm[l][k] = acc;
+            }
+    }
+    double xr[N], yr[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        xr[i] = sn * sa[i] + cs * Z(i);
+        yr[i] = -cs * sa[i] + sn * Z(i);
+    }
+    double Vy[N][N];
+    invert_n(cm, Vy);
+    double av0[N], av1[N];  // rows of A * Vy
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            a0 += Vy[i][j];
+            a1 += xr[i] * Vy[i][j];
+        }
+        av0[j] = a0;
+        av1[j] = a1;
+    }
+    double cp00 = 0.0, cp01 = 0.0, cp11 = 0.0, r0s = 0.0, r1s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        cp00 += av0[j];
+        cp01 += av0[j] * xr[j];
+        cp11 += av1[j] * xr[j];
+        r0s += av0[j] * yr[j];
+        r1s += av1[j] * yr[j];
+    }
+    // invert22 (cholesky_inversion.jl:8-19)
+    const double luc0 = 1.0 / cp00;
+    const double luc1 = cp01 * cp01 * luc0;
+    const double luc2 = 1.0 / (cp11 - luc1);
+    const double ci00 = luc1 * luc0 * luc2 + luc0, ci01 = -cp01 * luc0 * luc2, ci11 = luc2;
+    const double sol0 = ci00 * r0s + ci01 * r1s, sol1 = ci01 * r0s + ci11 * r1s;
+    const double cf = 1 / (sn - sol1 * cs);
+    const double Jl[2][2] = {{0.0, cf * cf}, {cf, sol0 * cs * cf * cf}};
+    const double m_ = cf * (sol1 * sn + cs);
+    const double q_ = cf * sol0;
+    const double Ci[2][2] = {{ci00, ci01}, {ci01, ci11}};
+    double lcov[2][2];
+#pragma unroll
+    for (int xk = 0; xk < 2; ++xk)
+#pragma unroll
+        for (int yk = 0; yk < 2; ++yk) {
+            double acc = 0.0;
+#pragma unroll
+            for (int zk = 0; zk < 2; ++zk) acc += (Jl[xk][0] * Ci[0][zk] + Jl[xk][1] * Ci[1][zk]) * Jl[yk][zk];
+            lcov[xk][yk] = acc;
+        }
+    double chi2_line = 0.0;
+    {
+        double res[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) res[i] = yr[i] - (sol0 + xr[i] * sol1);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc += res[j] * Vy[j][i];
+            chi2_line += acc * res[i];
+        }
+    }
+
+    // ---------------------------------------------------------------- perigee and TrackSOA
+    const double temp0 = X0 * X0 + Y0 * Y0;
+    const double temp1 = sqrt(temp0);
+    const double kp[3] = {atan2(q * X0, -q * Y0), q * (temp1 - Rc), q / Rc};
+    const double temp2 = sqr(X0) * 1.0 / temp0;
+    const double temp3 = 1.0 / temp1 * q;
+    const double J4[3][3] = {{-Y0 * temp2 / (X0 * X0), temp2 / X0, 0.0},
+                             {X0 * temp3, Y0 * temp3, (double)-q},
+                             {0.0, 0.0, -q / (Rc * Rc)}};
+    double kc[3][3];
+#pragma unroll
+    for (int xk = 0; xk < 3; ++xk)
+#pragma unroll
+        for (int yk = 0; yk < 3; ++yk) {
+            double acc = 0.0;
+#pragma unroll
+            for (int zk = 0; zk < 3; ++zk) {
+                double jc = 0.0;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) jc += J4[xk][m] * ccov[m][zk];
+                acc += jc * J4[yk][zk];
+            }
+            kc[xk][yk] = acc;
+        }
+    const double bi = 1.0 / B;
+    out.state[0] = kp[0];
+    out.state[1] = kp[1];
+    out.state[2] = kp[2] * bi;
+    out.state[3] = m_;
+    out.state[4] = q_;
+    out.cov15[0] = kc[0][0];
+    out.cov15[1] = kc[0][1];
+    out.cov15[2] = bi * kc[0][2];
+    out.cov15[3] = 0.0;
+    out.cov15[4] = 0.0;
+    out.cov15[5] = kc[1][1];
+    out.cov15[6] = bi * kc[1][2];
+    out.cov15[7] = 0.0;
+    out.cov15[8] = 0.0;
+    out.cov15[9] = (bi * bi) * kc[2][2];
+    out.cov15[10] = 0.0;
+    out.cov15[11] = 0.0;
+    out.cov15[12] = lcov[0][0];
+    out.cov15[13] = lcov[0][1];
+    out.cov15[14] = lcov[1][1];
+    out.pt = B / fabs(kp[2]);
+    out.eta = asinh(m_);
+    out.chi2 = (chi2_circle + chi2_line) / double(2 * N - 5);
+}
+
+template <int N>
+PT_HD void riemann_fit(const float (&x)[N], const float (&y)[N], const float (&z)[N], const float (&ge)[6][N],
                        double B, RiemannOut<N> &out) {
+    // the inputs stay float (they arrive as float) and are widened where used
+    auto X = [&](int i) -> double { return (double)x[i]; };
+    auto Y = [&](int i) -> double { return (double)y[i]; };
+    auto Z = [&](int i) -> double { return (double)z[i]; };
+    auto GE = [&](int c, int i) -> double { return (double)ge[c][i]; };
     double ff[4];
-    fast_fit<N>(x, y, z, ff);
+    {
+        double xd[N], yd[N], zd[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            xd[i] = X(i);
+            yd[i] = Y(i);
+            zd[i] = Z(i);
+        }
+        fast_fit<N>(xd, yd, zd, ff);
+    }
     double rad[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) rad[i] = sqrt(x[i] * x[i] + y[i] * y[i]);
+    for (int i = 0; i < N; ++i) rad[i] = sqrt(X(i) * X(i) + Y(i) * Y(i));
 
     // ---------------------------------------------------------------- circle weights
     // scattering in radial coordinates, scatter_cov_rad (:112-150)
@@ -162,7 +362,7 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
         double len[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const double p0 = x[i] - ff[0], p1 = y[i] - ff[1];
+            const double p0 = X(i) - ff[0], p1 = Y(i) - ff[1];
             const float cr = (float)((-ff[0]) * p1 - (-ff[1]) * p0);
             const float dt = (float)((-ff[0]) * p0 + (-ff[1]) * p1);
             len[i] = fabs((double)atan2_bl(cr, dt) * ff[2]) * fac;
@@ -196,20 +396,20 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
         const double crv = ms[lo][hi];
         double e;
         if (!ay && !by)
-            e = crv * y[lo] * (1.0 / rad[lo]) * y[hi] * (1.0 / rad[hi]);
+            e = crv * Y(lo) * (1.0 / rad[lo]) * Y(hi) * (1.0 / rad[hi]);
         else if (ay && by)
-            e = crv * x[lo] * (1.0 / rad[lo]) * x[hi] * (1.0 / rad[hi]);
+            e = crv * X(lo) * (1.0 / rad[lo]) * X(hi) * (1.0 / rad[hi]);
         else {
             // x index ix, y index iy: cc[ix][iy+N] = -cr * y_ix / r_ix * x_iy / r_iy when ix <= iy,
             // cc[iy+N][ix] = -cr * x_iy / r_iy * y_ix / r_ix when iy < ix (mirrored)
             const int ix = ay ? j : i, iy = ay ? i : j;
             if (ix <= iy)
-                e = -crv * y[ix] * (1.0 / rad[ix]) * x[iy] * (1.0 / rad[iy]);
+                e = -crv * Y(ix) * (1.0 / rad[ix]) * X(iy) * (1.0 / rad[iy]);
             else
-                e = -crv * x[iy] * (1.0 / rad[iy]) * y[ix] * (1.0 / rad[ix]);
+                e = -crv * X(iy) * (1.0 / rad[iy]) * Y(ix) * (1.0 / rad[ix]);
         }
         double hcv = 0.0;
-        if (i == j) hcv = (!ay && !by) ? ge[0][i] : ((ay && by) ? ge[2][i] : ge[1][i]);
+        if (i == j) hcv = (!ay && !by) ? GE(0, i) : ((ay && by) ? GE(2, i) : GE(1, i));
         return hcv + e;
     };
 
@@ -219,14 +419,14 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
     for (int i = 0; i < N; ++i) {
         double d;
         if (rad[i] < 0.0001) {
-            d = ge[0][i];
+            d = GE(0, i);
         } else {
-            const double b0 = x[i] - ff[0], b1 = y[i] - ff[1];
-            const double x2 = x[i] * b0 + y[i] * b1;
-            const double y2 = x[i] * b1 - y[i] * b0;
+            const double b0 = X(i) - ff[0], b1 = Y(i) - ff[1];
+            const double x2 = X(i) * b0 + Y(i) * b1;
+            const double y2 = X(i) * b1 - Y(i) * b0;
             const double tc = -y2 / x2;
             const double tc2 = tc * tc;
-            d = 1.0 / (1.0 + tc2) * (ge[0][i] + ge[2][i] * tc2 + 2 * ge[1][i] * tc);
+            d = 1.0 / (1.0 + tc2) * (GE(0, i) + GE(2, i) * tc2 + 2 * GE(1, i) * tc);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) cr[i][j] = (i == j ? d : 0.0) + ms[i][j];
@@ -253,8 +453,8 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
     double hx = 0.0, hy = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        hx += x[i];
-        hy += y[i];
+        hx += X(i);
+        hy += Y(i);
     }
     hx = hx / N;
     hy = hy / N;
@@ -262,8 +462,8 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
     double qn = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        mcx[i] = x[i] - hx;
-        mcy[i] = y[i] - hy;
+        mcx[i] = X(i) - hx;
+        mcy[i] = Y(i) - hy;
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) qn += mcx[i] * mcx[i];
@@ -287,11 +487,11 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
         for (int j = 0; j < N; ++j) acc += P[k][j] * w[j];
         r0[k] = acc;
     }
-    double X[3][N];
+    double Xc[3][N];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int j = 0; j < N; ++j) X[k][j] = P[k][j] - r0[k];
+        for (int j = 0; j < N; ++j) Xc[k][j] = P[k][j] - r0[k];
     double A[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -302,8 +502,8 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
             for (int j = 0; j < N; ++j) {
                 double xg = 0.0;
 #pragma unroll
-                for (int i = 0; i < N; ++i) xg += X[k][i] * G[i][j];
-                acc += xg * X[l][j];
+                for (int i = 0; i < N; ++i) xg += Xc[k][i] * G[i][j];
+                acc += xg * Xc[l][j];
             }
             A[k][l] = acc;
             A[l][k] = acc;
@@ -330,7 +530,7 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
     const double s_inv = 1 / s;
     const double pu[3] = {-v[0] * v2x2_inv, -v[1] * v2x2_inv, h * v2x2_inv};
     const double X0 = pu[0] * s_inv + hx, Y0 = pu[1] * s_inv + hy, Rc = pu[2] * s_inv;
-    const int q = (x[1] - x[0]) * (Y0 - y[0]) - (y[1] - y[0]) * (X0 - x[0]) > 0 ? -1 : 1;
+    const int q = (X(1) - X(0)) * (Y0 - Y(0)) - (Y(1) - Y(0)) * (X0 - X(0)) > 0 ? -1 : 1;
     const double chi2_circle = fabs(lambda) * renorm * 1 / sqr(2 * v[2] * pu[2] * s);
 
     // ---------------------------------------------------------------- circle errors
@@ -352,77 +552,106 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
         return s2 * Vab(a, b) + kap * (ma * mb);
     };
 
-    // y_k = w .* X[k,:]
+    // y_k = w .* Xc[k,:]
     double yv[3][N];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int a = 0; a < N; ++a) yv[k][a] = w[a] * X[k][a];
+        for (int a = 0; a < N; ++a) yv[k][a] = w[a] * Xc[k][a];
 
-    // C0 = w' C_jl w and E as bilinear forms over the (a, b) entries of the
-    // covariance blocks C_jl of the lifted points (:557-603)
-    double C0[3][3] = {};
+    // C0 = w' C_pr w and E (:611-742) from the bilinear forms
+    // M_pr[x][z] = y_x' C_pr y_z of the six upper blocks C_pr (p <= r) of the
+    // covariance of the lifted points (:557-603), one block at a time; the
+    // lower blocks are the transposes (C_rp = C_pr'), so F(x; r, p; z) is
+    // M_pr[z][x].  Term list of E[(i,j)][(k,l)] (:694-741):
+    //   k == l, i == j:  4 F(i; j,l; l)
+    //   k == l, i != j:  2 F(i; j,l; l) + 2 F(j; i,l; l)
+    //   k != l, i == j:  2 F(i; j,l; k) + 2 F(i; j,k; l)
+    //   k != l, i != j:  F(i; j,l; k) + F(i; j,k; l) + F(j; i,l; k) + F(j; i,k; l)
+    double C0[3][3];
     double E[6][6] = {};
     constexpr int nu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    // one block per iteration, not unrolled: the entries of V are recomputed
+    // per block instead of all being kept live across the six
+#pragma unroll 1
+    for (int pb = 0; pb < 6; ++pb) {
+        const int p = pb < 3 ? 0 : (pb < 5 ? 1 : 2), r = pb < 3 ? pb : (pb < 5 ? pb - 2 : 2);
+        double M[3][3] = {};
+        double c0 = 0.0;
 #pragma unroll
-    for (int a = 0; a < N; ++a)
+        for (int a = 0; a < N; ++a)
 #pragma unroll
-        for (int b = 0; b < N; ++b) {
-            const int lo = a < b ? a : b, hi = a < b ? b : a;
-            // blocks at (a, b) and at (b, a)
-            const double c00 = Vcs(lo, hi), c11 = Vcs(N + lo, N + hi);
-            const double c01ab = Vcs(a, N + b), c01ba = Vcs(b, N + a);
-            const double pxa = P[0][a], pxb = P[0][b], pya = P[1][a], pyb = P[1][b];
-            double cc[3][3];
-            cc[0][0] = c00;
-            cc[1][1] = c11;
-            cc[0][1] = c01ab;
-            cc[1][0] = c01ba;                                 // C10(a,b) = C01(b,a)
-            cc[0][2] = 2 * ((c00 * pxb) + (c01ab * pyb));     // C02(a,b)
-            cc[2][0] = 2 * ((c00 * pxa) + (c01ba * pya));     // C20(a,b) = C02(b,a)
-            cc[1][2] = 2 * (c01ba * pxb + c11 * pyb);         // C12(a,b), C10(a,b) = C01(b,a)
-            cc[2][1] = 2 * (c01ab * pxa + c11 * pya);         // C21(a,b) = C12(b,a)
-            {
-                // C22 = Symmetric(tmp, :U): tmp at (lo, hi)
-                const double d00 = c00, d11 = c11;
-                const double d01 = Vcs(lo, N + hi), d10 = Vcs(hi, N + lo);
-                const double plx = P[0][lo], phx = P[0][hi], ply = P[1][lo], phy = P[1][hi];
-                cc[2][2] = 2.0 * (d00 * d00 + d00 * d01 + d11 * d10 + d11 * d11) +
-                           4.0 * (d00 * (phx * plx) + d01 * (plx * phy) + d10 * (phx * ply) + d11 * (phy * ply));
-            }
-            const double wab = w[a] * w[b];
+            for (int b = 0; b < N; ++b) {
+                const int lo = a < b ? a : b, hi = a < b ? b : a;
+                double cv;
+                if (p == 0 && r == 0) {
+                    cv = Vcs(lo, hi);
+                } else if (p == 1 && r == 1) {
+                    cv = Vcs(N + lo, N + hi);
+                } else if (p == 0 && r == 1) {
+                    cv = Vcs(a, N + b);
+                } else if (p == 0 && r == 2) {
+                    cv = 2 * ((Vcs(lo, hi) * P[0][b]) + (Vcs(a, N + b) * P[1][b]));
+                } else if (p == 1 && r == 2) {
+                    cv = 2 * (Vcs(b, N + a) * P[0][b] + Vcs(N + lo, N + hi) * P[1][b]);
+                } else {
+                    // C22 = Symmetric(tmp, :U): tmp at (lo, hi), t01 = x_i y_j (:581)
+                    const double d00 = Vcs(lo, hi), d11 = Vcs(N + lo, N + hi);
+                    const double d01 = Vcs(lo, N + hi), d10 = Vcs(hi, N + lo);
+                    const double plx = P[0][lo], phx = P[0][hi], ply = P[1][lo], phy = P[1][hi];
+                    cv = 2.0 * (d00 * d00 + d00 * d01 + d11 * d10 + d11 * d11) +
+                         4.0 * (d00 * (phx * plx) + d01 * (plx * phy) + d10 * (phx * ply) + d11 * (phy * ply));
+                }
+                c0 += cv * (w[a] * w[b]);
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int xk = 0; xk < 3; ++xk) {
+                    const double yc = yv[xk][a] * cv;
 #pragma unroll
-                for (int j = i; j < 3; ++j) C0[i][j] += cc[i][j] * wab;
-            // E[A][B], A = (i, j) <= B = (k, l)
-#pragma unroll
-            for (int A_ = 0; A_ < 6; ++A_) {
-                const int i = nu[A_][0], j = nu[A_][1];
-#pragma unroll
-                for (int B_ = A_; B_ < 6; ++B_) {
-                    const int k = nu[B_][0], l = nu[B_][1];
-                    double e;
-                    if (l == k) {
-                        if (i == j)
-                            e = 4.0 * yv[i][a] * cc[j][l] * yv[l][b];
-                        else
-                            e = 2.0 * (yv[i][a] * cc[j][l] * yv[l][b] + yv[j][a] * cc[i][l] * yv[l][b]);
-                    } else {
-                        const double f = yv[i][a] * (cc[j][l] * yv[k][b] + cc[j][k] * yv[l][b]);
-                        if (i == j)
-                            e = 2.0 * f;
-                        else
-                            e = f + yv[j][a] * (cc[i][l] * yv[k][b] + cc[i][k] * yv[l][b]);
-                    }
-                    E[A_][B_] += e;
+                    for (int zk = 0; zk < 3; ++zk) M[xk][zk] += yc * yv[zk][b];
                 }
             }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if ((i == p && j == r) || (i == r && j == p)) C0[i][j] = c0;
+        // add this block's terms to E
+#pragma unroll
+        for (int A_ = 0; A_ < 6; ++A_) {
+            const int i = nu[A_][0], j = nu[A_][1];
+#pragma unroll
+            for (int B_ = A_; B_ < 6; ++B_) {
+                const int k = nu[B_][0], l = nu[B_][1];
+                double e = 0.0;
+                // weights of 0 or 1, not a choice between two addresses, so M
+                // stays in registers with the block index known only at run time
+                auto F = [&](double coef, int xk, int pp, int rr, int zk) {
+                    const double m1 = (pp == p && rr == r) ? 1.0 : 0.0;
+                    const double m2 = (pp != rr && pp == r && rr == p) ? 1.0 : 0.0;
+                    e += coef * (m1 * M[xk][zk] + m2 * M[zk][xk]);
+                };
+                if (k == l) {
+                    if (i == j) {
+                        F(4.0, i, j, l, l);
+                    } else {
+                        F(2.0, i, j, l, l);
+                        F(2.0, j, i, l, l);
+                    }
+                } else {
+                    if (i == j) {
+                        F(2.0, i, j, l, k);
+                        F(2.0, i, j, k, l);
+                    } else {
+                        F(1.0, i, j, l, k);
+                        F(1.0, i, j, k, l);
+                        F(1.0, j, i, l, k);
+                        F(1.0, j, i, k, l);
+                    }
+                }
+                E[A_][B_] += e;
+            }
         }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < i; ++j) C0[i][j] = C0[j][i];
+    }
 
     // J2: forward differences of the eigenvector (:746-775), each perturbed
     // matrix solved in the eigenbasis Q of A
@@ -534,174 +763,7 @@ PT_HD void riemann_fit(const double (&x)[N], const double (&y)[N], const double 
             }
     }
 
-    // ---------------------------------------------------------------- line fit (:841-955)
-    double theta = -q * atan(ff[3]);
-    theta = theta < 0 ? theta + kPiD : theta;
-    const double sn = sin(theta), cs = cos(theta);
-    double sa[N], czz[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double p0 = x[i] - X0, p1 = y[i] - Y0;
-        const double cross = (-X0) * p1 - (-Y0) * p0;
-        const double dot = (-X0) * p0 + (-Y0) * p1;
-        const double at = -q * atan2_bl(cross, dot);
-        sa[i] = at * Rc;
-        const double temp0 = -q * Rc * 1 / (sqr(dot) + sqr(cross));
-        const double jc[3] = {-temp0 * ((p1 + Y0) * dot - (p0 - X0) * cross),
-                              temp0 * ((p0 + X0) * dot - (Y0 - p1) * cross), at};
-        const double dx = temp0 * (Y0 * dot + X0 * cross);
-        const double dy = temp0 * (-X0 * dot + Y0 * cross);
-        // Jx Cov Jx' with Cov = diag(circle cov, hit cov), then the z'z' entry of rot (.) rot'
-        double t00 = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int n2 = 0; n2 < 3; ++n2) t00 += jc[m] * ccov[m][n2] * jc[n2];
-        t00 += dx * (ge[0][i] * dx + ge[1][i] * dy) + dy * (ge[1][i] * dx + ge[2][i] * dy);
-        const double t01 = dx * ge[3][i] + dy * ge[4][i];
-        const double t11 = ge[5][i];
-        czz[i] = cs * cs * t00 - 2.0 * cs * sn * t01 + sn * sn * t11;
-    }
-    double cm[N][N];
-    {
-        const double p_t = jl_min(20.0, ff[2] * B);
-        const double p_2 = p_t * p_t * (1.0 + 1.0 / (ff[3] * ff[3]));
-        double S[N], rl[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            S[i] = sqrt(sa[i] * sa[i] + z[i] * z[i]);
-            rl[i] = S[i];
-        }
-        rad_len_uniform<N>(rl);
-        double sig2[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const double tt = 1.0 + 0.038 * log(rl[i]);
-            sig2[i] = 0.000225 / p_2 * (tt * tt) * rl[i];
-        }
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-#pragma unroll
-            for (int l = k; l < N; ++l) {
-                double acc = k == l ? czz[k] : 0.0;
-#pragma unroll
-                for (int i = 0; i <= (k < l ? k : l); ++i) acc += fabs(S[k] - S[i]) * fabs(S[l] - S[i]) * sig2[i];
-                cm[k][l] = acc;
-                cm[l][k] = acc;
-            }
-    }
-    double xr[N], yr[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        xr[i] = sn * sa[i] + cs * z[i];
-        yr[i] = -cs * sa[i] + sn * z[i];
-    }
-    double Vy[N][N];
-    invert_n(cm, Vy);
-    double av0[N], av1[N];  // rows of A * Vy
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            a0 += Vy[i][j];
-            a1 += xr[i] * Vy[i][j];
-        }
-        av0[j] = a0;
-        av1[j] = a1;
-    }
-    double cp00 = 0.0, cp01 = 0.0, cp11 = 0.0, r0s = 0.0, r1s = 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        cp00 += av0[j];
-        cp01 += av0[j] * xr[j];
-        cp11 += av1[j] * xr[j];
-        r0s += av0[j] * yr[j];
-        r1s += av1[j] * yr[j];
-    }
-    // invert22 (cholesky_inversion.jl:8-19)
-    const double luc0 = 1.0 / cp00;
-    const double luc1 = cp01 * cp01 * luc0;
-    const double luc2 = 1.0 / (cp11 - luc1);
-    const double ci00 = luc1 * luc0 * luc2 + luc0, ci01 = -cp01 * luc0 * luc2, ci11 = luc2;
-    const double sol0 = ci00 * r0s + ci01 * r1s, sol1 = ci01 * r0s + ci11 * r1s;
-    const double cf = 1 / (sn - sol1 * cs);
-    const double Jl[2][2] = {{0.0, cf * cf}, {cf, sol0 * cs * cf * cf}};
-    const double m_ = cf * (sol1 * sn + cs);
-    const double q_ = cf * sol0;
-    const double Ci[2][2] = {{ci00, ci01}, {ci01, ci11}};
-    double lcov[2][2];
-#pragma unroll
-    for (int xk = 0; xk < 2; ++xk)
-#pragma unroll
-        for (int yk = 0; yk < 2; ++yk) {
-            double acc = 0.0;
-#pragma unroll
-            for (int zk = 0; zk < 2; ++zk) acc += (Jl[xk][0] * Ci[0][zk] + Jl[xk][1] * Ci[1][zk]) * Jl[yk][zk];
-            lcov[xk][yk] = acc;
-        }
-    double chi2_line = 0.0;
-    {
-        double res[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) res[i] = yr[i] - (sol0 + xr[i] * sol1);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < N; ++j) acc += res[j] * Vy[j][i];
-            chi2_line += acc * res[i];
-        }
-    }
-
-    // ---------------------------------------------------------------- perigee and TrackSOA
-    const double temp0 = X0 * X0 + Y0 * Y0;
-    const double temp1 = sqrt(temp0);
-    const double kp[3] = {atan2(q * X0, -q * Y0), q * (temp1 - Rc), q / Rc};
-    const double temp2 = sqr(X0) * 1.0 / temp0;
-    const double temp3 = 1.0 / temp1 * q;
-    const double J4[3][3] = {{-Y0 * temp2 / (X0 * X0), temp2 / X0, 0.0},
-                             {X0 * temp3, Y0 * temp3, (double)-q},
-                             {0.0, 0.0, -q / (Rc * Rc)}};
-    double kc[3][3];
-#pragma unroll
-    for (int xk = 0; xk < 3; ++xk)
-#pragma unroll
-        for (int yk = 0; yk < 3; ++yk) {
-            double acc = 0.0;
-#pragma unroll
-            for (int zk = 0; zk < 3; ++zk) {
-                double jc = 0.0;
-#pragma unroll
-                for (int m = 0; m < 3; ++m) jc += J4[xk][m] * ccov[m][zk];
-                acc += jc * J4[yk][zk];
-            }
-            kc[xk][yk] = acc;
-        }
-    const double bi = 1.0 / B;
-    out.state[0] = kp[0];
-    out.state[1] = kp[1];
-    out.state[2] = kp[2] * bi;
-    out.state[3] = m_;
-    out.state[4] = q_;
-    out.cov15[0] = kc[0][0];
-    out.cov15[1] = kc[0][1];
-    out.cov15[2] = bi * kc[0][2];
-    out.cov15[3] = 0.0;
-    out.cov15[4] = 0.0;
-    out.cov15[5] = kc[1][1];
-    out.cov15[6] = bi * kc[1][2];
-    out.cov15[7] = 0.0;
-    out.cov15[8] = 0.0;
-    out.cov15[9] = (bi * bi) * kc[2][2];
-    out.cov15[10] = 0.0;
-    out.cov15[11] = 0.0;
-    out.cov15[12] = lcov[0][0];
-    out.cov15[13] = lcov[0][1];
-    out.cov15[14] = lcov[1][1];
-    out.pt = B / fabs(kp[2]);
-    out.eta = asinh(m_);
-    out.chi2 = (chi2_circle + chi2_line) / double(2 * N - 5);
+    riemann_line_and_output<N>(x, y, z, ge, ff, X0, Y0, Rc, q, ccov, chi2_circle, B, out);
 }
 
 }  // namespace riemann

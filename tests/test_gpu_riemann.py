@@ -15,8 +15,9 @@ Tolerances, per quantity, against the oracle's double results:
   * 0.5-10 GeV (the fit-only workload's range): phi, tip, q/pT, cot(theta),
     zip and eta within 1e-5 x max(|value|, sigma); pT within
     1e-5 x max(pT, pT^2 sigma(q/pT)); chi2 within 1e-3 x max(chi2, 1);
-    covariances within 1e-3 x sqrt(C_ii C_jj), those with the tip (phi-tip,
-    tip-tip, tip-q/pT) within 0.2 x sqrt(C_ii C_jj);
+    covariances within 1e-3 x sqrt(C_ii C_jj); those with the tip (phi-tip,
+    tip-tip, tip-q/pT) within 1e-4 x sqrt(C_ii C_jj) for 99.9% of the tracks
+    and within 1 x sqrt(C_ii C_jj) for all;
   * 10-60 GeV: phi, tip, q/pT and pT as above; cot(theta), zip and eta within
     0.05 x max(|value|, sigma); the covariances are printed, not bounded (a
     one-ulp change of the hits alone moves the oracle's Var(tip) there by
@@ -62,15 +63,16 @@ def errors(res, out64):
     sig_eta = sig[:, 3] / np.sqrt(1 + out64[ok, 3] ** 2)
     err["eta"] = float(np.max(np.abs(res["eta"][ok] - out64[ok, 21]) / np.maximum(np.abs(out64[ok, 21]), sig_eta)))
     err["chi2"] = float(np.max(np.abs(res["chi2"][ok] - out64[ok, 22]) / np.maximum(out64[ok, 22], 1.0)))
-    cov, cov_tip = 0.0, 0.0
+    cov, cov_tip, cov_tip_999 = 0.0, 0.0, 0.0
     for m, (i, j) in enumerate(PAIR):
         scale = np.sqrt(np.abs(out64[ok, 5 + DIAG[i]] * out64[ok, 5 + DIAG[j]]))
-        e = float(np.max(np.abs(res["cov"][m, ok] - out64[ok, 5 + m]) / scale))
+        rel = np.abs(res["cov"][m, ok] - out64[ok, 5 + m]) / scale
         if m in TIP_COV:
-            cov_tip = max(cov_tip, e)
+            cov_tip = max(cov_tip, float(np.max(rel)))
+            cov_tip_999 = max(cov_tip_999, float(np.quantile(rel, 0.999)))
         else:
-            cov = max(cov, e)
-    err["cov"], err["cov_tip"] = cov, cov_tip
+            cov = max(cov, float(np.max(rel)))
+    err["cov"], err["cov_tip"], err["cov_tip_p99.9"] = cov, cov_tip, cov_tip_999
     return err, ok
 
 
@@ -88,7 +90,8 @@ def test_riemann_schedules_match_oracle(ctx, schedule):
         assert err[k] <= 1e-5, (k, err[k])
     assert err["chi2"] <= 1e-3
     assert err["cov"] <= 1e-3
-    assert err["cov_tip"] <= 0.2
+    assert err["cov_tip_p99.9"] <= 1e-4
+    assert err["cov_tip"] <= 1.0
 
 
 def test_riemann_schedules_bit_identical(ctx):
