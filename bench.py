@@ -278,8 +278,49 @@ def fit_only(ctx, n_tracks=1 << 20, steps=20, cpu=False):
                                    "sample": f"the same {n_tracks} tracks, FP64 oracle fit (restatement of "
                                              f"launchBrokenLineKernelsOnCPU's fit), {th} threads"}
             out["fp64_persistent_vs_cpu"] = round(out["fp64_persistent_tracks_per_s"] / (n_tracks / dt), 1)
+        out["riemann"] = riemann_fit_only(ctx, fb, tr, n_tracks, steps, cpu)
     finally:
         fb.free()
+    return out
+
+
+def riemann_fit_only(ctx, fb, tr, n_tracks, steps, cpu, cpu_sample=262144):
+    """The Riemann helix fit (pt_fit_riemann_*) on the fit-only tracks: tracks/s
+    under each schedule (persistent after the autotuner), against the FP64
+    broken-line fit; CPU baseline: the oracle's Riemann restatement on the
+    first cpu_sample tracks, all host threads."""
+    from pixeltrack import _abi as A
+    tuned, tuned_ms = fb.autotune(A.FIT_RIEMANN)
+    out = {"tuning": dict(tuned, ms=round(tuned_ms, 4))}
+    for sched, sname in ((A.SCHED_PLAIN, "plain"), (A.SCHED_PER_LENGTH, "per_length"),
+                         (A.SCHED_PERSISTENT, "persistent")):
+        fb.run_riemann_async(sched)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fb.run_riemann_async(sched)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"{sname}_tracks_per_s"] = round(n_tracks / dt, 1)
+        out[f"{sname}_ms"] = round(dt * 1e3, 4)
+    fb.run_async(A.FIT_FP64, A.SCHED_PERSISTENT)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fb.run_async(A.FIT_FP64, A.SCHED_PERSISTENT)
+    ctx.sync()
+    bl = n_tracks / ((time.perf_counter() - t0) / steps)
+    out["persistent_vs_fp64_broken_line"] = round(out["persistent_tracks_per_s"] / bl, 3)
+    if cpu:
+        O, info = oracle(), cpu_info()
+        th = info["threads_used"]
+        m = min(cpu_sample, n_tracks)
+        h, g, nh = tr["hits"][:, :, :m], tr["ge"][:, :, :m], tr["nhit"][:m]
+        dt = timed_cpu(lambda: O.fit_riemann(h, g, nh, n_threads=th))
+        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "tracks/s", "cores": th, "kind": "port",
+                               "sample": f"the first {m} of the same tracks, the oracle's Riemann fit "
+                                         f"(restatement of riemann_fit.jl's helix_fit), {th} threads"}
+        out["persistent_vs_cpu"] = round(out["persistent_tracks_per_s"] / (m / dt), 1)
     return out
 
 
