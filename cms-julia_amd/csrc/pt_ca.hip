@@ -131,6 +131,15 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
     return pos;
 }
 
+// position of the r-th set bit (0-based) of the 192-bit mask (w1 lowest)
+__device__ __forceinline__ uint32_t nth_set_bit3(uint64_t w1, uint64_t w2, uint64_t w3, uint32_t r) {
+    const uint32_t c1 = (uint32_t)__popcll(w1), c2 = (uint32_t)__popcll(w2);
+    if (r < c1) return nth_set_bit(w1, r);
+    r -= c1;
+    if (r < c2) return 64u + nth_set_bit(w2, r);
+    return 128u + nth_set_bit(w3, r - c2);
+}
+
 // Wavefront owners hold lane-contiguous items; after a round of 64 items
 // starting at `base`, fold the pass bits of this lane's own items (lanes
 // [s0, s1), item k at lane k - (excl - base)) into its count and 64-bit mask.
@@ -492,7 +501,9 @@ __global__ void __launch_bounds__(256) k_reset(const EvDesc *__restrict__ evs, E
 // slot through an owner map in LDS (a slot's lane at its first item, carried
 // forward by a running maximum), and each round's pass ballot is stored as a
 // 64-bit word; afterwards every lane reads its own slot's bits back: the
-// count and, for the first 64 candidates, the bit mask.  Windows of
+// count and, for the first 256 candidates, the bit mask (the first 64 in
+// slot_mask, the next 192 in slot_mask_ext for slots with more; at PU200 a
+// third of the slots have more than 64 candidates).  Windows of
 // kCountWin items bound the map.  Per tile of kSlotTile slots the counts are
 // also summed.
 constexpr uint32_t kCountWin = 1024;
@@ -532,7 +543,7 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
         const uint32_t total = __shfl(incl, 63, 64);
         ctx[w][lane] = SlotItem{s, excl};
         uint32_t cnt = 0;
-        uint64_t msk = 0;
+        uint64_t msk = 0, mx1 = 0, mx2 = 0, mx3 = 0;  // pass bits of candidates 0..255
         for (uint32_t w0 = 0; w0 < total; w0 += kCountWin) {
             const uint32_t wn = min(total - w0, kCountWin);
             // owner map of items [w0, w0 + wn): lane + 1 where a slot's items start
@@ -602,13 +613,25 @@ __global__ void __launch_bounds__(256, 5) k_doublet_count(const EvDesc *__restri
                 if (len < 64u) bits &= (1ull << len) - 1ull;
                 cnt += (uint32_t)__popcll(bits);
                 const uint32_t k0 = x - excl;
-                if (k0 < 64u) msk |= bits << k0;
+                if (k0 < 256u) {
+                    const uint32_t wi = k0 >> 6, s2 = k0 & 63u;
+                    const uint64_t lo_part = bits << s2, hi_part = s2 ? (bits >> (64u - s2)) : 0ull;
+                    msk |= (wi == 0u) ? lo_part : 0ull;
+                    mx1 |= (wi == 1u) ? lo_part : ((wi == 0u) ? hi_part : 0ull);
+                    mx2 |= (wi == 2u) ? lo_part : ((wi == 1u) ? hi_part : 0ull);
+                    mx3 |= (wi == 3u) ? lo_part : ((wi == 2u) ? hi_part : 0ull);
+                }
             }
             wave_sync();  // reads of omap and pbits done before the next window
         }
         if (j < n_slots) {
             e.s.slot_cnt[j] = cnt;
             e.s.slot_mask[j] = msk;
+            if (n > 64u) {
+                e.s.slot_mask_ext[3u * j] = mx1;
+                e.s.slot_mask_ext[3u * j + 1u] = mx2;
+                e.s.slot_mask_ext[3u * j + 2u] = mx3;
+            }
             e.s.slot_pos[j] = make_uint4(s.a0, s.na, s.b0, n | (s.pk & 0xFFFF0000u));
         }
         const uint32_t sum = wave_sum(cnt);
@@ -701,7 +724,7 @@ template <class HT> __device__ __forceinline__ uint32_t fill_hash_slot(HT &H, ui
 template <int U>
 __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
     __shared__ uint32_t lid[4][64];
-    __shared__ unsigned long long lmsk[4][64];
+    __shared__ unsigned long long lmsk[4][64];  // a slot's first 64 pass bits (the rest stay in slot_mask_ext)
     __shared__ uint4 lpos[4][64];
     __shared__ FillHash<U> hash_all[4];
     uint32_t ev, b;
@@ -736,13 +759,17 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
         const uint32_t t = take_ticket(&e.s.misc[kMiscTkFill]);
         if (t >= n_tiles) break;
         const uint32_t j = t * 64u + lane;
-        uint32_t id = 0;
+        uint32_t id = 0, nm = 0;  // nm: passing candidates among the slot's first 256
         uint64_t msk = 0;
         uint4 sp = make_uint4(0u, 0u, 0u, 0u);
         if (j < n_slots) {
             id = e.s.slot_cnt[j];
             msk = e.s.slot_mask[j];
             sp = e.s.slot_pos[j];
+            nm = (uint32_t)__popcll(msk);
+            if ((sp.w & 0xFFFFu) > 64u)
+                nm += (uint32_t)(__popcll(e.s.slot_mask_ext[3u * j]) + __popcll(e.s.slot_mask_ext[3u * j + 1u]) +
+                                 __popcll(e.s.slot_mask_ext[3u * j + 2u]));
         }
         wave_sync();  // the previous tile's reads of the LDS arrays are done
         if (__shfl(id, 0, 64) >= cap) continue;  // nothing of this tile is stored
@@ -777,7 +804,7 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
             write_cell(js, p, e.s.binrec[pos], e.s.binxy[pos], cid, oh, i1);
             list_add(oh, i1, cid, atomicAdd(&e.s.ohc_cursor[oh], 1u));
         };
-        const uint32_t nm = (uint32_t)__popcll(msk);
+
         const uint32_t fincl = wave_incl_scan(nm);
         const uint32_t fexcl = fincl - nm;
         const uint32_t ftotal = __shfl(fincl, 63, 64);
@@ -805,7 +832,19 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
                 const uint4 so = lpos[w][o];
                 pp[u] = (int)(so.w >> 16);
                 js[u] = jw + (uint32_t)o;
-                pos[u] = act[u] ? cand_pos(so.x, so.y, so.z, nth_set_bit(lmsk[w][o], rk)) : 0u;
+                pos[u] = 0u;
+                if (act[u]) {
+                    const uint64_t m0 = lmsk[w][o];
+                    const uint32_t c0 = (uint32_t)__popcll(m0);
+                    uint32_t k;
+                    if (rk < c0) {
+                        k = nth_set_bit(m0, rk);
+                    } else {  // past the first 64 candidates (PU200-like densities)
+                        const PT_G uint64_t *x = e.s.slot_mask_ext + 3u * (size_t)js[u];
+                        k = 64u + nth_set_bit3(x[0], x[1], x[2], rk - c0);
+                    }
+                    pos[u] = cand_pos(so.x, so.y, so.z, k);
+                }
                 if (act[u]) {
                     q[u] = e.s.binrec[pos[u]];
                     xy[u] = e.s.binxy[pos[u]];
@@ -861,11 +900,11 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
             for (uint32_t v = 0; v < kFillHash / 64u; ++v) H.key[64u * v + lane] = kFillEmpty;
             wave_sync();
         }
-        if (j < n_slots && (sp.w & 0xFFFFu) > 64u) {  // candidates beyond a slot's first 64: serial, rare
+        if (j < n_slots && (sp.w & 0xFFFFu) > 256u) {  // candidates beyond a slot's first 256: serial, rare
             SlotCtx s;
             const uint32_t n = slot_setup(e, cfg, PT, j, ref, s);
             uint32_t cid = id + nm;
-            for (uint32_t k = 64; k < n && cid < cap; ++k) {
+            for (uint32_t k = 256; k < n && cid < cap; ++k) {
                 const uint32_t pos = cand_pos(s, k);
                 if (cand_pass(e, cfg, s, e.s.binrec[pos], ref)) emit(j, slot_pair(s), pos, cid++);
             }

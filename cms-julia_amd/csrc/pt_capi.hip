@@ -76,6 +76,7 @@ size_t carve(char *base, pt::Scratch *s) {
     Scratch t;
     t.slot_cnt = (uint32_t *)take((kMaxSlots + 1) * 4);
     t.slot_mask = (uint64_t *)take((size_t)kMaxSlots * 8);
+    t.slot_mask_ext = (uint64_t *)take((size_t)kMaxSlots * 24);
     t.slot_pos = (uint4 *)take((size_t)kMaxSlots * 16);
     t.dbl_agg = (uint32_t *)take(kDblAgg * 4);
     t.cell_io = (uint32_t *)take(kMaxCells * 4);

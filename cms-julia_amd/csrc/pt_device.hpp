@@ -55,6 +55,7 @@ struct Scratch {
     PT_G uint32_t *slot_cnt;     // [kMaxSlots+1]  passing candidates per inner slot, then the
                             //                slot's first cell id (total at [n_slots])
     PT_G uint64_t *slot_mask;    // [kMaxSlots]    passing candidates among the slot's first 64
+    PT_G uint64_t *slot_mask_ext;  // [3*kMaxSlots] candidates 64..255, for slots with more than 64
     PT_G uint4 *slot_pos;        // [kMaxSlots]    candidate positions (a0, na, b0) | n | pair << 16
     PT_G uint32_t *dbl_agg;      // [kDblAgg]      passing candidates per tile of kSlotTile slots
     PT_G uint32_t *cell_io;      // [kMaxCells]    inner | outer << 16 (1-based hit ids)
