@@ -917,31 +917,69 @@ __global__ void __launch_bounds__(256) k_doublet_fill(const EvDesc *__restrict__
 }
 
 // Outer-hit lists that overflowed kOhcStride: rebuild them as the first 128
-// cells in id order (the entries push! keeps), one wavefront per event
-// scanning the cells.  Exits at once when no list overflowed.
-__global__ void __launch_bounds__(64) k_ohc_rebuild(const EvDesc *__restrict__ evs, EvGrid grid) {
+// cells in id order (the entries push! keeps).  Cell ids run pair by pair, so
+// only the id ranges of the layer pairs that end on the hit's layer are
+// scanned, in pair order, 2048 cells a round: each thread loads eight cells
+// 256 apart, and a match's rank in id order is the matches of the earlier
+// sub-rounds, of the earlier wavefronts in its sub-round (per-wavefront
+// ballot counts through LDS) and of the lanes below it.  One block per event;
+// exits at once when no list overflowed.
+__global__ void __launch_bounds__(256) k_ohc_rebuild(const EvDesc *__restrict__ evs, EvGrid grid, Config cfg) {
+    constexpr uint32_t U = 8;
+    __shared__ uint32_t wcnt[U][4];
     uint32_t ev, b;
     if (!ev_block(grid, ev, b)) return;
     const EvDesc &e = evs[ev];
-    uint32_t err = 0;
     const uint32_t n_ovf = e.s.misc[kMiscOhcOvf];
+    if (n_ovf == 0u) return;  // block-uniform
+    __shared__ PairLds PT;
+    load_pairs(PT, e, cfg);
+    const uint32_t w = threadIdx.x >> 6;
+    uint32_t err = 0;
     if (n_ovf > kOhcOvfCap) err |= PT_INT_OVF_LIST;
     const uint32_t n = min(n_ovf, kOhcOvfCap);
-    const uint32_t n_cells = e.cnt->n_cells;
+    const uint32_t n_cells = e.cnt->n_cells, n_slots = e.n_slots;
     for (uint32_t q = 0; q < n; ++q) {
         const uint32_t h = e.s.ohc_ovf[q];
+        const int L = layer_of(e, h);
         uint32_t r = 0;
-        for (uint32_t c0 = 0; c0 < n_cells && r < kOhcStride; c0 += 64u) {
-            const uint32_t c = c0 + lane_id();
-            const uint32_t io = (c < n_cells) ? e.s.cell_io[c] : 0u;
-            const bool match = c < n_cells && (io >> 16) == h + 1u;
-            const uint64_t bal = __ballot(match);
-            const uint32_t rank = r + (uint32_t)__popcll(bal & lanes_below());
-            if (match && rank < kOhcStride) e.s.ohc_list[ohc_index(h, rank)] = make_uint2(c, (io & 0xFFFFu) - 1u);
-            r += (uint32_t)__popcll(bal);
+        for (int p = 0; p < 19 && r < kOhcStride; ++p) {
+            if (PT.outer[p] != L) continue;
+            const uint32_t s0 = (p == 0) ? 0u : PT.slot_cum[p - 1], s1 = PT.slot_cum[p];
+            if (s1 == 0xFFFFFFFFu || s0 >= n_slots) break;  // pairs not in use
+            const uint32_t c0 = min(e.s.slot_cnt[s0], n_cells), c1 = min(e.s.slot_cnt[min(s1, n_slots)], n_cells);
+            for (uint32_t base = c0; base < c1 && r < kOhcStride; base += 256u * U) {
+                uint32_t io[U];
+                uint64_t bal[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t c = base + 256u * u + threadIdx.x;
+                    io[u] = (c < c1) ? e.s.cell_io[c] : 0u;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t c = base + 256u * u + threadIdx.x;
+                    bal[u] = __ballot(c < c1 && (io[u] >> 16) == h + 1u);
+                    if (lane_id() == 0) wcnt[u][w] = (uint32_t)__popcll(bal[u]);
+                }
+                __syncthreads();
+                uint32_t run = r;  // matches before sub-round u
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t n0 = wcnt[u][0], n1 = wcnt[u][1], n2 = wcnt[u][2], n3 = wcnt[u][3];
+                    const uint32_t before = (w > 0 ? n0 : 0u) + (w > 1 ? n1 : 0u) + (w > 2 ? n2 : 0u);
+                    const uint32_t rank = run + before + (uint32_t)__popcll(bal[u] & lanes_below());
+                    if (((bal[u] >> lane_id()) & 1u) && rank < kOhcStride)
+                        e.s.ohc_list[ohc_index(h, rank)] =
+                            make_uint2(base + 256u * u + threadIdx.x, (io[u] & 0xFFFFu) - 1u);
+                    run += n0 + n1 + n2 + n3;
+                }
+                r = run;
+                __syncthreads();  // wcnt is rewritten by the next round
+            }
         }
     }
-    if (err && lane_id() == 0) atomicOr(&e.cnt->internal_error, err);
+    if (err && threadIdx.x == 0) atomicOr(&e.cnt->internal_error, err);
 }
 
 // ------------------------------------------------------------------ connect
@@ -2376,7 +2414,7 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
         default: resident(k_doublet_fill<3>, cfg); break;
     }
     mark("k_doublet_fill");
-    k_ohc_rebuild<<<ev_grid_dim(one), 64, 0, st>>>(d_evs, one);
+    k_ohc_rebuild<<<ev_grid_dim(one), 256, 0, st>>>(d_evs, one, cfg);
     mark("k_ohc_rebuild");
     switch (cfg.conn_hits * 1000u + cfg.conn_stage) {
         case 4128: resident(k_connect<4, 128>, cfg); break;

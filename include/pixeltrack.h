@@ -145,7 +145,10 @@ typedef struct pt_hits_view {
 /* Output: TrackSOA = TrackSOAT{32768,163840} (PixelTrackHeterogeneous.jl:10-33,59).
  * state and covariance are column-major (one column of 32768 per component,
  * TrajectoryStateSOA.jl:8-19).  Fields the reference leaves uninitialised
- * (quality of absent tuples, pt/eta/chi2 of unfitted tuples) are not written. */
+ * (quality of absent tuples, pt/eta/chi2 of unfitted tuples) are not written.
+ * hit_bins and det_bins are defined up to hit_off[32768] - 1 (the container);
+ * past it they hold whatever an earlier event left there, as the reference's
+ * reused buffers do. */
 typedef struct pt_track_soa {
     int32_t  *quality;     /* [32768]                                   */
     float    *chi2;        /* [32768]                                   */

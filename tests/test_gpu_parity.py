@@ -74,9 +74,10 @@ def stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg):
             errors.append(f"{name}: {len(bad)} lists differ, first {bad[:3]}: "
                           f"{[gl[i].tolist() for i in bad[:3]]} vs {[rl[i].tolist() for i in bad[:3]]}")
     m = min(rcnt["n_tuples"], A.MAX_TUPLES)
-    nb = min(rcnt["n_tuple_hits"], A.MAX_TUPLE_HITS)
+    # the container's hit and module ids: past its last offset both sides hold
+    # whatever an earlier event left (the reference reuses the buffers)
     sz = int(ref.hit_off[-1]) - 1
-    for name, g, r in (("hit_off", out.hit_off, ref.hit_off), ("hit_bins", out.hit_bins[:nb], ref.hit_bins[:nb]),
+    for name, g, r in (("hit_off", out.hit_off, ref.hit_off), ("hit_bins", out.hit_bins[:sz], ref.hit_bins[:sz]),
                        ("det_off", out.det_off, ref.det_off), ("det_bins", out.det_bins[:sz], ref.det_bins[:sz]),
                        ("quality", out.quality[:m], ref.quality[:m])):
         d = diff_report(g, r)

@@ -89,6 +89,32 @@ def test_neighbour_list_columns_at_the_limit(frames, case):
     np.testing.assert_allclose(out.state[:, fitted], ref.state[:, fitted], rtol=FIT_RTOL, atol=FIT_ATOL)
 
 
+def test_outer_hit_lists_past_their_capacity(frames):
+    """Outer hits with more than 128 cells (2 azimuths x 3 pseudorapidities x
+    30 copies): the capped push! (vec_array.jl:7-17, called at
+    gpu_pixel_doublets.jl:349) keeps each list's first 128 cells in id order,
+    over the cells of every layer pair that ends on the hit's layer."""
+    hits = K.make_event(K.bundles(2, (0.0, 0.3, -0.3), 30))
+    print(f"[switch] bundles past the outer-hit list capacity: {hits.n_hits} hits", flush=True)
+    ref, rcnt, rdbg = O.make_tuples(hits, frames, stop_after=0)
+    assert rcnt["overflow"] & A.OVF["OUTER_HIT_CELLS"]
+    c = rdbg.cells()
+    outer, pair = np.asarray(c["outer"]), np.asarray(c["pair"])
+    full = np.nonzero(np.bincount(outer) > A.MAX_CELLS_PER_HIT)[0]
+    multi = [h for h in full if np.unique(pair[outer == h]).size > 1]
+    assert full.size > 0 and len(multi) > 0  # lists that span two layer pairs
+    gen = ca.CAHitNtupletGeneratorOnGPU(frames)
+    try:
+        out = gen.make_tuples(hits)
+        gcnt = gen.last_counters
+        gdbg = gen.ctx.ca_debug(0, hits.n_hits)
+    finally:
+        gen.ctx.close()
+    assert gcnt["internal_error"] == 0
+    errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
+    assert not errors, "\n".join(errors)
+
+
 @pytest.mark.parametrize("same_module", [False, True], ids=["modules_differ", "same_module"])
 def test_stage_parity_on_crafted_fishbone_events(frames, same_module):
     """The hand-checked fishbone cases of the oracle's tests (the `break` after
