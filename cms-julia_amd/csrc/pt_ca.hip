@@ -2174,6 +2174,32 @@ __global__ void __launch_bounds__(256) k_tuples_post(const EvDesc *__restrict__ 
 __device__ void trk_exact(const EvDesc &e, uint32_t cell, uint32_t mc) {
     const uint4 *tcs = reinterpret_cast<const uint4 *>(e.s.tuple_cells);
     auto size_of = [&](uint32_t t) { return e.hit_off[t + 1] - e.hit_off[t]; };
+    const uint32_t cnt = e.s.trk[cell].x;  // stored tuples on the cell
+    if (cnt < 2u) return;
+    if (cnt <= PT_MAX_CELL_TRACKS) {
+        // the list is every stored tuple on the cell, so the order of the scan
+        // does not matter: from the last stored tuple down (the cells of the
+        // container's last tuple come here), stopping once all cnt are found
+        uint32_t maxh = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            uint32_t r = 0;
+            for (uint32_t t1 = mc; t1 > 0u && r < cnt; t1 = (t1 > 64u) ? t1 - 64u : 0u) {
+                const uint32_t t = t1 - 1u - lane_id();
+                bool has = false;
+                if (lane_id() < t1) {
+                    const uint4 tc = tcs[t];
+                    has = tc.x == cell || tc.y == cell || tc.z == cell || tc.w == cell;
+                }
+                if (pass == 0)
+                    maxh = max(maxh, wave_max(has ? size_of(t) : 0u));
+                else if (has && size_of(t) != maxh)
+                    e.quality[t] = PT_QUALITY_DUP;
+                r += (uint32_t)__popcll(__ballot(has));
+            }
+        }
+        return;
+    }
+    // a capped list: its first PT_MAX_CELL_TRACKS tuples in id order
     uint32_t r = 0, maxh = 0;
     for (uint32_t t0 = 0; t0 < mc && r < PT_MAX_CELL_TRACKS; t0 += 64u) {
         const uint32_t t = t0 + lane_id();
@@ -2205,7 +2231,10 @@ __device__ void trk_exact(const EvDesc &e, uint32_t cell, uint32_t mc) {
 
 // kernel_countMultiplicity / finalize! / kernel_fillMultiplicity
 // (ca_hit_ntuplet_generator_kernels_impl.jl:174-211): bucket = hit count, each
-// bucket lists tuple ids in DESCENDING order (fill_direct decrements).
+// bucket lists tuple ids in DESCENDING order (fill_direct decrements).  Two
+// blocks per event run its independent parts side by side: block 0 the
+// offsets of the shared cells' tuple lists, block 1 the exact early-duplicate
+// pass, the event's flags and the buckets.
 constexpr uint32_t kMulU = 8, kMulChunk = 1024u * kMulU;  // shared cells per chunk of the list offsets
 constexpr uint8_t kMulNone = 5, kMulLong = 6;
 __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict__ evs, EvGrid grid) {
@@ -2217,34 +2246,7 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
     if (!ev_block(grid, ev, blk)) return;
     const EvDesc &e = evs[ev];
     const uint32_t m = e.cnt->n_tuples < kMaxTuples ? e.cnt->n_tuples : kMaxTuples;
-    {
-        const uint32_t n_ovf = min(e.s.misc[kMiscTrkOvf], kTrkOvfCap);
-        const uint32_t extra = (e.cnt->n_tuples > kMaxTuples) ? 4u : 0u;
-        const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (uint32_t k = wv; k < n_ovf + extra; k += nw) {
-            uint32_t cell;
-            if (k < n_ovf) {
-                cell = e.s.trk_ovf[k];
-            } else {
-                cell = e.s.tuple_cells[4u * (kMaxTuples - 1u) + (k - n_ovf)];
-                if (cell == ~0u) continue;
-            }
-            trk_exact(e, cell, m);
-        }
-        if (threadIdx.x == 0) {
-            if (e.cnt->n_cells == 0) atomicOr(&e.cnt->ref_assert, PT_REF_NO_CELLS);
-            uint32_t o = 0;
-            if (e.s.misc[kMiscNbrLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_NEIGHBOR_LISTS;
-            if (e.s.misc[kMiscTrkLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_TRACK_LISTS;
-            if (o) atomicOr(&e.cnt->overflow, o);
-            // tuple lists: the reference's later cells would share its last
-            // column (extend!, simple_matrix.jl:13-19); kept per cell here, so
-            // the event's results are not the reference's
-            if (o & PT_OVF_TRACK_LISTS) atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
-        }
-        __syncthreads();
-    }
-    {
+    if (blk == 0) {
         // offsets of the shared cells' tuple lists (classification), in slot
         // order, kMulChunk cells at a time: their tuple counts are gathered with
         // coalesced loads into LDS, then each thread scans a contiguous run
@@ -2285,21 +2287,50 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
             base += total;
             __syncthreads();  // s_dcn is refilled by the next chunk
         }
+        return;
+    }
+    // block 1: the exact early-duplicate pass, the event's flags, the buckets
+    {
+        const uint32_t n_ovf = min(e.s.misc[kMiscTrkOvf], kTrkOvfCap);
+        const uint32_t extra = (e.cnt->n_tuples > kMaxTuples) ? 4u : 0u;
+        const uint32_t wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (uint32_t k = wv; k < n_ovf + extra; k += nw) {
+            uint32_t cell;
+            if (k < n_ovf) {
+                cell = e.s.trk_ovf[k];
+            } else {
+                cell = e.s.tuple_cells[4u * (kMaxTuples - 1u) + (k - n_ovf)];
+                if (cell == ~0u) continue;
+            }
+            trk_exact(e, cell, m);
+        }
+        if (threadIdx.x == 0) {
+            if (e.cnt->n_cells == 0) atomicOr(&e.cnt->ref_assert, PT_REF_NO_CELLS);
+            uint32_t o = 0;
+            if (e.s.misc[kMiscNbrLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_NEIGHBOR_LISTS;
+            if (e.s.misc[kMiscTrkLists] > PT_MAX_ACTIVE_LISTS - 1u) o |= PT_OVF_TRACK_LISTS;
+            if (o) atomicOr(&e.cnt->overflow, o);
+            // tuple lists: the reference's later cells would share its last
+            // column (extend!, simple_matrix.jl:13-19); kept per cell here, so
+            // the event's results are not the reference's
+            if (o & PT_OVF_TRACK_LISTS) atomicOr(&e.cnt->internal_error, PT_INT_LIST_SHARE);
+        }
+        __syncthreads();
     }
     // per tuple, coalesced: its bucket (hit count 3..7 -> 0..4), kMulNone when
     // it takes no part (fewer than 3 hits or dup), kMulLong for 8 or more hits
-    for (uint32_t t0 = 0; t0 < m; t0 += 4096u) {
-        uint32_t a[4], b[4];
-        int32_t q[4];
+    for (uint32_t t0 = 0; t0 < m; t0 += 8192u) {
+        uint32_t a[8], b[8];
+        int32_t q[8];
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
+        for (uint32_t u = 0; u < 8u; ++u) {
             const uint32_t t = t0 + threadIdx.x + 1024u * u;
             a[u] = (t < m) ? e.hit_off[t] : 0u;
             b[u] = (t < m) ? e.hit_off[t + 1] : 0u;
             q[u] = (t < m) ? e.quality[t] : 0;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u) {
+        for (uint32_t u = 0; u < 8u; ++u) {
             const uint32_t t = t0 + threadIdx.x + 1024u * u;
             const uint32_t nh = b[u] - a[u];
             if (t < m)
@@ -2336,12 +2367,25 @@ __global__ void __launch_bounds__(1024) k_multiplicity(const EvDesc *__restrict_
         for (int bk = 3; bk <= 5; ++bk)
             if (tot[bk - 3] > PT_MAX_FITS) atomicOr(&e.cnt->overflow, PT_OVF_FITS);
     }
-    __syncthreads();
+    // last position of each bucket (tm_off[bk + 2] + tot[bk] - 1), in registers:
+    // a bucket's ranks fill it from the end
+    uint32_t last[5];
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int bk = 0; bk < 5; ++bk) {
+            run += tot[bk];
+            last[bk] = run - 1u;
+        }
+    }
     for (uint32_t t = t0; t < t1; ++t) {
         const uint32_t bk = s_code[t];
         if (bk >= 5u) continue;
-        const uint32_t rank = ex[bk]++;
-        e.s.tm_bins[e.s.tm_off[bk + 2u] + (tot[bk] - 1u - rank)] = (uint16_t)(t + 1u);
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 5u; ++j)
+            if (bk == j) pos = last[j] - ex[j]++;
+        e.s.tm_bins[pos] = (uint16_t)(t + 1u);
     }
     if (ref) atomicOr(&e.cnt->ref_assert, ref);
 }
@@ -2465,7 +2509,8 @@ void launch_ca(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, uint32
     const EvGrid g_post = ev_grid(kMaxTuples + 1u + PT_MAX_TUPLE_HITS / 4u, n_events, xcd);
     k_tuples_post<<<ev_grid_dim(g_post), 256, 0, st>>>(d_evs, g_post);
     mark("k_tuples_post");
-    k_multiplicity<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
+    const EvGrid two{n_events, 2u, xcd};
+    k_multiplicity<<<ev_grid_dim(two), 1024, 0, st>>>(d_evs, two);
     mark("k_multiplicity");
 }
 
