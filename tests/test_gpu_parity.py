@@ -15,8 +15,9 @@ from pixeltrack import ca, generator as G
 
 pytestmark = pytest.mark.gpu
 
-# pu200 passes the doublet cap (458,752): the truncation must keep the same cells
-CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2), ("pu200", 1001)]
+# pu200 passes the doublet cap (458,752): the truncation must keep the same cells;
+# seed 1187 also has an outer hit with 129 cells (the 128-cell list cap)
+CASES = [("single_event", 1), ("pu50", 1), ("pu50", 2), ("pu200", 1001), ("pu200", 1187)]
 # the caps further on: a dense cone of 2,500 helices gives about 52,000 tuples (the
 # 32,768-tuple container fills, and the triplet bucket passes the 24,576 fits), 8,000
 # helices pass the fit cap alone; ten hits, one per layer, give no doublet at all
@@ -91,6 +92,8 @@ def test_stage_parity(gen_gpu, frames, preset, seed):
     hits, ref, rcnt, rdbg, out, gcnt, gdbg = run_both(gen_gpu, frames, preset, seed)
     if preset == "pu200":
         assert rcnt["overflow"] & A.OVF["DOUBLETS"] and rcnt["n_cells_found"] > rcnt["n_cells"] == A.MAX_DOUBLETS
+    if seed == 1187:
+        assert rcnt["overflow"] & A.OVF["OUTER_HIT_CELLS"]
     errors = stage_errors(ref, rcnt, rdbg, out, gcnt, gdbg)
     assert not errors, "\n".join(errors)
 
