@@ -57,13 +57,15 @@ constexpr uint32_t kDoneLds = 8192u;
 
 // classify_tuples after the fit (classification itself is in the fit
 // epilogue) runs in five phases; every phase reads what the previous one wrote
-// for the same event.  Two launch shapes (tuned per context): five kernels,
-// where phases 1, 2, 4 and 5, independent per tuple or per shared cell, take
-// several blocks per event (item i of the event goes to the block-strided loop
-// of that event's blocks) and phase 3's rounds, which need block barriers, run
-// in one block of 1024 threads per event; or one block of 1024 threads per
-// event for all five phases with barriers between them, which saves four
-// launches where events are small and many.
+// for the same event.  Two launch shapes (tuned per context): separate
+// kernels, where phases 1, 2, 4 and 5, independent per tuple or per shared
+// cell, take several blocks per event (item i of the event goes to the
+// block-strided loop of that event's blocks), phase 3's first kParRounds rounds
+// are a launch each (the launch boundary is the round's barrier) and its
+// remaining rounds, which need block barriers, run in one block of 1024
+// threads per event; or one block of 1024 threads per event for all five
+// phases with barriers between them, which saves launches where events are
+// small and many.
 
 // The phases are chains of dependent loads (tuple -> cells -> cell fields ->
 // lists); each is written so that the loads of one step are issued together
@@ -240,13 +242,16 @@ __device__ __forceinline__ void cl_head(const EvDesc &e, const uint32_t *L, uint
 // 0 clears the counter of round r + 1 during round r, when every wave has
 // passed the barrier that follows its read of that counter in round r - 2, so
 // no wave can read a counter that was cleared under it.
+// The rounds before first_round ran as kernels of their own (cl_round_par):
+// their stamps are read from global memory here.
 constexpr uint32_t kRoundCache = 2;
-__device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uint32_t *s_pending) {
+__device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uint32_t *s_pending,
+                                          uint32_t first_round) {
     const uint32_t nd = n_shared(e);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
-    for (uint32_t s = tid; s < min(nd, kDoneLds); s += nthr) s_done[s] = 0u;
-    if (tid == 0) s_pending[1] = 0u;
-    // the stamps beyond kDoneLds were cleared by the multiplicity kernel
+    for (uint32_t s = tid; s < min(nd, kDoneLds); s += nthr) s_done[s] = e.s.dc_done[s];
+    if (tid == 0) s_pending[first_round % 3u] = 0u;
+    // the stamps start cleared by the multiplicity kernel
     auto done_of = [&](uint32_t s) -> uint32_t { return s < kDoneLds ? s_done[s] : e.s.dc_done[s]; };
     // predecessors ready: each resolved in an earlier round
     auto ready_of = [&](uint32_t s, const uint4 &pred, const uint32_t *L, uint32_t n, uint32_t round) {
@@ -290,7 +295,7 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
 #pragma unroll
     for (uint32_t k = 0; k < kRoundCache; ++k) cl_head(e, c_L[k], c_n[k], c_t[k], c_tip[k]);
     __syncthreads();
-    for (uint32_t round = 1;; ++round) {
+    for (uint32_t round = first_round;; ++round) {
         if (tid == 0) s_pending[(round + 1u) % 3u] = 0u;
         uint32_t pending = 0;
 #pragma unroll
@@ -327,10 +332,52 @@ __device__ __forceinline__ void cl_rounds(const EvDesc &e, uint32_t *s_done, uin
         if (pending) atomicAdd(&s_pending[round % 3u], pending);
         __syncthreads();
         if (s_pending[round % 3u] == 0u) break;
-        if (round >= nd) {  // every round resolves at least one cell: not reached
+        if (round - first_round + 1u >= nd) {  // every round resolves at least one cell: not reached
             if (tid == 0) atomicOr(&e.cnt->internal_error, PT_INT_ROUNDS);
             break;
         }
+    }
+}
+
+// One round of phase 3 in a kernel of its own, several blocks per event: the
+// shared cells whose predecessors were all stamped by earlier launches are
+// resolved and stamped `round`.  A predecessor stamped `round` in this launch
+// is not ready (the rule of the rounds above), so the cells resolved together
+// share no tuple and the result is the serial one.
+constexpr uint32_t kParRounds = 3;
+__device__ __forceinline__ void cl_round_par(const EvDesc &e, uint32_t round, uint32_t i0, uint32_t step) {
+    auto done_before = [&](uint32_t p) {
+        const uint32_t d = e.s.dc_done[p];
+        return d != 0u && d != round;
+    };
+    for (uint32_t s = i0; s < n_shared(e); s += step) {
+        if (e.s.dc_done[s]) continue;
+        const uint4 pred = e.s.dc_pred[s];
+        const uint32_t *L = e.s.dc_ent + e.s.dc_off[s];
+        const uint32_t n = min(e.s.dc_cur[s], PT_MAX_CELL_TRACKS);
+        bool ready = true;
+        if (pred.x != kPredMany) {
+            const uint32_t ps[4] = {pred.x, pred.y, pred.z, pred.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ps[q] != ~0u && !done_before(ps[q])) ready = false;
+        } else {
+            const uint32_t c = e.s.dc_cells[s];
+            for (uint32_t k = 0; k < n && ready; ++k) {
+                uint32_t cs[4], sl[4];
+                bool v[4];
+                shared_cells_of(e, L[k], cs, sl, v);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (v[q] && cs[q] < c && !done_before(sl[q])) ready = false;
+            }
+        }
+        if (!ready) continue;
+        uint32_t t4[4];
+        float tip4[4];
+        cl_head(e, L, n, t4, tip4);
+        cl_resolve(e, L, n, t4, tip4);
+        e.s.dc_done[s] = round;
     }
 }
 
@@ -431,12 +478,18 @@ __global__ void __launch_bounds__(256) k_cl_prepare(const EvDesc *__restrict__ e
     cl_prepare(e, i0, step);
 }
 
+__global__ void __launch_bounds__(256) k_cl_round_par(const EvDesc *__restrict__ evs, EvGrid grid, uint32_t round) {
+    CL_PHASE(e, i0, step);
+    cl_round_par(e, round, i0, step);
+}
+
+// the rounds after the parallel ones (few cells are left for them)
 __global__ void __launch_bounds__(1024) k_cl_rounds(const EvDesc *__restrict__ evs, EvGrid grid) {
     __shared__ uint32_t s_done[kDoneLds];
     __shared__ uint32_t s_pending[3];
     uint32_t ev, blk;
     if (!ev_block(grid, ev, blk)) return;
-    cl_rounds(evs[ev], s_done, s_pending);
+    cl_rounds(evs[ev], s_done, s_pending, kParRounds + 1u);
 }
 
 __global__ void __launch_bounds__(256) k_cl_h2t_reduce(const EvDesc *__restrict__ evs, EvGrid grid) {
@@ -459,7 +512,7 @@ __global__ void __launch_bounds__(1024) k_classify_event(const EvDesc *__restric
     __syncthreads();
     cl_prepare(e, threadIdx.x, blockDim.x);
     __syncthreads();
-    cl_rounds(e, s_done, s_pending);
+    cl_rounds(e, s_done, s_pending, 1u);
     __syncthreads();
     cl_h2t_reduce(e, threadIdx.x, blockDim.x);
     __syncthreads();
@@ -482,6 +535,8 @@ void launch_classify(const EvDesc *d_evs, uint32_t n_events, const Config &cfg, 
     mark("k_cl_lists");
     k_cl_prepare<<<ev_grid_dim(shr), 256, 0, st>>>(d_evs, shr);
     mark("k_cl_prepare");
+    for (uint32_t r = 1; r <= kParRounds; ++r) k_cl_round_par<<<ev_grid_dim(shr), 256, 0, st>>>(d_evs, shr, r);
+    mark("k_cl_round_par");
     k_cl_rounds<<<ev_grid_dim(one), 1024, 0, st>>>(d_evs, one);
     mark("k_cl_rounds");
     k_cl_h2t_reduce<<<ev_grid_dim(tup), 256, 0, st>>>(d_evs, tup);
