@@ -232,6 +232,22 @@ def host_tasks(events, frames, n_tasks, seconds=3.0):
             "events_per_s_per_task": round(sum(done) / wall / n_tasks, 1)}
 
 
+def host_tasks_native(n_threads, max_batch, seconds=3.0):
+    """The same runner model from C++ threads (cms-julia_amd/lib/pt_host_tasks,
+    a child process): a context per thread (max_batch 0) or one batcher shared
+    by all threads (pt_batcher_make_tuples), PU50 events of seeds 1..100, each
+    thread refilling its own TrackSOA, which the library clears."""
+    import subprocess
+    exe = ROOT / "cms-julia_amd" / "lib" / "pt_host_tasks"
+    if not exe.exists():
+        return {"skipped": f"{exe} is not built"}
+    r = subprocess.run([str(exe), str(n_threads), str(seconds), "100", "1", str(max_batch)], capture_output=True,
+                       text=True, timeout=120)
+    if r.returncode != 0:
+        raise RuntimeError(f"pt_host_tasks failed ({r.returncode}): {r.stderr.strip()}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def fit_only(ctx, n_tracks=1 << 20, steps=20, cpu=False):
     """Fit-only workload: 1,048,576 pre-gathered tracks (seed 7, lengths
     3/4/5 in a 30/55/15% mix in random order, pT log-uniform 0.5-10 GeV,
@@ -519,6 +535,8 @@ def main():
         host_path = {"events_per_s": round(n_host / w_p, 1), "events": n_host,
                      "note": "pt_make_tuples: host hits in, host TrackSOA out, one event per call, synchronous"}
         host_path["tasks"] = host_tasks(events, frames, BOX_CPU_SHARE)
+        host_path["native_tasks"] = host_tasks_native(BOX_CPU_SHARE, 0)
+        host_path["native_tasks_shared_batcher"] = host_tasks_native(BOX_CPU_SHARE, 16)
         print(f"[bench] host buffers: {host_path}", file=sys.stderr, flush=True)
 
     n_fl = max(1, args.inflight)

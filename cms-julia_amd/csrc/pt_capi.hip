@@ -7,9 +7,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -30,6 +32,7 @@ void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st);
 void launch_phi_hist(const EvDesc *d_evs, uint32_t n_events, uint32_t xcd, hipStream_t st);
 void launch_load_tracks(const EvDesc *d_evs, uint32_t n_events, float pt_min, hipStream_t st);
 int noalloc_scratch(int device, uint32_t **counts, uint32_t **lists);
+void launch_pack_segments(const PackSeg *d_segs, uint32_t n_segs, char *d_dst, hipStream_t st);
 }  // namespace pt
 
 namespace {
@@ -656,41 +659,50 @@ int pt_get_counters(pt_ctx *c, uint32_t event, pt_counters *cnt) {
     return PT_OK;
 }
 
-static int ensure_staging(pt_ctx *c) {
-    if (c->stg.block) return PT_OK;
-    Staging &s = c->stg;
+// Device TrackSOA arrays of full capacity at b; returns their bytes (b and o
+// may be null for the size alone).
+static size_t carve_track_soa(char *b, pt_track_soa *o) {
     const size_t T = PT_MAX_TUPLES, M = PT_MAX_TUPLE_HITS;
     size_t off = 0;
-    auto take = [&](size_t b) {
-        size_t o = off;
-        off += align256(b);
-        return o;
+    auto take = [&](size_t bytes) {
+        size_t r = off;
+        off += align256(bytes);
+        return r;
     };
     const size_t o_q = take(T * 4), o_f = take(3 * T * 4), o_st = take(5 * T * 4), o_cv = take(15 * T * 4);
     const size_t o_off = take(2 * (T + 1) * 4), o_bins = take(2 * M * 2);
+    if (b && o) {
+        o->quality = (int32_t *)(b + o_q);
+        float *g = (float *)(b + o_f);
+        o->chi2 = g, o->eta = g + T, o->pt = g + 2 * T;
+        o->state = (float *)(b + o_st);
+        o->covariance = (float *)(b + o_cv);
+        uint32_t *u = (uint32_t *)(b + o_off);
+        o->hit_off = u, o->det_off = u + (T + 1);
+        uint16_t *u16 = (uint16_t *)(b + o_bins);
+        o->hit_bins = u16, o->det_bins = u16 + M;
+    }
+    return off;
+}
+
+static int ensure_staging(pt_ctx *c) {
+    if (c->stg.block) return PT_OK;
+    Staging &s = c->stg;
     s.in_bytes = PackedInputs(PT_MAX_HITS).total;
-    s.out_bytes = off;
-    HIP_OK(hipMalloc(&s.block, off));
+    s.out_bytes = carve_track_soa(nullptr, nullptr);
+    HIP_OK(hipMalloc(&s.block, s.out_bytes));
     HIP_OK(hipMalloc((void **)&s.in, s.in_bytes));
     HIP_OK(hipHostMalloc((void **)&s.h_in, s.in_bytes, hipHostMallocDefault));
     HIP_OK(hipHostMalloc((void **)&s.h_out, s.out_bytes, hipHostMallocDefault));
-    char *b = (char *)s.block;
-    s.quality = (int32_t *)(b + o_q);
-    float *g = (float *)(b + o_f);
-    s.chi2 = g, s.eta = g + T, s.pt = g + 2 * T;
-    s.state = (float *)(b + o_st);
-    s.cov = (float *)(b + o_cv);
-    uint32_t *u = (uint32_t *)(b + o_off);
-    s.hit_off = u, s.det_off = u + (T + 1);
-    uint16_t *u16 = (uint16_t *)(b + o_bins);
-    s.hit_bins = u16, s.det_bins = u16 + M;
+    pt_track_soa t;
+    carve_track_soa((char *)s.block, &t);
+    s.quality = t.quality, s.chi2 = t.chi2, s.eta = t.eta, s.pt = t.pt, s.state = t.state, s.cov = t.covariance;
+    s.hit_off = t.hit_off, s.det_off = t.det_off, s.hit_bins = t.hit_bins, s.det_bins = t.det_bins;
     return PT_OK;
 }
 
-int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track_soa *out, int out_zeroed,
-                   pt_counters *cnt) {
-    if (!c || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
-    HIP_OK(hipSetDevice(c->device));
+// ---- the host-buffer entry points (pt_make_tuples, pt_batcher_make_tuples): shared pieces
+static int check_host_args(const pt_hits_view *hits, const pt_track_soa *out) {
     if (hits->n_hits > PT_MAX_HITS)
         return fail(PT_ERR_CAPACITY, "%u hits above MAX_NUMBER_OF_HITS = %u", hits->n_hits, PT_MAX_HITS);
     if (hits->n_hits && (!hits->xg || !hits->yg || !hits->zg || !hits->rg || !hits->xerr || !hits->yerr ||
@@ -701,32 +713,95 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     if (!out->quality || !out->chi2 || !out->eta || !out->pt || !out->state || !out->covariance || !out->hit_off ||
         !out->hit_bins || !out->det_off || !out->det_bins)
         return fail(PT_ERR_ARG, "null TrackSOA array");
-    int rc = ensure_staging(c);
+    return PT_OK;
+}
+// an event's inputs into a host block in the PackedInputs layout
+static void pack_inputs(char *blk, const PackedInputs &L, const pt_hits_view *hits) {
+    const size_t n = hits->n_hits;
+    const float *fsrc[6] = {hits->xg, hits->yg, hits->zg, hits->rg, hits->xerr, hits->yerr};
+    const int16_t *isrc[3] = {hits->iphi, hits->ysize, hits->det_ind};
+    for (int k = 0; k < 6; ++k)
+        if (n) std::memcpy(blk + L.f[k], fsrc[k], 4 * n);
+    for (int k = 0; k < 3; ++k)
+        if (n) std::memcpy(blk + L.i16[k], isrc[k], 2 * n);
+    if (hits->hist_off) std::memcpy(blk + L.hist_off, hits->hist_off, 4 * PT_HIST_NOFF);
+    if (hits->hist_bins && n) std::memcpy(blk + L.hist_bins, hits->hist_bins, 2 * n);
+}
+// the view of those inputs once copied to d_blk on the device
+static pt_hits_view packed_view(const char *d_blk, const PackedInputs &L, const pt_hits_view *hits) {
+    pt_hits_view dv = *hits;
+    dv.xg = (const float *)(d_blk + L.f[0]), dv.yg = (const float *)(d_blk + L.f[1]);
+    dv.zg = (const float *)(d_blk + L.f[2]), dv.rg = (const float *)(d_blk + L.f[3]);
+    dv.xerr = (const float *)(d_blk + L.f[4]), dv.yerr = (const float *)(d_blk + L.f[5]);
+    dv.iphi = (const int16_t *)(d_blk + L.i16[0]), dv.ysize = (const int16_t *)(d_blk + L.i16[1]);
+    dv.det_ind = (const int16_t *)(d_blk + L.i16[2]);
+    dv.hist_off = hits->hist_off ? (const uint32_t *)(d_blk + L.hist_off) : nullptr;  // none given: built on the device
+    dv.hist_bins = hits->hist_bins ? (const uint16_t *)(d_blk + L.hist_bins) : nullptr;
+    return dv;
+}
+// The defined rows of an event's TrackSOA in a readback block: the stored
+// tuples' quality, chi2, eta, pt, state and covariance columns (only rows of
+// stored tuples can be non-zero), the offsets up to the stored tuples' end
+// (past it they all equal off[m], bulk_finalize_fill, unless the container is
+// full) and the container's hit and module ids; each field 16-byte aligned.
+struct OutLayout {
+    size_t m, nb, n_off;
+    size_t q, chi2, eta, pt, st, cv, ho, dof, hb, db, total;
+    explicit OutLayout(const pt_counters &k) {
+        const size_t T = PT_MAX_TUPLES;
+        m = std::min<size_t>(k.n_tuples, T);
+        nb = std::min<size_t>(k.n_tuple_hits, PT_MAX_TUPLE_HITS);
+        n_off = (k.n_tuples < T) ? m + 1 : T + 1;
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            const size_t r = o;
+            o += (bytes + 15) & ~size_t(15);
+            return r;
+        };
+        q = take(m * 4), chi2 = take(m * 4), eta = take(m * 4), pt = take(m * 4);
+        st = take(5 * m * 4), cv = take(15 * m * 4);
+        ho = take(n_off * 4), dof = take(n_off * 4), hb = take(nb * 2), db = take(nb * 2);
+        total = o;
+    }
+};
+// the caller's TrackSOA from a readback block
+static void unpack_outputs(const char *blk, const OutLayout &L, pt_track_soa *out, int out_zeroed) {
+    const size_t T = PT_MAX_TUPLES, m = L.m;
+    std::memcpy(out->quality, blk + L.q, m * 4);
+    std::memcpy(out->chi2, blk + L.chi2, m * 4);
+    std::memcpy(out->eta, blk + L.eta, m * 4);
+    std::memcpy(out->pt, blk + L.pt, m * 4);
+    for (int comp = 0; comp < 20; ++comp) {
+        float *col = (comp < 5 ? out->state + comp * T : out->covariance + (comp - 5) * T);
+        const char *src = comp < 5 ? blk + L.st + comp * m * 4 : blk + L.cv + (comp - 5) * m * 4;
+        std::memcpy(col, src, m * 4);
+        if (!out_zeroed) std::memset(col + m, 0, (T - m) * 4);
+    }
+    std::memcpy(out->hit_off, blk + L.ho, L.n_off * 4);
+    std::memcpy(out->det_off, blk + L.dof, L.n_off * 4);
+    std::memcpy(out->hit_bins, blk + L.hb, L.nb * 2);
+    std::memcpy(out->det_bins, blk + L.db, L.nb * 2);
+    for (size_t i = L.n_off; i <= T; ++i) {
+        out->hit_off[i] = out->hit_off[m];
+        out->det_off[i] = out->det_off[m];
+    }
+}
+
+int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track_soa *out, int out_zeroed,
+                   pt_counters *cnt) {
+    if (!c || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
+    HIP_OK(hipSetDevice(c->device));
+    int rc = check_host_args(hits, out);
+    if (rc) return rc;
+    rc = ensure_staging(c);
     if (rc) return rc;
     Staging &s = c->stg;
-    const size_t n = hits->n_hits;
     hipStream_t st = c->stream;
     // inputs: packed into the pinned block, one copy
-    const PackedInputs L(n);
-    {
-        const float *fsrc[6] = {hits->xg, hits->yg, hits->zg, hits->rg, hits->xerr, hits->yerr};
-        const int16_t *isrc[3] = {hits->iphi, hits->ysize, hits->det_ind};
-        for (int k = 0; k < 6; ++k)
-            if (n) std::memcpy(s.h_in + L.f[k], fsrc[k], 4 * n);
-        for (int k = 0; k < 3; ++k)
-            if (n) std::memcpy(s.h_in + L.i16[k], isrc[k], 2 * n);
-        if (hits->hist_off) std::memcpy(s.h_in + L.hist_off, hits->hist_off, 4 * PT_HIST_NOFF);
-        if (hits->hist_bins && n) std::memcpy(s.h_in + L.hist_bins, hits->hist_bins, 2 * n);
-        HIP_OK(hipMemcpyAsync(s.in, s.h_in, L.total, hipMemcpyHostToDevice, st));
-    }
-    pt_hits_view dv = *hits;
-    dv.xg = (const float *)(s.in + L.f[0]), dv.yg = (const float *)(s.in + L.f[1]);
-    dv.zg = (const float *)(s.in + L.f[2]), dv.rg = (const float *)(s.in + L.f[3]);
-    dv.xerr = (const float *)(s.in + L.f[4]), dv.yerr = (const float *)(s.in + L.f[5]);
-    dv.iphi = (const int16_t *)(s.in + L.i16[0]), dv.ysize = (const int16_t *)(s.in + L.i16[1]);
-    dv.det_ind = (const int16_t *)(s.in + L.i16[2]);
-    dv.hist_off = hits->hist_off ? (const uint32_t *)(s.in + L.hist_off) : nullptr;  // none given: built on the device
-    dv.hist_bins = hits->hist_bins ? (const uint16_t *)(s.in + L.hist_bins) : nullptr;
+    const PackedInputs L(hits->n_hits);
+    pack_inputs(s.h_in, L, hits);
+    HIP_OK(hipMemcpyAsync(s.in, s.h_in, L.total, hipMemcpyHostToDevice, st));
+    const pt_hits_view dv = packed_view(s.in, L, hits);
     pt_track_soa dout;
     dout.quality = s.quality, dout.chi2 = s.chi2, dout.eta = s.eta, dout.pt = s.pt, dout.state = s.state;
     dout.covariance = s.cov, dout.hit_off = s.hit_off, dout.hit_bins = s.hit_bins, dout.det_off = s.det_off;
@@ -740,57 +815,282 @@ int pt_make_tuples(pt_ctx *c, const pt_hits_view *hits, double b_field, pt_track
     if (rc) return rc;
     const pt_counters &k = c->h_cnt[0];
     const size_t T = PT_MAX_TUPLES;
-    const size_t m = std::min<size_t>(k.n_tuples, T);
-    const size_t nb = std::min<size_t>(k.n_tuple_hits, PT_MAX_TUPLE_HITS);
-    // offsets: past the stored tuples they all equal off[m] (bulk_finalize_fill),
-    // unless the container is full; the copy stops there and the host fills the rest
-    const size_t n_off = (k.n_tuples < T) ? m + 1 : T + 1;
-    // the defined rows into the pinned block (state and covariance: only rows of
-    // stored tuples can be non-zero), then into the caller's arrays
-    size_t po = 0;
+    const OutLayout O(k);
+    const size_t m = O.m;
+    char *blk = s.h_out;
     hipError_t copy_err = hipSuccess;
-    auto grab = [&](const void *src, size_t bytes) -> char * {
-        char *dst = s.h_out + po;
-        po += (bytes + 15) & ~size_t(15);
-        if (bytes) {
-            const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
-            if (e != hipSuccess) copy_err = e;
-        }
-        return dst;
+    auto grab = [&](size_t off, const void *src, size_t bytes) {
+        if (!bytes) return;
+        const hipError_t e = hipMemcpyAsync(blk + off, src, bytes, hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) copy_err = e;
     };
-    char *p_q = grab(s.quality, m * 4), *p_chi2 = grab(s.chi2, m * 4), *p_eta = grab(s.eta, m * 4);
-    char *p_pt = grab(s.pt, m * 4);
-    char *p_st = s.h_out + po;
-    po += ((5 * m * 4) + 15) & ~size_t(15);
-    char *p_cv = s.h_out + po;
-    po += ((15 * m * 4) + 15) & ~size_t(15);
+    grab(O.q, s.quality, m * 4), grab(O.chi2, s.chi2, m * 4), grab(O.eta, s.eta, m * 4), grab(O.pt, s.pt, m * 4);
     if (m) {
-        HIP_OK(hipMemcpy2DAsync(p_st, m * 4, s.state, T * 4, m * 4, 5, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpy2DAsync(p_cv, m * 4, s.cov, T * 4, m * 4, 15, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpy2DAsync(blk + O.st, m * 4, s.state, T * 4, m * 4, 5, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpy2DAsync(blk + O.cv, m * 4, s.cov, T * 4, m * 4, 15, hipMemcpyDeviceToHost, st));
     }
-    char *p_ho = grab(s.hit_off, n_off * 4), *p_do = grab(s.det_off, n_off * 4);
-    char *p_hb = grab(s.hit_bins, nb * 2), *p_db = grab(s.det_bins, nb * 2);
+    grab(O.ho, s.hit_off, O.n_off * 4), grab(O.dof, s.det_off, O.n_off * 4);
+    grab(O.hb, s.hit_bins, O.nb * 2), grab(O.db, s.det_bins, O.nb * 2);
     HIP_OK(copy_err);
     HIP_OK(hipStreamSynchronize(st));
-    std::memcpy(out->quality, p_q, m * 4);
-    std::memcpy(out->chi2, p_chi2, m * 4);
-    std::memcpy(out->eta, p_eta, m * 4);
-    std::memcpy(out->pt, p_pt, m * 4);
-    for (int comp = 0; comp < 20; ++comp) {
-        float *col = (comp < 5 ? out->state + comp * T : out->covariance + (comp - 5) * T);
-        const char *src = comp < 5 ? p_st + comp * m * 4 : p_cv + (comp - 5) * m * 4;
-        std::memcpy(col, src, m * 4);
-        if (!out_zeroed) std::memset(col + m, 0, (T - m) * 4);
-    }
-    std::memcpy(out->hit_off, p_ho, n_off * 4);
-    std::memcpy(out->det_off, p_do, n_off * 4);
-    std::memcpy(out->hit_bins, p_hb, nb * 2);
-    std::memcpy(out->det_bins, p_db, nb * 2);
-    for (size_t i = n_off; i <= T; ++i) {
-        out->hit_off[i] = out->hit_off[m];
-        out->det_off[i] = out->det_off[m];
-    }
+    unpack_outputs(blk, O, out, out_zeroed);
     if (cnt) *cnt = k;
+    return PT_OK;
+}
+
+// ---- single events from many host threads in shared launches (pt_batcher_*)
+// kBatchRuns batch runs rotate, up to kBatchOnGpu of them on the GPU at once
+// (each on its own context and stream; a small batch's kernels are latency-
+// bound, so runs side by side fill the GPU); callers join an open run.  The
+// first caller of a run leads it: once a GPU place is free and every caller
+// that joined has packed its inputs, it sends the run's inputs with one copy,
+// runs the stage for all its events, gathers their defined TrackSOA rows into
+// one block on the device and brings it back with one copy.  Every caller then
+// unpacks its own event; the run is free again when the last one has.
+struct BatchReq {
+    const pt_hits_view *hits;
+    pt_track_soa *out;
+    int out_zeroed;
+    size_t in_off;
+    int rc;
+    std::string err;
+    pt_counters cnt;
+};
+
+struct BatchRun {
+    enum State { kFree, kOpen, kRunning, kDone };
+    State state = kFree;
+    pt_ctx *ctx = nullptr;
+    double b_field = 0.0;
+    char *h_in = nullptr, *d_in = nullptr;  // inputs, packed back to back (pinned / device)
+    size_t in_cap = 0, in_used = 0;
+    char *d_out = nullptr;  // one full-capacity TrackSOA per event
+    size_t out_stride = 0;
+    char *h_pack = nullptr, *d_pack = nullptr;  // gathered defined rows
+    size_t pack_cap = 0;
+    pt::PackSeg *h_seg = nullptr, *d_seg = nullptr;
+    std::vector<BatchReq *> reqs;
+    std::vector<pt_hits_view> views;
+    std::vector<pt_track_soa> outs;
+    std::vector<size_t> pack_off;
+    uint32_t n_packed = 0, n_unpacked = 0;
+};
+
+}  // extern "C"
+
+constexpr int kBatchRuns = 4, kBatchOnGpu = 3;
+
+struct pt_batcher {
+    int device = 0;
+    uint32_t max_batch = 0;
+    BatchRun run[kBatchRuns];
+    std::mutex mu;
+    std::condition_variable cv;
+    int on_gpu = 0;  // runs between launch and readback
+};
+
+extern "C" {
+
+constexpr uint32_t kSegsPerEvent = 28;  // quality, chi2, eta, pt, 5 + 15 columns, 2 offsets, 2 id arrays
+
+int pt_batcher_destroy(pt_batcher *b) {
+    if (!b) return PT_OK;
+    (void)hipSetDevice(b->device);
+    for (BatchRun &r : b->run) {
+        if (r.ctx) pt_ctx_destroy(r.ctx);
+        if (r.h_in) (void)hipHostFree(r.h_in);
+        if (r.d_in) (void)hipFree(r.d_in);
+        if (r.d_out) (void)hipFree(r.d_out);
+        if (r.h_pack) (void)hipHostFree(r.h_pack);
+        if (r.d_pack) (void)hipFree(r.d_pack);
+        if (r.h_seg) (void)hipHostFree(r.h_seg);
+        if (r.d_seg) (void)hipFree(r.d_seg);
+    }
+    delete b;
+    return PT_OK;
+}
+
+static int batcher_alloc(pt_batcher *b, const pt_params *p, const float *rot, uint32_t n_modules) {
+    const uint32_t n = b->max_batch;
+    pt_counters full;
+    std::memset(&full, 0, sizeof(full));
+    full.n_tuples = PT_MAX_TUPLES + 1u, full.n_tuple_hits = PT_MAX_TUPLE_HITS;
+    for (BatchRun &r : b->run) {
+        int rc = pt_ctx_create(&r.ctx, p, b->device, n);
+        if (rc) return rc;
+        rc = pt_ctx_set_frames(r.ctx, rot, n_modules);
+        if (rc) return rc;
+        r.in_cap = (size_t)n * align256(PackedInputs(PT_MAX_HITS).total);
+        r.out_stride = align256(carve_track_soa(nullptr, nullptr));
+        r.pack_cap = (size_t)n * OutLayout(full).total;
+        HIP_OK(hipHostMalloc((void **)&r.h_in, r.in_cap, hipHostMallocDefault));
+        HIP_OK(hipMalloc((void **)&r.d_in, r.in_cap));
+        HIP_OK(hipMalloc((void **)&r.d_out, (size_t)n * r.out_stride));
+        HIP_OK(hipHostMalloc((void **)&r.h_pack, r.pack_cap, hipHostMallocDefault));
+        HIP_OK(hipMalloc((void **)&r.d_pack, r.pack_cap));
+        HIP_OK(hipHostMalloc((void **)&r.h_seg, (size_t)n * kSegsPerEvent * sizeof(pt::PackSeg), hipHostMallocDefault));
+        HIP_OK(hipMalloc((void **)&r.d_seg, (size_t)n * kSegsPerEvent * sizeof(pt::PackSeg)));
+        r.reqs.reserve(n);
+        r.views.resize(n);
+        r.outs.resize(n);
+        r.pack_off.resize(n);
+    }
+    return PT_OK;
+}
+
+int pt_batcher_create(pt_batcher **out, const pt_params *p, int device, uint32_t max_batch, const float *rot,
+                      uint32_t n_modules) {
+    if (!out || !p || !rot) return fail(PT_ERR_ARG, "null pointer");
+    *out = nullptr;
+    if (max_batch < 1 || max_batch > 64) return fail(PT_ERR_ARG, "max_batch %u outside 1..64", max_batch);
+    pt_batcher *b = new pt_batcher();
+    b->device = device;
+    b->max_batch = max_batch;
+    const int rc = batcher_alloc(b, p, rot, n_modules);
+    if (rc) {
+        const std::string err = g_err;
+        pt_batcher_destroy(b);
+        return fail(rc, "%s", err.c_str());
+    }
+    *out = b;
+    return PT_OK;
+}
+
+// The GPU part of a run, on its context's stream (the leader's thread).
+static void batch_gpu(pt_batcher *b, BatchRun &R) {
+    const uint32_t n = (uint32_t)R.reqs.size();
+    auto fail_all = [&](int rc) {
+        for (BatchReq *q : R.reqs)
+            if (!q->rc) q->rc = rc, q->err = g_err;
+    };
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        fail(PT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+        fail_all(PT_ERR_HIP);
+        return false;
+    };
+    if (!hip_ok(hipSetDevice(b->device), "hipSetDevice")) return;
+    pt_ctx *c = R.ctx;
+    hipStream_t st = c->stream;
+    for (uint32_t k = 0; k < n; ++k) {
+        const pt_hits_view *h = R.reqs[k]->hits;
+        R.views[k] = packed_view(R.d_in + R.reqs[k]->in_off, PackedInputs(h->n_hits), h);
+        carve_track_soa(R.d_out + (size_t)k * R.out_stride, &R.outs[k]);
+    }
+    if (!hip_ok(hipMemcpyAsync(R.d_in, R.h_in, R.in_used, hipMemcpyHostToDevice, st), "input copy")) return;
+    const int rc = run_batch(c, n, R.views.data(), R.outs.data(), R.b_field, 0);
+    if (rc) {
+        fail_all(rc);
+        return;
+    }
+    if (!hip_ok(hipStreamSynchronize(st), "stage")) return;
+    const size_t T = PT_MAX_TUPLES;
+    size_t off = 0;
+    uint32_t ns = 0;
+    auto seg = [&](const void *src, size_t dst_off, size_t bytes) {
+        if (bytes) R.h_seg[ns++] = pt::PackSeg{(const uint16_t *)src, dst_off, (uint32_t)(bytes / 2), 0u};
+    };
+    for (uint32_t k = 0; k < n; ++k) {
+        BatchReq &q = *R.reqs[k];
+        q.cnt = c->h_cnt[k];
+        if (q.cnt.internal_error) {
+            q.rc = PT_ERR_CAPACITY;
+            char buf[96];
+            snprintf(buf, sizeof(buf), "internal limit hit (PT_INT_* bits 0x%x)", q.cnt.internal_error);
+            q.err = buf;
+            continue;
+        }
+        const OutLayout O(q.cnt);
+        const pt_track_soa &o = R.outs[k];
+        R.pack_off[k] = off;
+        seg(o.quality, off + O.q, O.m * 4), seg(o.chi2, off + O.chi2, O.m * 4);
+        seg(o.eta, off + O.eta, O.m * 4), seg(o.pt, off + O.pt, O.m * 4);
+        for (size_t col = 0; col < 5; ++col) seg(o.state + col * T, off + O.st + col * O.m * 4, O.m * 4);
+        for (size_t col = 0; col < 15; ++col) seg(o.covariance + col * T, off + O.cv + col * O.m * 4, O.m * 4);
+        seg(o.hit_off, off + O.ho, O.n_off * 4), seg(o.det_off, off + O.dof, O.n_off * 4);
+        seg(o.hit_bins, off + O.hb, O.nb * 2), seg(o.det_bins, off + O.db, O.nb * 2);
+        off += O.total;
+    }
+    if (!hip_ok(hipMemcpyAsync(R.d_seg, R.h_seg, ns * sizeof(pt::PackSeg), hipMemcpyHostToDevice, st), "segments"))
+        return;
+    pt::launch_pack_segments(R.d_seg, ns, R.d_pack, st);
+    if (!hip_ok(hipGetLastError(), "gather")) return;
+    if (off && !hip_ok(hipMemcpyAsync(R.h_pack, R.d_pack, off, hipMemcpyDeviceToHost, st), "output copy")) return;
+    hip_ok(hipStreamSynchronize(st), "output copy");
+}
+
+int pt_batcher_make_tuples(pt_batcher *b, const pt_hits_view *hits, double b_field, pt_track_soa *out,
+                           int out_zeroed, pt_counters *cnt) {
+    if (!b || !hits || !out) return fail(PT_ERR_ARG, "null pointer");
+    int rc = check_host_args(hits, out);
+    if (rc) return rc;
+    const PackedInputs L(hits->n_hits);
+    const size_t need = align256(L.total);
+    BatchReq req{hits, out, out_zeroed, 0, PT_OK, std::string(), pt_counters()};
+    BatchRun *R = nullptr;
+    bool leader = false;
+    {
+        std::unique_lock<std::mutex> lk(b->mu);
+        for (;;) {
+            for (BatchRun &r : b->run)
+                if (r.state == BatchRun::kOpen && r.b_field == b_field && r.reqs.size() < b->max_batch &&
+                    r.in_used + need <= r.in_cap) {
+                    R = &r;
+                    break;
+                }
+            if (!R)
+                for (BatchRun &r : b->run)
+                    if (r.state == BatchRun::kFree) {
+                        R = &r;
+                        R->state = BatchRun::kOpen;
+                        R->b_field = b_field;
+                        leader = true;
+                        break;
+                    }
+            if (R) break;
+            b->cv.wait(lk);
+        }
+        req.in_off = R->in_used;
+        R->in_used += need;
+        R->reqs.push_back(&req);
+    }
+    pack_inputs(R->h_in + req.in_off, L, hits);
+    {
+        std::unique_lock<std::mutex> lk(b->mu);
+        ++R->n_packed;
+        b->cv.notify_all();
+        if (leader) {
+            // the run starts once a GPU place is free and every caller that
+            // joined has packed; callers join until then
+            b->cv.wait(lk, [&] { return b->on_gpu < kBatchOnGpu && R->n_packed == R->reqs.size(); });
+            R->state = BatchRun::kRunning;
+            ++b->on_gpu;
+            lk.unlock();
+            batch_gpu(b, *R);
+            lk.lock();
+            --b->on_gpu;
+            R->state = BatchRun::kDone;
+            b->cv.notify_all();
+        } else {
+            b->cv.wait(lk, [&] { return R->state == BatchRun::kDone; });
+        }
+    }
+    if (req.rc == PT_OK) {
+        const size_t k = (size_t)(std::find(R->reqs.begin(), R->reqs.end(), &req) - R->reqs.begin());
+        unpack_outputs(R->h_pack + R->pack_off[k], OutLayout(req.cnt), out, out_zeroed);
+        if (cnt) *cnt = req.cnt;
+    }
+    {
+        std::unique_lock<std::mutex> lk(b->mu);
+        if (++R->n_unpacked == R->reqs.size()) {
+            R->reqs.clear();
+            R->in_used = 0;
+            R->n_packed = R->n_unpacked = 0;
+            R->state = BatchRun::kFree;
+            b->cv.notify_all();
+        }
+    }
+    if (req.rc) return fail(req.rc, "%s", req.err.c_str());
     return PT_OK;
 }
 

@@ -108,6 +108,14 @@ struct Scratch {
 
 constexpr uint8_t kUsed = 1, kKilled = 2;
 
+// One contiguous piece of a device array copied into a readback block
+// (launch_pack_segments): n16 16-bit words from src to dst + dst_off.
+struct PackSeg {
+    const uint16_t *src;
+    uint64_t dst_off;
+    uint32_t n16, pad;
+};
+
 enum Misc : int {
     kMiscStart = 0,       // entries of start_list
     kMiscExt = 1,         // entries of ext_list

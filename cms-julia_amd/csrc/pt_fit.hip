@@ -449,6 +449,22 @@ void launch_fill_bytes(void *dst, uint8_t value, size_t bytes, hipStream_t st) {
     k_fill_bytes<<<(unsigned)blocks, 256, 0, st>>>((uint8_t *)dst, value, bytes);
 }
 
+// Readback gather: block per segment (the defined rows of the events of a
+// batch, packed for one device-to-host copy).  Whole 32-bit words when the
+// segment's ends allow, the odd 16-bit word last.
+__global__ void __launch_bounds__(256) k_pack_segments(const PackSeg *__restrict__ segs, char *dst) {
+    const PackSeg sg = segs[blockIdx.x];
+    uint16_t *d = reinterpret_cast<uint16_t *>(dst + sg.dst_off);
+    const bool words = ((reinterpret_cast<uintptr_t>(sg.src) | reinterpret_cast<uintptr_t>(d)) & 3u) == 0u;
+    const uint32_t n32 = words ? sg.n16 / 2u : 0u;
+    for (uint32_t i = threadIdx.x; i < n32; i += blockDim.x)
+        reinterpret_cast<uint32_t *>(d)[i] = reinterpret_cast<const uint32_t *>(sg.src)[i];
+    for (uint32_t i = 2u * n32 + threadIdx.x; i < sg.n16; i += blockDim.x) d[i] = sg.src[i];
+}
+void launch_pack_segments(const PackSeg *d_segs, uint32_t n_segs, char *d_dst, hipStream_t st) {
+    if (n_segs) k_pack_segments<<<n_segs, 256, 0, st>>>(d_segs, d_dst);
+}
+
 template <class J>
 static void launch_job(const J &j, uint32_t n, const FitLaunch &L, hipStream_t st) {
     if (n == 0) return;

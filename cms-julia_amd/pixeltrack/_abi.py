@@ -210,7 +210,8 @@ HIP_SYMBOLS = [
     "pt_pipeline_submit", "pt_pipeline_wait", "pt_pipeline_get_counters", "pt_fill_phi_histogram_async",
     "pt_set_hits_layer_start", "pt_load_tracks_async", "pt_fit_bl_sched_noalloc_async",
     "pt_fit_set_schedule", "pt_ca_set_tuning", "pt_ca_get_tuning", "pt_ca_autotune",
-    "pt_fit_riemann_device_async", "pt_fit_riemann_sched_async",
+    "pt_fit_riemann_device_async", "pt_fit_riemann_sched_async", "pt_batcher_create", "pt_batcher_destroy",
+    "pt_batcher_make_tuples",
 ]
 GEN_SYMBOLS = [
     "ptg_config_default", "ptg_generate_event", "ptg_free_event", "ptg_frames", "ptg_build_phi_hist",
@@ -252,6 +253,11 @@ def hip_lib():
                                        C.POINTER(PtCounters)]
         lib.pt_make_tuples_device_async.argtypes = [vp, C.c_uint32, C.POINTER(PtHitsView), C.c_double,
                                                     C.POINTER(PtTrackSoA), C.c_int]
+        lib.pt_batcher_create.argtypes = [C.POINTER(vp), C.POINTER(PtParams), C.c_int, C.c_uint32, c_f32p,
+                                          C.c_uint32]
+        lib.pt_batcher_destroy.argtypes = [vp]
+        lib.pt_batcher_make_tuples.argtypes = [vp, C.POINTER(PtHitsView), C.c_double, C.POINTER(PtTrackSoA),
+                                               C.c_int, C.POINTER(PtCounters)]
         lib.pt_ctx_sync.argtypes = [vp]
         lib.pt_get_counters.argtypes = [vp, C.c_uint32, C.POINTER(PtCounters)]
         lib.pt_get_ca_debug.argtypes = [vp, C.c_uint32, C.POINTER(PtCaDebug)]

@@ -190,6 +190,44 @@ class CAHitNtupletGeneratorOnGPU:
         return out
 
 
+class Batcher:
+    """make_tuples shared by many host threads (pt_batcher_*): each call has the
+    contract of CAHitNtupletGeneratorOnGPU.make_tuples, and calls made while the
+    GPU runs a batch form the next one (up to max_batch events)."""
+
+    def __init__(self, frames, params=None, device=0, max_batch=16):
+        self.lib = A.hip_lib()
+        self.params = params if params is not None else default_params()
+        self._frames = np.ascontiguousarray(frames, np.float32).reshape(-1)
+        h = C.c_void_p()
+        A.check(self.lib.pt_batcher_create(C.byref(h), C.byref(self.params), device, max_batch,
+                                           A.ptr(self._frames, C.c_float), A.N_MODULES), "pt_batcher_create")
+        self.h = h
+
+    def make_tuples(self, hits: HitsSoA, b_field=A.B_FIELD_INV_GEV, out=None, out_zeroed=True):
+        """Returns (TrackSOA, counters).  out: a TrackSOA to fill instead of a new
+        one (out_zeroed: whether its state and covariance are already zero)."""
+        if out is None:
+            out, out_zeroed = TrackSOA(), True
+        view = hits.view()
+        tv = out.view()
+        cnt = A.PtCounters()
+        A.check(self.lib.pt_batcher_make_tuples(self.h, C.byref(view), b_field, C.byref(tv), 1 if out_zeroed else 0,
+                                                C.byref(cnt)), "pt_batcher_make_tuples")
+        return out, cnt.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pt_batcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class CAHitNtuplet:
     """EDProducer mirror (ca_hit_ntuplet.jl:1-19): consumes hits, produces TrackSOA.
 

@@ -145,7 +145,10 @@ typedef struct pt_hits_view {
 /* Output: TrackSOA = TrackSOAT{32768,163840} (PixelTrackHeterogeneous.jl:10-33,59).
  * state and covariance are column-major (one column of 32768 per component,
  * TrajectoryStateSOA.jl:8-19).  Fields the reference leaves uninitialised
- * (quality of absent tuples, pt/eta/chi2 of unfitted tuples) are not written.
+ * (quality of absent tuples, pt/eta/chi2 of unfitted tuples) are not written by
+ * the stage; the host-buffer entry points copy the stored tuples' rows of
+ * chi2, eta and pt back whole, so there the unfitted ones hold whatever the
+ * library's device buffers held.
  * hit_bins and det_bins are defined up to hit_off[32768] - 1 (the container);
  * past it they hold whatever an earlier event left there, as the reference's
  * reused buffers do. */
@@ -230,6 +233,24 @@ int pt_make_tuples(pt_ctx *ctx, const pt_hits_view *hits, double b_field,
  * Call pt_ctx_sync, then pt_get_counters. */
 int pt_make_tuples_device_async(pt_ctx *ctx, uint32_t n_events, const pt_hits_view *hits,
                                 double b_field, const pt_track_soa *out, int out_zeroed);
+
+/* make_tuples for host tasks that share one batcher instead of holding a
+ * context each (CAHitNtuplet.produce, ca_hit_ntuplet.jl:10-16, which the
+ * reference's tasks call with per-task scratch, ca_hit_ntuplet_generator.jl:9).
+ * pt_batcher_make_tuples has the contract of pt_make_tuples and may be called
+ * from any number of threads at once: calls that arrive while the GPU runs a
+ * batch form the next one (up to max_batch events, 1..64, with the same
+ * b_field), which runs as one launch sequence with one copy each way.
+ * Up to three batches run on the GPU side by side.  pt_batcher_destroy must
+ * not overlap the calls.  Holds four contexts of max_batch event slots and,
+ * per slot, about 1.6 MB of pinned input staging and 4.1 MB of pinned
+ * readback, four times. */
+typedef struct pt_batcher pt_batcher;
+int pt_batcher_create(pt_batcher **out, const pt_params *p, int device, uint32_t max_batch, const float *rot,
+                      uint32_t n_modules);
+int pt_batcher_destroy(pt_batcher *b);
+int pt_batcher_make_tuples(pt_batcher *b, const pt_hits_view *hits, double b_field, pt_track_soa *out,
+                           int out_zeroed, pt_counters *cnt);
 int pt_ctx_sync(pt_ctx *ctx);
 int pt_get_counters(pt_ctx *ctx, uint32_t event, pt_counters *cnt);
 
