@@ -113,8 +113,11 @@ def profiled_traffic(kernel, path=ROOT / "profiles" / "hbm_traffic_pu50_batch100
     try:
         for line in path.read_text().splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 4 and cells[0].strip("`").split("::")[-1] == kernel:
-                return int(float(cells[3]) * 1e6)
+            if len(cells) < 4 or "`" not in cells[0]:
+                continue
+            name = cells[0].split("`")[1].split("<")[0].split("::")[-1]  # template arguments dropped
+            if name == kernel:
+                return int(float(cells[-1].replace(",", "")) * 1e6)
     except (OSError, ValueError):
         pass
     return None

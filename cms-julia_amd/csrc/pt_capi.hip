@@ -702,9 +702,15 @@ static int ensure_staging(pt_ctx *c) {
 }
 
 // ---- the host-buffer entry points (pt_make_tuples, pt_batcher_make_tuples): shared pieces
+// Everything build_desc would reject, checked before a call joins a batch, so
+// that a bad call fails alone.
 static int check_host_args(const pt_hits_view *hits, const pt_track_soa *out) {
     if (hits->n_hits > PT_MAX_HITS)
         return fail(PT_ERR_CAPACITY, "%u hits above MAX_NUMBER_OF_HITS = %u", hits->n_hits, PT_MAX_HITS);
+    if (hits->layer_start[0] != 0 || hits->layer_start[10] != hits->n_hits)
+        return fail(PT_ERR_ARG, "layer_start must start at 0 and end at n_hits");
+    for (int L = 0; L < 10; ++L)
+        if (hits->layer_start[L + 1] < hits->layer_start[L]) return fail(PT_ERR_ARG, "layer_start not sorted");
     if (hits->n_hits && (!hits->xg || !hits->yg || !hits->zg || !hits->rg || !hits->xerr || !hits->yerr ||
                          !hits->iphi || !hits->ysize || !hits->det_ind))
         return fail(PT_ERR_ARG, "null hit array");

@@ -152,3 +152,35 @@ def test_batcher_rejects_bad_calls(frames, events):
         assert cnt["n_tuples"] > 0 and out.hit_off[1] > 1
     finally:
         b.close()
+
+
+def test_batcher_bad_call_fails_alone(frames, events, reference):
+    """A call with unsorted layer starts, sent among good calls from other
+    threads, fails by itself: the good calls that share its batches come back
+    as pt_make_tuples returns them."""
+    bad = G.generate_event("single_event", seed=2)
+    ls = np.array(bad.layer_start, dtype=np.uint32)
+    ls[3], ls[4] = ls[4] + 1, ls[3]  # not sorted
+    bad.layer_start = ls
+    b = ca.Batcher(frames, max_batch=8)
+    n = 8
+    got, raised = [None] * n, []
+    print("[batcher] one bad call among 8 good ones, 5 threads", flush=True)
+    try:
+        def work(t):
+            if t == 4:
+                for _ in range(3):
+                    try:
+                        b.make_tuples(bad)
+                    except RuntimeError as err:
+                        raised.append(str(err))
+                return
+            for i in range(t, n, 4):
+                got[i] = b.make_tuples(events[i])
+        run_threads(5, work)
+    finally:
+        b.close()
+    assert len(raised) == 3 and all("layer_start" in m for m in raised), raised
+    for i in range(n):
+        out, cnt = got[i]
+        assert same(out, reference[i][0], cnt, reference[i][1]) is None, i
