@@ -1086,8 +1086,9 @@ __device__ __forceinline__ bool start_pair(const Config &cfg, int p) {
 // then every lane takes one row and walks its hit's staged columns kConnCols
 // at a time, keeping the passing ones in order - the serial push! order of the
 // row cell's neighbour list.  The alignment cut runs in every lane; the dca
-// cut (about 2% of the pairs get there) runs on the survivors of a whole round
-// of columns, compacted through LDS.  Each mark of a used cell is counted per
+// cut (about 2% of the pairs get there) runs in place when no lane has more
+// than one survivor in the round, else on the survivors of the whole round,
+// compacted through LDS.  Each mark of a used cell is counted per
 // outer hit for the fishbone's work list.
 // H hits per ticket (4, 8, 16, 24 or 32) and ST columns staged at a time (128,
 // 192, 256 or 384) are tuned: denser events favour fewer hits per ticket.
@@ -1269,14 +1270,33 @@ __global__ void __launch_bounds__(256, 5) k_connect(const EvDesc *__restrict__ e
                                 if ((open >> k) & 1u)
                                     al[k] = are_aligned(r1, z1, ri, zi, o3[k].y, o3[k].x, cfg.p.pt_min, th);
                         }
+                        uint32_t am4 = 0;  // this lane's aligned columns of the round, bit k
 #pragma unroll
-                        for (int k = 0; k < kConnCols; ++k) {
-                            const uint64_t am = __ballot(al[k]);
-                            qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
-                            nq += (uint32_t)__popcll(am);
-                        }
+                        for (int k = 0; k < kConnCols; ++k) am4 |= al[k] ? 1u << k : 0u;
                         uint32_t pm = 0;  // this lane's passing columns of the round, bit k
-                        // survivors in (k, lane) order, the dca cut 64 at a time
+                        const bool many = __ballot((am4 & (am4 - 1u)) != 0u) != 0ull;
+                        if (!many && __ballot(am4 != 0u)) {
+                            // at most one aligned column per lane (the common case): the dca
+                            // cut in place, on the same values the compacted path reads
+                            if (am4) {
+                                const int k1 = __builtin_ctz(am4);
+                                uint32_t s1 = sl[0];
+#pragma unroll
+                                for (int k = 1; k < kConnCols; ++k) s1 = (k1 == k) ? sl[k] : s1;
+                                const float2 o = S.cxy[s1];
+                                if (dca_cut(x1, y1, S.hx[ohl], S.hy[ohl], o.x, o.y, reg, cfg.p.hard_curv_cut))
+                                    pm = am4;
+                            }
+                        }
+                        if (many) {
+#pragma unroll
+                            for (int k = 0; k < kConnCols; ++k) {
+                                const uint64_t am = __ballot(al[k]);
+                                qi[k] = nq + (uint32_t)__popcll(am & lanes_below());
+                                nq += (uint32_t)__popcll(am);
+                            }
+                        }
+                        // otherwise the survivors in (k, lane) order, the dca cut 64 at a time
                         for (uint32_t qb = 0; qb < nq; qb += 64u) {
 #pragma unroll
                             for (int k = 0; k < kConnCols; ++k)
