@@ -494,7 +494,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the pixel-track stage shards by event with no exchange; "
-              f"--gpus {args.gpus} is not supported in this build", file=sys.stderr)
+              f"--gpus {args.gpus} is not supported (run one process per GPU, each on every N-th event)",
+              file=sys.stderr)
         sys.exit(2)
 
     from pixeltrack import ca, generator as G
